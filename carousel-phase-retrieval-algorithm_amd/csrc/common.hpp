@@ -1,0 +1,123 @@
+// common.hpp -- handle state, error plumbing and launch helpers shared by the
+// translation units of libcpra_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <utility>
+
+#include "../../include/cpra_hip.h"
+#include "../../include/cpra_hip_engine.h"
+#include "fft_core.hpp"
+
+namespace cpra {
+
+// Error carrying a C-ABI status; thrown inside the library, converted to a
+// status code at the extern "C" boundary.
+struct Status {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw Status{code, buf};
+}
+
+#define CPRA_CHECK_HIP(expr)                                                                 \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            ::cpra::fail(e_ == hipErrorOutOfMemory ? CPRA_HIP_ERR_NOMEM : CPRA_HIP_ERR_HIP, \
+                         "%s failed: %s", #expr, hipGetErrorString(e_));                     \
+    } while (0)
+
+#define CPRA_ARG(cond, ...)                                   \
+    do {                                                      \
+        if (!(cond)) ::cpra::fail(CPRA_HIP_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+// Device workspace owned by a handle; grows on demand, freed at destroy.
+struct Buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t want)
+    {
+        if (want <= bytes) return;
+        if (ptr) CPRA_CHECK_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        CPRA_CHECK_HIP(hipMalloc(&ptr, want));
+        bytes = want;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+struct EngineGraph;  // engine2d.hip
+
+}  // namespace cpra
+
+struct cpra_hip_ctx {
+    int dtype = CPRA_HIP_F32;
+    uint64_t m = 1, n = 1, l = 1, batch = 1;
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipStream_t aux_stream = nullptr;  // graph capture / replay
+    std::string last_error;
+    // twiddle tables keyed by (n << 1) | is_double
+    std::map<uint64_t, void*> twiddles;
+    // engine workspaces
+    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc;
+    // tuning choice per (kind, n) set by the autotuner (0 = default)
+    std::map<uint64_t, int> tuning;
+    cpra::EngineGraph* graph = nullptr;
+};
+
+namespace cpra {
+
+size_t dtype_size(int dtype);
+// device twiddle table exp(-2 pi i k / n), k < n, in the handle's precision
+// (or the requested one), cached per handle.
+const void* twiddle_table(cpra_hip_ctx* h, int n, bool dbl);
+
+inline unsigned grid_1d(uint64_t work, unsigned block, unsigned cap = 1u << 20)
+{
+    uint64_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+inline void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(CPRA_HIP_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+}
+
+// FFT pass entry points (fft.hip).  `lines` transforms of length n; contig:
+// line j starts at j*n; strided: line j starts at (j / G)*GS + (j % G), element
+// stride es.  shift: fold the centring shift S into the index maps.
+void fft_pass_contig(cpra_hip_ctx* h, void* data, int n, int64_t lines, int dir, bool shift,
+                     double scale, bool dbl);
+void fft_pass_strided(cpra_hip_ctx* h, void* data, int n, int64_t lines, int64_t es, int64_t G,
+                      int64_t GS, int dir, bool shift, double scale, bool dbl);
+bool fft_fast_size(int n);
+// frees the captured engine graphs of a handle (engine2d.hip)
+void cpra_engine_release(cpra_hip_ctx* h);
+
+}  // namespace cpra

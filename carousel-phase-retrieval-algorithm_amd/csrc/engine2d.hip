@@ -1,0 +1,705 @@
+// engine2d.hip -- fused batched 2D phase-retrieval iterations (FP32).
+//
+// State layout.  The iterate x [B][M][N] is kept between iterations in the
+// row-spectral domain Y = S_r DFT_r S_r x (centred DFT along rows only), in
+// the caller's x buffer.  One HIO iteration is two HBM passes:
+//
+//   column kernel  Y --(centred column DFT)--> X = Forward2D(x)
+//                    --(modulus projection with A, scale 1/(MN))-->
+//                    --(inverse column DFT)--> Z            (Z workspace)
+//   row kernel     x' = IDFT_r(Z), x = IDFT_r(Y)/N, HIO update with the
+//                  support mask, Y <- DFT_r(x_new)           (in place)
+//
+// so the compulsory traffic per pixel and iteration is 20 B (column: read Y,
+// read A, write Z) + 25 B (row: read Z, read Y, read mask, write Y) instead of
+// the ~50 separate full-array passes of the reference's primitive sequence
+// (src/CPRA_MKL.cpp:94-110 over include/detail/MKL_Impl.inl).  All centring
+// shifts are register renamings (fft_core.hpp).
+//
+// Shrinkwrap (north-star addition) reuses the same passes with different
+// epilogues: |x| -> row DFT; column DFT, x Gaussian transfer, column IDFT;
+// row IDFT -> blurred image + per-pattern max; threshold -> mask.
+#include <cmath>
+#include <vector>
+
+#include "../../include/cpra_hip_engine.h"
+#include "common.hpp"
+#include "plans.hpp"
+
+namespace cpra {
+
+// Float <-> order-preserving unsigned key for atomicMax on floats.
+__device__ __forceinline__ unsigned f2key(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(unsigned k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+enum ColMode { COL_MOD = 0, COL_MOD_RF = 1, COL_BLUR = 2 };
+
+// Column kernel.  Block = C consecutive columns of one pattern (N % C == 0).
+// Y and Z may alias (each block reads its columns fully before writing).
+template <int M, int MODE>
+__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
+    const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
+    float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<M>::P;
+    constexpr int E = PL::E, TT = PL::T, C = SizeCfg<M>::C;
+    __shared__ V lds[M * C];
+    __shared__ float red[2][SizeCfg<M>::TPB];
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const int64_t line = (int64_t)blockIdx.x * C + c;
+    const int64_t b = line / N;
+    const int col = (int)(line % N);
+    const int64_t base = b * (int64_t)M * N + col;
+    V v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = Y[base + (int64_t)(t + PL::rot(m) * TT) * N];
+    });
+    LdsCol<V, C> acc{lds + c};
+    fft_line<PL, -1>(v, t, g_tw<M>(), acc);
+    float rnum = 0.f, rden = 0.f;
+    if constexpr (MODE == COL_BLUR) {
+        const float dc = (float)(col - N / 2);
+        const float gc = gcoef_c * dc * dc;
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const int row = t + PL::rot(m) * TT;
+            const float dr = (float)(row - M / 2);
+            const float g = expf(gcoef_r * dr * dr + gc) * scale;
+            v[m] = cscale(v[m], g);
+        });
+    } else {
+        const float* Ab = A + (b % amp_batch) * (int64_t)M * N + col;
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const int row = t + PL::rot(m) * TT;
+            const float a = Ab[(int64_t)row * N];
+            V x = v[m];
+            const float m2 = x.x * x.x + x.y * x.y;
+            if constexpr (MODE == COL_MOD_RF) {
+                if (a > 0.f) {
+                    rnum += fabsf(sqrtf(m2) - a);
+                    rden += a;
+                }
+            }
+            float s = scale;
+            if (!(skip_zero && a == 0.f) && m2 > 0.f) s = scale * a / sqrtf(m2);
+            v[m] = cscale(x, s);
+        });
+    }
+    fft_line<PL, +1>(v, t, g_tw<M>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        Z[base + (int64_t)(t + PL::rot(m) * TT) * N] = v[m];
+    });
+    if constexpr (MODE == COL_MOD_RF) {
+        red[0][tid] = rnum;
+        red[1][tid] = rden;
+        __syncthreads();
+        for (int s = SizeCfg<M>::TPB / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                red[0][tid] += red[0][tid + s];
+                red[1][tid] += red[1][tid + s];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            rf_part[2 * blockIdx.x] = red[0][0];
+            rf_part[2 * blockIdx.x + 1] = red[1][0];
+        }
+    }
+}
+
+enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
+
+// Row kernel of the HIO iteration (Fienup 1982; see orc_hio in the oracle).
+template <int N, int MODE>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
+    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
+    int64_t nlines, float beta, float invN, int real_positive)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int64_t base = (valid ? line : 0) * N;
+    LdsRow<V, N> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    V xp[E], xo[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        xp[m] = valid ? Z[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(xp, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        xo[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(xo, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const bool s = valid ? mask[base + t + PL::rot(m) * TT] != 0 : false;
+        const V p = xp[m];
+        const bool ok = s && (!real_positive || p.x >= 0.f);
+        V r;
+        if (ok) {
+            r = real_positive ? mk<V>(p.x, 0.f) : p;
+        } else if constexpr (MODE == ROW_ER) {
+            r = mk<V>(0.f, 0.f);
+        } else {
+            r = mk<V>(xo[m].x * invN - beta * p.x, xo[m].y * invN - beta * p.y);
+        }
+        xo[m] = r;
+    });
+    fft_line<PL, -1>(xo, t, tw, acc);
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            Y[base + t + PL::rot(m) * TT] = xo[m];
+        });
+    }
+}
+
+// Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
+// read directly.
+template <int N, bool FROM_STATE>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __restrict__ Y,
+                                                           c2<float>* __restrict__ W,
+                                                           int64_t nlines, float invN)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int64_t base = (valid ? line : 0) * N;
+    LdsRow<V, N> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    V v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    if constexpr (FROM_STATE) fft_line<PL, +1>(v, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float sc = FROM_STATE ? invN : 1.f;
+        const float a = sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc;
+        v[m] = mk<V>(a, 0.f);
+    });
+    fft_line<PL, -1>(v, t, tw, acc);
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            W[base + t + PL::rot(m) * TT] = v[m];
+        });
+    }
+}
+
+// Shrinkwrap, row stage 3: blurred image b = Re IDFT_r(W) and per-pattern max.
+template <int N>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_blurmax(const c2<float>* __restrict__ W,
+                                                               float* __restrict__ bl,
+                                                               unsigned* __restrict__ keys,
+                                                               int64_t nlines, int M)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    __shared__ unsigned bmax[LPB];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int64_t base = (valid ? line : 0) * N;
+    LdsRow<V, N> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    V v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = valid ? W[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(v, t, tw, acc);
+    float mx = -INFINITY;
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            bl[base + t + PL::rot(m) * TT] = v[m].x;
+            mx = fmaxf(mx, v[m].x);
+        });
+    }
+    if (tid < LPB) bmax[tid] = 0u;
+    __syncthreads();
+    if (valid) atomicMax(&bmax[ll], f2key(mx));
+    __syncthreads();
+    if (tid < LPB) {
+        const int64_t ln = (int64_t)blockIdx.x * LPB + tid;
+        if (ln < nlines) atomicMax(&keys[ln / M], bmax[tid]);
+    }
+}
+
+__global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// mask = b >= frac * max_pattern(b)
+__global__ void k_sw_thresh(const float* __restrict__ bl, const unsigned* __restrict__ keys,
+                            uint8_t* __restrict__ mask, int64_t per, int64_t total, float frac)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float thr = frac * key2f(keys[i / per]);
+        mask[i] = bl[i] >= thr ? 1 : 0;
+    }
+}
+
+// R-factor: sum the per-block partials of each pattern (fixed order, double).
+__global__ void k_rf_fin(const float* __restrict__ part, int blocks_per_pattern, int64_t B,
+                         float* __restrict__ rf)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < blocks_per_pattern; ++k) {
+        num += part[2 * (b * blocks_per_pattern + k)];
+        den += part[2 * (b * blocks_per_pattern + k) + 1];
+    }
+    rf[b] = den > 0.0 ? (float)(num / den) : 0.f;
+}
+
+__global__ void k_u8_from_S(const float* __restrict__ S, uint8_t* __restrict__ mask, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        mask[i] = (double)S[i] < 0.9 ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct EngineGraph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    // key of what was captured
+    const void* x = nullptr;
+    const void* amp = nullptr;
+    const void* mask = nullptr;
+    const void* ws = nullptr;
+    uint64_t amp_batch = 0;
+    int iters = 0;
+    float beta = 0.f;
+    int rp = 0, skip = 0;
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+struct Geo {
+    int M, N;
+    int64_t B;
+};
+
+template <int M> static int col_tile() { return SizeCfg<M>::C; }
+
+static int col_tile_rt(int M)
+{
+    switch (M) {
+    case 64: return col_tile<64>();
+    case 128: return col_tile<128>();
+    case 192: return col_tile<192>();
+    case 256: return col_tile<256>();
+    case 384: return col_tile<384>();
+    case 512: return col_tile<512>();
+    }
+    return 0;
+}
+
+static int row_lpb_rt(int N)
+{
+    switch (N) {
+    case 64: return SizeCfg<64>::LPB;
+    case 128: return SizeCfg<128>::LPB;
+    case 192: return SizeCfg<192>::LPB;
+    case 256: return SizeCfg<256>::LPB;
+    case 384: return SizeCfg<384>::LPB;
+    case 512: return SizeCfg<512>::LPB;
+    }
+    return 0;
+}
+
+void engine_check_geometry(cpra_hip_ctx* h, const char* who)
+{
+    if (h->dtype != CPRA_HIP_F32)
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: the fused engine runs in FP32 (complex64) only", who);
+    const int M = (int)h->m, N = (int)h->n;
+    if (!fft_fast_size(M) || !fft_fast_size(N))
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: pattern %dx%d not in {64,128,192,256,384,512}^2", who, M, N);
+    if (N % col_tile_rt(M) != 0)
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: N=%d not a multiple of the column tile %d", who, N, col_tile_rt(M));
+}
+
+template <int M>
+static void launch_cols_m(hipStream_t st, const Geo& g, int mode, const void* Y, void* Z,
+                          const float* A, int64_t amp_batch, float scale, int skip,
+                          float* rf_part, float gr, float gc)
+{
+    const unsigned grid = (unsigned)(g.B * g.N / SizeCfg<M>::C);
+    const dim3 blk(SizeCfg<M>::TPB);
+    auto* y = static_cast<const c2<float>*>(Y);
+    auto* z = static_cast<c2<float>*>(Z);
+    if (mode == COL_MOD)
+        hipLaunchKernelGGL((k_eng_cols<M, COL_MOD>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc);
+    else if (mode == COL_MOD_RF)
+        hipLaunchKernelGGL((k_eng_cols<M, COL_MOD_RF>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc);
+    else
+        hipLaunchKernelGGL((k_eng_cols<M, COL_BLUR>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc);
+}
+
+static void launch_cols(hipStream_t st, const Geo& g, int mode, const void* Y, void* Z,
+                        const float* A, int64_t amp_batch, float scale, int skip, float* rf_part,
+                        float gr = 0.f, float gc = 0.f)
+{
+    switch (g.M) {
+    case 64: launch_cols_m<64>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 128: launch_cols_m<128>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 192: launch_cols_m<192>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 256: launch_cols_m<256>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 384: launch_cols_m<384>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 512: launch_cols_m<512>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    }
+    check_launch("k_eng_cols");
+}
+
+template <int N>
+static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
+                              const uint8_t* mask, float beta, int rp, int er)
+{
+    const int64_t lines = g.B * g.M;
+    const unsigned grid = (unsigned)((lines + SizeCfg<N>::LPB - 1) / SizeCfg<N>::LPB);
+    auto* z = static_cast<const c2<float>*>(Z);
+    auto* y = static_cast<c2<float>*>(Y);
+    const float invN = 1.0f / (float)N;
+    if (er)
+        hipLaunchKernelGGL((k_eng_rows_hio<N, ROW_ER>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, lines, beta, invN, rp);
+    else
+        hipLaunchKernelGGL((k_eng_rows_hio<N, ROW_HIO>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, lines, beta, invN, rp);
+}
+
+static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
+                            const uint8_t* mask, float beta, int rp, int er)
+{
+    switch (g.N) {
+    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er); break;
+    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er); break;
+    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er); break;
+    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er); break;
+    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er); break;
+    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er); break;
+    }
+    check_launch("k_eng_rows_hio");
+}
+
+template <int N>
+static void launch_sw_rows_n(hipStream_t st, const Geo& g, int stage, bool from_state,
+                             const void* in, void* out, float* bl, unsigned* keys)
+{
+    const int64_t lines = g.B * g.M;
+    const unsigned grid = (unsigned)((lines + SizeCfg<N>::LPB - 1) / SizeCfg<N>::LPB);
+    const dim3 blk(SizeCfg<N>::TPB);
+    auto* i = static_cast<const c2<float>*>(in);
+    if (stage == 1) {
+        auto* o = static_cast<c2<float>*>(out);
+        if (from_state)
+            hipLaunchKernelGGL((k_sw_mag<N, true>), dim3(grid), blk, 0, st, i, o, lines, 1.0f / (float)N);
+        else
+            hipLaunchKernelGGL((k_sw_mag<N, false>), dim3(grid), blk, 0, st, i, o, lines, 1.0f);
+    } else {
+        hipLaunchKernelGGL((k_sw_blurmax<N>), dim3(grid), blk, 0, st, i, bl, keys, lines, g.M);
+    }
+}
+
+static void launch_sw_rows(hipStream_t st, const Geo& g, int stage, bool from_state,
+                           const void* in, void* out, float* bl, unsigned* keys)
+{
+    switch (g.N) {
+    case 64: launch_sw_rows_n<64>(st, g, stage, from_state, in, out, bl, keys); break;
+    case 128: launch_sw_rows_n<128>(st, g, stage, from_state, in, out, bl, keys); break;
+    case 192: launch_sw_rows_n<192>(st, g, stage, from_state, in, out, bl, keys); break;
+    case 256: launch_sw_rows_n<256>(st, g, stage, from_state, in, out, bl, keys); break;
+    case 384: launch_sw_rows_n<384>(st, g, stage, from_state, in, out, bl, keys); break;
+    case 512: launch_sw_rows_n<512>(st, g, stage, from_state, in, out, bl, keys); break;
+    }
+    check_launch("k_sw_rows");
+}
+
+static Geo geo_of(cpra_hip_ctx* h) { return Geo{(int)h->m, (int)h->n, (int64_t)h->batch}; }
+
+// Shrinkwrap on the state (from_state) or on a real-space x; writes the mask
+// and, if blur_out != NULL, copies the blurred image there.
+static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, bool from_state,
+                              uint8_t* mask, float sigma, float frac, float* blur_out)
+{
+    const Geo g = geo_of(h);
+    const int64_t px = (int64_t)g.M * g.N;
+    const int64_t total = px * g.B;
+    h->ws_z.ensure(sizeof(c2<float>) * total);
+    h->ws_blur.ensure(sizeof(float) * total);
+    h->ws_keys.ensure(sizeof(unsigned) * g.B);
+    void* W = h->ws_z.ptr;
+    float* bl = blur_out ? blur_out : static_cast<float*>(h->ws_blur.ptr);
+    unsigned* keys = static_cast<unsigned*>(h->ws_keys.ptr);
+    launch_sw_rows(st, g, 1, from_state, src, W, nullptr, nullptr);
+    // G(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-M/2)/M)^2 + ((kc-N/2)/N)^2])
+    const double k = -2.0 * M_PI * M_PI * (double)sigma * (double)sigma;
+    const float gr = (float)(k / ((double)g.M * g.M));
+    const float gc = (float)(k / ((double)g.N * g.N));
+    const float scale = (float)(1.0 / ((double)g.M * g.N));
+    launch_cols(st, g, COL_BLUR, W, W, nullptr, 1, scale, 0, nullptr, gr, gc);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(g.B, 256)), dim3(256), 0, st, keys, 0u, g.B);
+    launch_sw_rows(st, g, 3, false, W, nullptr, bl, keys);
+    hipLaunchKernelGGL(k_sw_thresh, dim3(grid_1d(total, 256, 65536)), dim3(256), 0, st, bl, keys, mask, px, total, frac);
+    check_launch("k_sw_thresh");
+}
+
+static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, const float* amp,
+                           int64_t amp_batch, int skip, float* rf_dev)
+{
+    const Geo g = geo_of(h);
+    const int C = col_tile_rt(g.M);
+    const int bpp = g.N / C;
+    h->ws_part.ensure(sizeof(float) * 2 * (size_t)bpp * g.B);
+    float* part = static_cast<float*>(h->ws_part.ptr);
+    const float scale = (float)(1.0 / ((double)g.M * g.N));
+    launch_cols(st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part);
+    hipLaunchKernelGGL(k_rf_fin, dim3(grid_1d(g.B, 64)), dim3(64), 0, st, part, bpp, g.B, rf_dev);
+    check_launch("k_rf_fin");
+}
+
+static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
+                          int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p)
+{
+    const Geo g = geo_of(h);
+    const float scale = (float)(1.0 / ((double)g.M * g.N));
+    launch_cols(st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr);
+    launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction);
+}
+
+static hipStream_t aux_stream(cpra_hip_ctx* h)
+{
+    if (!h->aux_stream) CPRA_CHECK_HIP(hipStreamCreateWithFlags(&h->aux_stream, hipStreamNonBlocking));
+    return h->aux_stream;
+}
+
+void cpra_engine_release(cpra_hip_ctx* h)
+{
+    if (h->graph) {
+        h->graph->release();
+        delete h->graph;
+        h->graph = nullptr;
+    }
+}
+
+}  // namespace cpra
+
+using namespace cpra;
+
+template <typename F> static int eguard(cpra_hip_ctx* h, F&& f)
+{
+    try {
+        return f();
+    } catch (const Status& s) {
+        if (h) h->last_error = s.msg;
+        return s.code;
+    } catch (const std::exception& e) {
+        if (h) h->last_error = e.what();
+        return CPRA_HIP_ERR_HIP;
+    }
+}
+
+extern "C" {
+
+void cpra_hip_hio_default_params(cpra_hip_hio_params* p)
+{
+    if (!p) return;
+    p->iters = 200;
+    p->beta = 0.9f;
+    p->real_positive = 1;
+    p->dc_skip_zero = 1;
+    p->error_reduction = 0;
+    p->sw_every = 20;
+    p->sw_sigma0 = 3.0f;
+    p->sw_sigma1 = 1.5f;
+    p->sw_frac = 0.1f;
+    p->use_graph = 1;
+}
+
+int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                     uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && x && amp && support && p, "cpra_hip_hio_run: NULL argument");
+        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0,
+                 "cpra_hip_hio_run: amp_batch must divide batch");
+        CPRA_ARG(p->iters >= 0, "cpra_hip_hio_run: iters < 0");
+        CPRA_ARG(p->sw_every >= 0, "cpra_hip_hio_run: sw_every < 0");
+        engine_check_geometry(h, "cpra_hip_hio_run");
+        const Geo g = geo_of(h);
+        const int64_t total = (int64_t)g.M * g.N * g.B;
+        h->ws_z.ensure(sizeof(c2<float>) * total);
+        h->ws_blur.ensure(sizeof(float) * total);
+        h->ws_keys.ensure(sizeof(unsigned) * g.B);
+        h->ws_misc.ensure(sizeof(float) * g.B + 256);
+        hipStream_t st = h->stream;
+        const int nsw = p->sw_every > 0 ? p->iters / p->sw_every : 0;
+        // enter the row-spectral state in place
+        fft_pass_contig(h, x, g.N, g.B * g.M, -1, true, 1.0, false);
+        const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
+        int done = 0;
+        bool use_graph = p->use_graph && p->iters >= chunk && chunk > 0;
+        hipStream_t gst = st;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (use_graph) {
+            gst = aux_stream(h);
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            EngineGraph* eg = h->graph;
+            const bool same = eg && eg->x == x && eg->amp == amp && eg->mask == support &&
+                              eg->ws == h->ws_z.ptr &&
+                              eg->amp_batch == amp_batch && eg->iters == chunk &&
+                              eg->beta == p->beta && eg->rp == p->real_positive &&
+                              eg->skip == p->dc_skip_zero;
+            if (!same) {
+                if (!eg) eg = h->graph = new EngineGraph();
+                eg->release();
+                CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < chunk; ++i)
+                    hio_iteration(h, gst, x, amp, (int64_t)amp_batch, support, p);
+                CPRA_CHECK_HIP(hipStreamEndCapture(gst, &eg->graph));
+                CPRA_CHECK_HIP(hipGraphInstantiate(&eg->exec, eg->graph, nullptr, nullptr, 0));
+                eg->x = x;
+                eg->amp = amp;
+                eg->mask = support;
+                eg->ws = h->ws_z.ptr;
+                eg->amp_batch = amp_batch;
+                eg->iters = chunk;
+                eg->beta = p->beta;
+                eg->rp = p->real_positive;
+                eg->skip = p->dc_skip_zero;
+            }
+        }
+        int k = 0;
+        while (done < p->iters) {
+            const int todo = (p->iters - done) >= chunk ? chunk : (p->iters - done);
+            if (use_graph && todo == chunk) {
+                CPRA_CHECK_HIP(hipGraphLaunch(h->graph->exec, gst));
+            } else {
+                for (int i = 0; i < todo; ++i)
+                    hio_iteration(h, gst, x, amp, (int64_t)amp_batch, support, p);
+            }
+            done += todo;
+            if (p->sw_every > 0 && todo == chunk && done % p->sw_every == 0) {
+                const float sigma =
+                    nsw > 1 ? p->sw_sigma0 + (p->sw_sigma1 - p->sw_sigma0) * (float)k / (float)(nsw - 1)
+                            : p->sw_sigma0;
+                // run the shrinkwrap on the same stream as the iterations
+                hipStream_t save = h->stream;
+                h->stream = gst;
+                shrinkwrap_launch(h, gst, x, true, support, sigma, p->sw_frac, nullptr);
+                h->stream = save;
+                ++k;
+            }
+        }
+        if (use_graph) {
+            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            CPRA_CHECK_HIP(hipEventDestroy(ev0));
+            CPRA_CHECK_HIP(hipEventDestroy(ev1));
+        }
+        float* rf_dev = static_cast<float*>(h->ws_misc.ptr);
+        if (rfactor_host) rfactor_launch(h, st, x, amp, (int64_t)amp_batch, 1, rf_dev);
+        // leave the row-spectral state
+        fft_pass_contig(h, x, g.N, g.B * g.M, +1, true, 1.0 / (double)g.N, false);
+        if (rfactor_host) {
+            CPRA_CHECK_HIP(hipMemcpyAsync(rfactor_host, rf_dev, sizeof(float) * g.B, hipMemcpyDeviceToHost, st));
+            CPRA_CHECK_HIP(hipStreamSynchronize(st));
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_shrinkwrap(cpra_hip_handle h, const void* x, uint8_t* support, float sigma,
+                        float frac, float* blur_out)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && x && support, "cpra_hip_shrinkwrap: NULL argument");
+        CPRA_ARG(sigma >= 0.f, "cpra_hip_shrinkwrap: sigma < 0");
+        engine_check_geometry(h, "cpra_hip_shrinkwrap");
+        shrinkwrap_launch(h, h->stream, x, false, support, sigma, frac, blur_out);
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_rfactor(cpra_hip_handle h, const void* x, const float* amp, uint64_t amp_batch,
+                     float* r_host)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && x && amp && r_host, "cpra_hip_rfactor: NULL argument");
+        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0,
+                 "cpra_hip_rfactor: amp_batch must divide batch");
+        engine_check_geometry(h, "cpra_hip_rfactor");
+        const Geo g = geo_of(h);
+        const int64_t total = (int64_t)g.M * g.N * g.B;
+        h->ws_z.ensure(sizeof(c2<float>) * total);
+        h->ws_misc.ensure(sizeof(float) * g.B + 256);
+        hipStream_t st = h->stream;
+        CPRA_CHECK_HIP(hipMemcpyAsync(h->ws_z.ptr, x, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
+        fft_pass_contig(h, h->ws_z.ptr, g.N, g.B * g.M, -1, true, 1.0, false);
+        float* rf_dev = static_cast<float*>(h->ws_misc.ptr);
+        rfactor_launch(h, st, h->ws_z.ptr, amp, (int64_t)amp_batch, 1, rf_dev);
+        CPRA_CHECK_HIP(hipMemcpyAsync(r_host, rf_dev, sizeof(float) * g.B, hipMemcpyDeviceToHost, st));
+        CPRA_CHECK_HIP(hipStreamSynchronize(st));
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t* mask, uint64_t num)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && S && mask, "cpra_hip_support_from_constraint: NULL argument");
+        if (num == 0) return CPRA_HIP_FALSE;
+        hipLaunchKernelGGL(k_u8_from_S, dim3(grid_1d(num, 256, 65536)), dim3(256), 0, h->stream, S, mask, (int64_t)num);
+        check_launch("k_u8_from_S");
+        return CPRA_HIP_OK;
+    });
+}
+
+}  // extern "C"

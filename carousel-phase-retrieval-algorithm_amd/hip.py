@@ -1,0 +1,289 @@
+"""Python front end of libcpra_hip.so (ctypes; no torch types cross the ABI).
+
+``HipImpl`` mirrors the reference's backend interface ``CPRA::CpraImpl<T>``
+(include/CPRA_Impl.hpp:18-63): same method names, argument meaning and bool
+results (``True`` = the reference's ``true``, ``False`` = its ``false`` for a
+well-formed no-op such as ``num == 0`` or ``alpha == 0``).  Errors that the
+reference would throw or abort on raise ``CpraHipError``.
+
+Arguments that the C++ interface takes as pointers accept either a torch
+tensor (its ``data_ptr()`` is passed; the tensor must live on the GPU) or a
+raw integer device address.  The handle launches on torch's current stream
+by default so that work is ordered with torch operations.
+
+The library must be built (``build.py``); nothing here falls back to a CPU
+implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpra_hip.so")
+
+OK = 0
+FALSE = 1
+ERR_ARG = -1
+ERR_UNSUPPORTED = -2
+ERR_NOMEM = -3
+ERR_HIP = -4
+ERR_NODEV = -5
+
+F32 = 0
+F64 = 1
+
+SPACE_MKL = 0
+SPACE_POSITIVE = 1
+
+
+class CpraHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class HioParams(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int32),
+        ("beta", C.c_float),
+        ("real_positive", C.c_int32),
+        ("dc_skip_zero", C.c_int32),
+        ("error_reduction", C.c_int32),
+        ("sw_every", C.c_int32),
+        ("sw_sigma0", C.c_float),
+        ("sw_sigma1", C.c_float),
+        ("sw_frac", C.c_float),
+        ("use_graph", C.c_int32),
+    ]
+
+
+_lib = None
+
+_P = C.c_void_p
+_U64 = C.c_uint64
+_H = C.c_void_p
+
+_SIGS = {
+    "cpra_hip_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, _U64, _U64, _U64, _U64]),
+    "cpra_hip_destroy": (C.c_int, [_H]),
+    "cpra_hip_last_error": (C.c_char_p, [_H]),
+    "cpra_hip_version": (C.c_char_p, []),
+    "cpra_hip_device_count": (C.c_int, []),
+    "cpra_hip_set_stream": (C.c_int, [_H, _P]),
+    "cpra_hip_get_stream": (C.c_void_p, [_H]),
+    "cpra_hip_malloc": (C.c_int, [_H, C.POINTER(C.c_void_p), _U64, C.c_int]),
+    "cpra_hip_free": (C.c_int, [_H, _P]),
+    "cpra_hip_memcpy": (C.c_int, [_H, _P, _P, _U64]),
+    "cpra_hip_sync": (C.c_int, [_H]),
+    "cpra_hip_initialize": (C.c_int, [_H, _P, _U64, _U64]),
+    "cpra_hip_forward2d": (C.c_int, [_H, _P]),
+    "cpra_hip_backward2d": (C.c_int, [_H, _P]),
+    "cpra_hip_forward3d": (C.c_int, [_H, _P]),
+    "cpra_hip_backward3d": (C.c_int, [_H, _P]),
+    "cpra_hip_fftshift2d": (C.c_int, [_H, _P, _U64, _U64, _U64]),
+    "cpra_hip_fftshift3d": (C.c_int, [_H, _P, _U64, _U64, _U64, _U64]),
+    "cpra_hip_space_constraint": (C.c_int, [_H, _P, _P, _U64, _U64, C.c_int]),
+    "cpra_hip_data_constraint_real": (C.c_int, [_H, _P, _P, _U64, _U64, C.c_int]),
+    "cpra_hip_data_constraint_complex": (C.c_int, [_H, _P, _P, _U64, _U64]),
+    "cpra_hip_merge_add": (C.c_int, [_H, _P, _P, C.c_double, C.c_double, _U64]),
+    "cpra_hip_normalize": (C.c_int, [_H, _P, C.c_double, _U64]),
+    "cpra_hip_converge_error": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
+    "cpra_hip_hio_default_params": (None, [C.POINTER(HioParams)]),
+    "cpra_hip_hio_run": (C.c_int, [_H, _P, _P, _U64, _P, C.POINTER(HioParams), _P]),
+    "cpra_hip_shrinkwrap": (C.c_int, [_H, _P, _P, C.c_float, C.c_float, _P]),
+    "cpra_hip_rfactor": (C.c_int, [_H, _P, _P, _U64, _P]),
+    "cpra_hip_support_from_constraint": (C.c_int, [_H, _P, _P, _U64]),
+}
+
+
+def exported_symbols() -> Sequence[str]:
+    return tuple(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """Load libcpra_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with build.py (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _addr(x) -> Optional[int]:
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if hasattr(x, "is_cuda") and not x.is_cuda:
+            raise CpraHipError(ERR_ARG, "tensor must be on the GPU")
+        if hasattr(x, "is_contiguous") and not x.is_contiguous():
+            raise CpraHipError(ERR_ARG, "tensor must be contiguous")
+        return x.data_ptr()
+    raise TypeError(f"cannot pass {type(x)} as a device pointer")
+
+
+def default_hio_params(**kw) -> HioParams:
+    p = HioParams()
+    lib().cpra_hip_hio_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+class HipImpl:
+    """One handle of the HIP backend; geometry fixed at construction like
+    ``NewCpraImpl<T>(IMPL_TYPE::HIP, m, n, l, batch_size)``."""
+
+    def __init__(self, m: int, n: int, l: int = 1, batch_size: int = 1, dtype: str = "float32",
+                 stream=None, use_torch_stream: bool = True):
+        self.m, self.n, self.l, self.batch = int(m), int(n), int(l), int(batch_size)
+        self.dtype = dtype
+        code = F64 if dtype in ("float64", "double") else F32
+        h = C.c_void_p()
+        rc = lib().cpra_hip_create(C.byref(h), code, self.m, self.n, self.l, self.batch)
+        if rc < 0:
+            raise CpraHipError(rc, lib().cpra_hip_last_error(None).decode())
+        self._h = h
+        if stream is not None:
+            self.set_stream(stream)
+        elif use_torch_stream:
+            try:
+                import torch
+                if torch.cuda.is_available():
+                    self.set_stream(torch.cuda.current_stream().cuda_stream)
+            except ImportError:
+                pass
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc: int) -> bool:
+        if rc < 0:
+            raise CpraHipError(rc, lib().cpra_hip_last_error(self._h).decode())
+        return rc == OK
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cpra_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream) -> bool:
+        s = stream if isinstance(stream, int) else getattr(stream, "cuda_stream", stream)
+        return self._check(lib().cpra_hip_set_stream(self._h, s))
+
+    # -- CpraImpl<T> mirror (include/CPRA_Impl.hpp:23-61) -------------------
+    def Initialize(self, data, num: int, seed: int = 0) -> bool:
+        return self._check(lib().cpra_hip_initialize(self._h, _addr(data), num, seed))
+
+    def Forward2D(self, x) -> bool:
+        return self._check(lib().cpra_hip_forward2d(self._h, _addr(x)))
+
+    def Backward2D(self, x) -> bool:
+        return self._check(lib().cpra_hip_backward2d(self._h, _addr(x)))
+
+    def FFTShift2D(self, x, M: int, N: int, Batch: int) -> bool:
+        return self._check(lib().cpra_hip_fftshift2d(self._h, _addr(x), M, N, Batch))
+
+    def Forward3D(self, x) -> bool:
+        return self._check(lib().cpra_hip_forward3d(self._h, _addr(x)))
+
+    def Backward3D(self, x) -> bool:
+        return self._check(lib().cpra_hip_backward3d(self._h, _addr(x)))
+
+    def FFTShift3D(self, x, M: int, N: int, L: int, Batch: int) -> bool:
+        return self._check(lib().cpra_hip_fftshift3d(self._h, _addr(x), M, N, L, Batch))
+
+    def SpaceConstraint(self, x, S, num: int, batch_size: int, mode: int = SPACE_MKL) -> bool:
+        return self._check(
+            lib().cpra_hip_space_constraint(self._h, _addr(x), _addr(S), num, batch_size, mode))
+
+    def DataConstraint(self, x, A, num: int, batch_size: int, skip_zero: bool = True) -> bool:
+        """Real amplitude (float A) or complex (complex A) data constraint,
+        dispatched on A's dtype like the two C++ overloads."""
+        if hasattr(A, "is_complex") and A.is_complex():
+            return self._check(lib().cpra_hip_data_constraint_complex(
+                self._h, _addr(x), _addr(A), num, batch_size))
+        return self._check(lib().cpra_hip_data_constraint_real(
+            self._h, _addr(x), _addr(A), num, batch_size, int(skip_zero)))
+
+    def MergeAddData(self, src, dst, alpha: float, beta: float, num: int) -> bool:
+        return self._check(
+            lib().cpra_hip_merge_add(self._h, _addr(src), _addr(dst), alpha, beta, num))
+
+    def Normalization(self, x, norm: float, num: int) -> bool:
+        return self._check(lib().cpra_hip_normalize(self._h, _addr(x), norm, num))
+
+    def ConvergeError(self, old, new, m: int, n: int, l: int = 1, batch_size: int = 1):
+        """Returns (ok, errors) with errors a numpy array of batch_size."""
+        import numpy as np
+        err = np.zeros(batch_size, dtype=np.float64 if self.dtype in ("float64", "double")
+                       else np.float32)
+        ok = self._check(lib().cpra_hip_converge_error(
+            self._h, _addr(old), _addr(new), err.ctypes.data, m, n, l, batch_size))
+        return ok, err
+
+    def Memcpy(self, dst, src, nbytes: int) -> bool:
+        return self._check(lib().cpra_hip_memcpy(self._h, _addr(dst), _addr(src), nbytes))
+
+    def Sync(self) -> bool:
+        return self._check(lib().cpra_hip_sync(self._h))
+
+    # -- fused engines (include/cpra_hip_engine.h) -------------------------
+    def hio_run(self, x, amp, support, params: Optional[HioParams] = None, amp_batch=None,
+                want_rfactor: bool = False, **kw):
+        import numpy as np
+        p = params if params is not None else default_hio_params(**kw)
+        ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n)
+        rf = np.zeros(self.batch, dtype=np.float32) if want_rfactor else None
+        self._check(lib().cpra_hip_hio_run(self._h, _addr(x), _addr(amp), ab, _addr(support),
+                                           C.byref(p), rf.ctypes.data if rf is not None else None))
+        return rf
+
+    def shrinkwrap(self, x, support, sigma: float, frac: float, blur_out=None) -> bool:
+        return self._check(lib().cpra_hip_shrinkwrap(self._h, _addr(x), _addr(support), sigma,
+                                                     frac, _addr(blur_out)))
+
+    def rfactor(self, x, amp, amp_batch=None):
+        import numpy as np
+        ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n)
+        r = np.zeros(self.batch, dtype=np.float32)
+        self._check(lib().cpra_hip_rfactor(self._h, _addr(x), _addr(amp), ab, r.ctypes.data))
+        return r
+
+    def support_from_constraint(self, S, mask, num: int) -> bool:
+        return self._check(lib().cpra_hip_support_from_constraint(self._h, _addr(S), _addr(mask),
+                                                                  num))
+
+
+def device_count() -> int:
+    return lib().cpra_hip_device_count()
+
+
+def version() -> str:
+    return lib().cpra_hip_version().decode()

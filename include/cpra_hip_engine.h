@@ -1,0 +1,66 @@
+/*
+ * cpra_hip_engine.h -- fused iteration engines of libcpra_hip.so.
+ *
+ * These entry points have no single counterpart in the reference's
+ * CpraImpl<T> interface: the reference spells its iterations out as
+ * sequences of primitive calls in the drivers (src/CPRA_MKL.cpp:37-55,
+ * 94-110, 126-140; src/benchmark_mkl.cpp:45-63).  The engines run the same
+ * mathematics (or the north-star additions: HIO, shrinkwrap, R-factor) as a
+ * few fused HBM passes per iteration.  FP32 (complex64) handles only; the
+ * pattern size {m, n} of the handle must be in {64,128,192,256,384,512}.
+ */
+#ifndef CPRA_HIP_ENGINE_H
+#define CPRA_HIP_ENGINE_H
+#include "cpra_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* HIO (Fienup 1982) with a shrinkwrap schedule. */
+typedef struct {
+    int32_t iters;            /* HIO iterations                                      */
+    float beta;               /* feedback parameter                                  */
+    int32_t real_positive;    /* 1: inside the support keep Re x' >= 0, drop Im      */
+    int32_t dc_skip_zero;     /* 1: A == 0 leaves the Fourier pixel unconstrained    */
+                              /*    (MklImpl::DataConstraint, MKL_Impl.inl:95-101)   */
+    int32_t error_reduction;  /* 1: error reduction instead of HIO (outside -> 0)    */
+    int32_t sw_every;         /* shrinkwrap after every sw_every iterations (0: off) */
+    float sw_sigma0;          /* sigma of the first update (pixels)                  */
+    float sw_sigma1;          /* sigma of the last update (linear in between)        */
+    float sw_frac;            /* threshold fraction of the per-pattern blur maximum  */
+    int32_t use_graph;        /* capture sw_every iterations in a hipGraph           */
+} cpra_hip_hio_params;
+
+void cpra_hip_hio_default_params(cpra_hip_hio_params* p);
+
+/* Runs p->iters HIO iterations on x [batch][m][n] (complex64, in place) with
+ * amplitudes amp [amp_batch][m][n] (float; pattern b uses amp[b % amp_batch])
+ * and the u8 support [batch][m][n] (updated in place by shrinkwrap).
+ * After iteration i (1-based) with i % sw_every == 0, update k = i/sw_every-1
+ * of K = iters/sw_every uses sigma_k = sigma0 + (sigma1 - sigma0) k/(K-1).
+ * If rfactor_host != NULL it receives, per pattern, the R-factor
+ * sum_{A>0} | |Forward2D(x)| - A | / sum_{A>0} A of the final x. */
+int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                     uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host);
+
+/* Shrinkwrap of x [batch][m][n]: b = Re Backward2D(Forward2D(|x|) * G_sigma),
+ * G_sigma(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-m/2)/m)^2 + ((kc-n/2)/n)^2]);
+ * support = (b >= frac * max_pattern(b)).  blur_out (device, [batch][m][n]
+ * float) receives b when not NULL. */
+int cpra_hip_shrinkwrap(cpra_hip_handle h, const void* x, uint8_t* support, float sigma,
+                        float frac, float* blur_out);
+
+/* R-factor per pattern of x (see cpra_hip_hio_run); r_host has batch floats. */
+int cpra_hip_rfactor(cpra_hip_handle h, const void* x, const float* amp, uint64_t amp_batch,
+                     float* r_host);
+
+/* u8 support equivalent to the reference SpaceConstraint's float mask:
+ * mask[i] = !((double)S[i] < 0.9)  (MKL_Impl.inl:80). */
+int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t* mask,
+                                     uint64_t num);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
