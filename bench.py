@@ -1,0 +1,263 @@
+"""Benchmark: batched shrinkwrap-HIO phase retrieval on one MI355X.
+
+Workload (the headline configuration): 512 patterns of 256x256 complex64,
+each from its own seeded disc object (radii U[30, 50] px, Poisson noise at
+1e6 photons), HIO beta = 0.9 with real-positive support, shrinkwrap every 20
+iterations (sigma 3.0 -> 1.5 px, threshold 0.1 x max), the iteration body
+captured in a hipGraph.  One step = one HIO iteration of the whole batch
+(512 pattern-iterations), shrinkwrap updates included.  Inputs are resident
+in HBM before the timed region.
+
+Prints one JSON line (rank 0) with:
+  value      batched HIO iterations per second over the K timed steps;
+  roofline   the dominant fused kernel: algorithmic bytes per launch / its
+             average duration (HIP events on the stream it runs on, from a
+             separate eager run of the same workload), against 8 TB/s;
+  cpu_baseline  the FP64 CPU oracle timed on a bounded sample of the same
+             workload on this host's cores;
+  fp64_check the same 200-iteration schedule on two of the patterns, GPU vs
+             the FP64 oracle (R-factor and x).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "phase-retrieval iters/s (512×256² batch, 128³ carousel); % HBM peak"
+
+# algorithmic bytes per pixel of each fused kernel (complex64 = 8 B,
+# amplitude float = 4 B, support u8 = 1 B)
+COL_BYTES_PER_PX = 8 + 4 + 8        # read Y, read A, write Z
+ROW_BYTES_PER_PX = 8 + 8 + 1 + 8    # read Z, read Y, read support, write Y
+HIO_ALGO_BYTES_PER_PX = 21          # x read + write, A, support (one iteration)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_oracle(data, n_patterns, min_seconds=10.0):
+    """Times the FP64 oracle's HIO (same schedule) on a sample of patterns."""
+    import numpy as np
+
+    import __graft_entry__ as ge
+
+    o = ge.load_oracle()
+    x = data["x0"][:n_patterns].astype(np.complex128)
+    A = data["amp"][:n_patterns].astype(np.float64)
+    s = data["support"][:n_patterns]
+    iters = 2
+    while True:
+        t0 = time.perf_counter()
+        o.hio(x, A, s, iters, beta=0.9, real_positive=True, dc_skip_zero=True, sw_every=0)
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds or iters >= 4096:
+            break
+        iters = max(iters * 2, int(iters * min_seconds / max(dt, 1e-3) * 1.1))
+    pit_per_s = n_patterns * iters / dt
+    return {
+        "value": pit_per_s / 512.0,
+        "unit": "iters/s",
+        "cores": min(o.num_threads(), n_patterns),
+        "kind": "port",
+        "sample": f"FP64 oracle HIO, {n_patterns} of the 512 patterns x {iters} iterations "
+                  f"({dt:.1f} s), scaled to 512-pattern batched iterations",
+        "pattern_iters_per_s": pit_per_s,
+    }
+
+
+_MKL_SNIPPET = r"""
+import json, sys, time
+import numpy as np
+import torch
+sys.path.insert(0, sys.argv[1])
+import __graft_entry__ as ge
+ge.load_package()
+from cpra_amd import synth
+n_patterns, iters = int(sys.argv[2]), int(sys.argv[3])
+data = synth.make_2d_batch(n_patterns, 256, 30.0, 50.0, 1e6, starts_per_pattern=1, config=1)
+x = torch.from_numpy(data["x0"].copy())
+A = torch.from_numpy(data["amp"].copy())
+s = torch.from_numpy(data["support"].astype(bool))
+"""
+
+
+def cpu_baseline_mkl(n_patterns=16, iters=4):
+    """HIO on the host with torch's CPU FFT (MKL-backed), FP32, for reference.
+    Runs in a child process so that torch's bundled HIP runtime never shares
+    a process with the benchmark's."""
+    import subprocess
+
+    code = _MKL_SNIPPET + _MKL_BODY
+    r = subprocess.run([sys.executable, "-c", code, ROOT, str(n_patterns), str(iters)],
+                       capture_output=True, text=True, timeout=600,
+                       env={**os.environ, "HIP_VISIBLE_DEVICES": "", "CUDA_VISIBLE_DEVICES": ""})
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+_MKL_BODY = r"""
+
+def step(x):
+    X = torch.fft.fftshift(torch.fft.fft2(torch.fft.fftshift(x, dim=(-2, -1))), dim=(-2, -1))
+    m = X.abs()
+    Xp = torch.where((A > 0) & (m > 0), X * (A / torch.where(m > 0, m, 1)), X)
+    xp = torch.fft.fftshift(torch.fft.ifft2(torch.fft.fftshift(Xp, dim=(-2, -1))),
+                            dim=(-2, -1))
+    ok = s & (xp.real >= 0)
+    return torch.where(ok, torch.complex(xp.real, torch.zeros_like(xp.real)), x - 0.9 * xp)
+
+step(x)
+t0 = time.perf_counter()
+for _ in range(iters):
+    x = step(x)
+dt = time.perf_counter() - t0
+print(json.dumps({
+    "value": n_patterns * iters / dt / 512.0,
+    "unit": "iters/s",
+    "cores": torch.get_num_threads(),
+    "kind": "torch-cpu-mkl-fp32",
+    "sample": f"{n_patterns} patterns x {iters} HIO iterations (no shrinkwrap)",
+}))
+"""
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--patterns", type=int, default=512)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--timing-iters", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; every box here has one GPU, "
+                                   "see the sharding design for more", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+
+    import __graft_entry__ as ge
+
+    ge.load_package()
+    from cpra_amd import hip, synth
+
+    assert hip.device_count() > 0, "bench.py needs a GPU"
+    n, P = args.size, args.patterns
+    log(f"bench: generating {P} x {n}^2 patterns")
+    t0 = time.perf_counter()
+    data = synth.make_2d_batch(P, n, 30.0 * n / 256, 50.0 * n / 256, 1e6, starts_per_pattern=1,
+                               config=1)
+    log(f"bench: data ready in {time.perf_counter() - t0:.1f} s")
+    x = hip.to_device(data["x0"])
+    amp = hip.to_device(data["amp"])
+    sup = hip.to_device(data["support"])
+    sched = dict(synth.HIO_SCHEDULE)
+    sched.pop("iters")
+    h = hip.HipImpl(n, n, 1, P)
+
+    # warmup (also captures the 20-iteration graph)
+    h.hio_run(x, amp, sup, iters=args.warmup, use_graph=1, **sched)
+    h.Sync()
+    log("bench: warmup done")
+
+    h.Sync()
+    t0 = time.perf_counter()
+    h.hio_run(x, amp, sup, iters=args.steps, use_graph=1, **sched)
+    h.Sync()
+    elapsed = time.perf_counter() - t0
+    its = args.steps / elapsed
+    log(f"bench: {args.steps} iterations in {elapsed:.3f} s -> {its:.1f} it/s")
+
+    # per-kernel durations with HIP events (eager launches, same workload)
+    h.set_kernel_timing(True)
+    h.hio_run(x, amp, sup, iters=args.timing_iters, use_graph=0, **sched)
+    h.Sync()
+    kt = h.kernel_times()
+    h.set_kernel_timing(False)
+    px = P * n * n
+    kern = {}
+    for name, bpp in (("column", COL_BYTES_PER_PX), ("row", ROW_BYTES_PER_PX)):
+        ms, cnt = kt[name]
+        avg = ms / max(cnt, 1)
+        ach = px * bpp / (avg * 1e-3) / 1e9
+        kern[name] = {"avg_ms": avg, "launches": cnt, "bytes_per_launch": px * bpp,
+                      "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS}
+    sw_ms, sw_n = kt["shrinkwrap"]
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"])
+    roof = {"bound": "hbm", "achieved": round(kern[dom]["achieved_GBs"], 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(kern[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": None, "kernel": f"k_eng_{'cols' if dom == 'column' else 'rows_hio'}<256>",
+            "kernels": kern,
+            "shrinkwrap_update_ms": sw_ms / max(sw_n, 1)}
+    iter_algo_frac = px * HIO_ALGO_BYTES_PER_PX * its / 1e9 / HBM_PEAK_GBS
+    h.close()
+
+    result = {
+        "metric": METRIC,
+        "value": round(its, 2),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"batched 2D shrinkwrap-HIO, {P} x {n}x{n} complex64, "
+                               f"beta 0.9, shrinkwrap every 20 (sigma 3.0->1.5, thr 0.1)",
+                   "patterns": P, "size": n, "graph": True},
+        "pattern_iters_per_s": round(its * P, 1),
+        "hbm_frac_algorithmic_21Bpx": round(iter_algo_frac, 4),
+        "roofline": roof,
+    }
+
+    if not args.no_fp64_check:
+        import __graft_entry__ as ge2
+
+        o = ge2.load_oracle()
+        k = 2
+        xs = hip.to_device(data["x0"][:k])
+        ss = hip.to_device(data["support"][:k])
+        as_ = hip.to_device(data["amp"][:k])
+        with hip.HipImpl(n, n, 1, k) as hs:
+            rf = hs.hio_run(xs, as_, ss, iters=200, use_graph=1, want_rfactor=True, **sched)
+        ref, _ = o.hio(data["x0"][:k], data["amp"][:k], data["support"][:k], 200, **sched)
+        rref = o.rfactor(ref, data["amp"][:k])
+        xg = xs.numpy().astype(np.complex128)
+        result["fp64_check"] = {
+            "patterns": k, "iters": 200,
+            "rfactor_gpu": [round(float(v), 6) for v in rf],
+            "rfactor_fp64": [round(float(v), 6) for v in rref],
+            "rfactor_rel_err": float(np.max(np.abs(rf - rref) / rref)),
+            "x_rel_l2": float(np.linalg.norm(xg - ref) / np.linalg.norm(ref)),
+        }
+        log(f"bench: fp64 check {result['fp64_check']}")
+
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle)")
+        result["cpu_baseline"] = cpu_baseline_oracle(data, 16)
+        try:
+            result["cpu_baseline_mkl"] = cpu_baseline_mkl()
+        except Exception as e:  # noqa: BLE001
+            result["cpu_baseline_mkl"] = {"error": str(e)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,10 +26,13 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Wall",
+    # code object v5: loadable by the ROCm 7.0 runtime that torch wheels bundle
+    # as well as by /opt/rocm's (v6 objects are rejected by the former)
+    "-mcode-object-version=5",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["abi.hip", "fft.hip", "engine2d.hip"]
+SOURCES = ["abi.hip", "fft.hip", "engine2d.hip", "merge3d.hip"]
 
 
 def hipcc() -> str:
@@ -48,17 +51,43 @@ def _compile(src: str) -> str:
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+CPP_TESTS = {"test_hipimpl": os.path.join(ROOT, "tests", "cpp", "test_hipimpl.cpp")}
+CPP_BIN = os.path.join(ROOT, "tests", "cpp", "bin")
+
+
+def build_cpp_checks() -> list:
+    """Host-only C++ programs that exercise HipImpl<T> (include/HIP_Impl.hpp)
+    through libcpra_hip.so; built with g++ so the header is checked to need
+    no HIP headers."""
+    os.makedirs(CPP_BIN, exist_ok=True)
+    out = []
+    gxx = shutil.which("g++") or "g++"
+    for name, src in CPP_TESTS.items():
+        exe = os.path.join(CPP_BIN, name)
+        cmd = [gxx, "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+               "-o", exe, "-L", HERE, "-lcpra_hip",
+               f"-Wl,-rpath,{os.path.relpath(HERE, CPP_BIN).join(['$ORIGIN/', ''])}"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-4000:]}")
+        out.append(exe)
+    return out
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     jobs = min(len(SOURCES), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-mcode-object-version=5", "-shared", "-fPIC",
+           "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_cpp_checks()
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -5,6 +5,7 @@
 // Every entry point validates its arguments before any HIP call and maps
 // exceptions to the status codes of include/cpra_hip.h.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -30,6 +31,15 @@ template <typename F> static int guarded(cpra_hip_ctx* h, F&& f)
 }
 
 size_t dtype_size(int dtype) { return dtype == CPRA_HIP_F64 ? 8 : 4; }
+
+bool debug_launches()
+{
+    static const bool on = [] {
+        const char* v = std::getenv("CPRA_HIP_DEBUG");
+        return v && v[0] && v[0] != '0';
+    }();
+    return on;
+}
 
 const void* twiddle_table(cpra_hip_ctx* h, int n, bool dbl)
 {
@@ -410,7 +420,7 @@ void* cpra_hip_get_stream(cpra_hip_handle h) { return h ? (void*)h->stream : nul
 int cpra_hip_malloc(cpra_hip_handle h, void** ptr, uint64_t bytes, int managed)
 {
     return guarded(h, [&]() -> int {
-        CPRA_ARG(h && ptr, "cpra_hip_malloc: NULL argument");
+        CPRA_ARG(ptr, "cpra_hip_malloc: NULL argument");
         *ptr = nullptr;
         if (bytes == 0) return CPRA_HIP_FALSE;
         if (managed) CPRA_CHECK_HIP(hipMallocManaged(ptr, bytes));
@@ -422,7 +432,6 @@ int cpra_hip_malloc(cpra_hip_handle h, void** ptr, uint64_t bytes, int managed)
 int cpra_hip_free(cpra_hip_handle h, void* ptr)
 {
     return guarded(h, [&]() -> int {
-        CPRA_ARG(h, "cpra_hip_free: NULL handle");
         if (!ptr) return CPRA_HIP_FALSE;
         CPRA_CHECK_HIP(hipFree(ptr));
         return CPRA_HIP_OK;
@@ -432,10 +441,10 @@ int cpra_hip_free(cpra_hip_handle h, void* ptr)
 int cpra_hip_memcpy(cpra_hip_handle h, void* dst, const void* src, uint64_t bytes)
 {
     return guarded(h, [&]() -> int {
-        CPRA_ARG(h, "cpra_hip_memcpy: NULL handle");
         if (bytes == 0) return CPRA_HIP_OK;
         CPRA_ARG(dst && src, "cpra_hip_memcpy: NULL pointer");
-        CPRA_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, h->stream));
+        if (h) CPRA_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, h->stream));
+        else CPRA_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
         return CPRA_HIP_OK;
     });
 }

@@ -86,6 +86,10 @@ struct cpra_hip_ctx {
     // tuning choice per (kind, n) set by the autotuner (0 = default)
     std::map<uint64_t, int> tuning;
     cpra::EngineGraph* graph = nullptr;
+    // per-kernel event timing of the engines (eager launches only)
+    bool timing = false;
+    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace cpra {
@@ -103,10 +107,20 @@ inline unsigned grid_1d(uint64_t work, unsigned block, unsigned cap = 1u << 20)
     return (unsigned)g;
 }
 
+// CPRA_HIP_DEBUG=1 in the environment: name every launch on stderr and
+// synchronise after it, so a failing kernel is identified by name.
+bool debug_launches();
+
 inline void check_launch(const char* what)
 {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(CPRA_HIP_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    if (debug_launches()) {
+        std::fprintf(stderr, "[cpra_hip] %s\n", what);
+        std::fflush(stderr);
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) fail(CPRA_HIP_ERR_HIP, "%s failed at run time: %s", what, hipGetErrorString(e));
+    }
 }
 
 // FFT pass entry points (fft.hip).  `lines` transforms of length n; contig:

@@ -502,13 +502,57 @@ static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, co
     check_launch("k_rf_fin");
 }
 
+// Event pair around one engine kernel when the handle's timing is on; the
+// elapsed time is read back after the run (kernel_timer_collect).
+struct KernelTimer {
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    void begin(cpra_hip_ctx* h, hipStream_t st, int kind)
+    {
+        if (!h->timing) return;
+        Rec r{kind, nullptr, nullptr};
+        CPRA_CHECK_HIP(hipEventCreate(&r.a));
+        CPRA_CHECK_HIP(hipEventCreate(&r.b));
+        CPRA_CHECK_HIP(hipEventRecord(r.a, st));
+        recs.push_back(r);
+    }
+    void end(cpra_hip_ctx* h, hipStream_t st)
+    {
+        if (!h->timing) return;
+        CPRA_CHECK_HIP(hipEventRecord(recs.back().b, st));
+    }
+    void collect(cpra_hip_ctx* h)
+    {
+        for (auto& r : recs) {
+            float ms = 0.f;
+            CPRA_CHECK_HIP(hipEventSynchronize(r.b));
+            CPRA_CHECK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            h->kt_ms[r.kind] += ms;
+            h->kt_n[r.kind] += 1;
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        recs.clear();
+    }
+};
+
+enum KernelKind { KT_COL = 0, KT_ROW = 1, KT_SW = 2 };
+
 static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
-                          int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p)
+                          int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p,
+                          KernelTimer* kt = nullptr)
 {
     const Geo g = geo_of(h);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
+    if (kt) kt->begin(h, st, KT_COL);
     launch_cols(st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr);
+    if (kt) kt->end(h, st);
+    if (kt) kt->begin(h, st, KT_ROW);
     launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction);
+    if (kt) kt->end(h, st);
 }
 
 static hipStream_t aux_stream(cpra_hip_ctx* h)
@@ -582,7 +626,8 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         fft_pass_contig(h, x, g.N, g.B * g.M, -1, true, 1.0, false);
         const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
         int done = 0;
-        bool use_graph = p->use_graph && p->iters >= chunk && chunk > 0;
+        bool use_graph = p->use_graph && !h->timing && p->iters >= chunk && chunk > 0;
+        KernelTimer kt;
         hipStream_t gst = st;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (use_graph) {
@@ -623,7 +668,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 CPRA_CHECK_HIP(hipGraphLaunch(h->graph->exec, gst));
             } else {
                 for (int i = 0; i < todo; ++i)
-                    hio_iteration(h, gst, x, amp, (int64_t)amp_batch, support, p);
+                    hio_iteration(h, gst, x, amp, (int64_t)amp_batch, support, p, &kt);
             }
             done += todo;
             if (p->sw_every > 0 && todo == chunk && done % p->sw_every == 0) {
@@ -633,7 +678,9 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 // run the shrinkwrap on the same stream as the iterations
                 hipStream_t save = h->stream;
                 h->stream = gst;
+                kt.begin(h, gst, KT_SW);
                 shrinkwrap_launch(h, gst, x, true, support, sigma, p->sw_frac, nullptr);
+                kt.end(h, gst);
                 h->stream = save;
                 ++k;
             }
@@ -652,6 +699,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             CPRA_CHECK_HIP(hipMemcpyAsync(rfactor_host, rf_dev, sizeof(float) * g.B, hipMemcpyDeviceToHost, st));
             CPRA_CHECK_HIP(hipStreamSynchronize(st));
         }
+        kt.collect(h);
         return CPRA_HIP_OK;
     });
 }
@@ -687,6 +735,31 @@ int cpra_hip_rfactor(cpra_hip_handle h, const void* x, const float* amp, uint64_
         rfactor_launch(h, st, h->ws_z.ptr, amp, (int64_t)amp_batch, 1, rf_dev);
         CPRA_CHECK_HIP(hipMemcpyAsync(r_host, rf_dev, sizeof(float) * g.B, hipMemcpyDeviceToHost, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h, "cpra_hip_set_kernel_timing: NULL handle");
+        h->timing = enable != 0;
+        for (int i = 0; i < 8; ++i) {
+            h->kt_ms[i] = 0.0;
+            h->kt_n[i] = 0;
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && ms_total && counts && n >= 1 && n <= 8, "cpra_hip_get_kernel_times: bad argument");
+        for (int i = 0; i < n; ++i) {
+            ms_total[i] = h->kt_ms[i];
+            counts[i] = h->kt_n[i];
+        }
         return CPRA_HIP_OK;
     });
 }

@@ -1,0 +1,297 @@
+// merge3d.hip -- carousel 2D -> 3D merge (gather, trilinear) and re-projection.
+//
+// Geometry (cube of side n, centre c = n/2): slice p at angle th_p is an
+// [n][n] image with rows u along the rotation axis z and columns v; its sample
+// (u, v) sits at the volume point x = c + (v-c) cos th, y = c + (v-c) sin th,
+// z = u.  R (re-projection, cpra_hip_slice_extract) samples the volume
+// [z][y][x] trilinearly at those points (zero outside); since z is on the grid
+// the weights are bilinear in (x, y).  The merge is V = R^T(s) / R^T(1) where
+// R^T(1) > 0, else 0.  It replaces the reference's racy nearest-voxel scatter
+// (include/detail/CUDA_Impl_Kernel.cuh:68-119; a stub in the MKL backend,
+// include/MKL_Impl.hpp:151-152) with a gather: every voxel column (x, y) finds
+// the (p, v, w) samples whose footprint covers it -- the same for every z --
+// and then sums w * s[p][z][v] over that list.
+//
+// Kernel shape: one wave64 per voxel column at a time; the lanes first walk
+// 64 angles each (a sample s = v - c can touch (x, y) only if
+// |s - s0| <= sqrt 2 with s0 the projection of (x - c, y - c) on the slice
+// direction, so 4 candidates per angle), compact the non-zero weights into an
+// LDS list with a ballot prefix, then walk z (lanes over z, slices transposed
+// to [p][v][z] so the reads coalesce).  A block owns XT consecutive columns at
+// one y and stages its [n][XT] output tile in LDS so the volume rows are
+// written contiguously.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace cpra {
+
+// s[p][z][v] -> sT[p][v][z], 32 x 32 LDS tiles
+template <typename V>
+__global__ __launch_bounds__(256) void k_transpose_slices(const V* __restrict__ s, V* __restrict__ sT,
+                                                         int n)
+{
+    __shared__ V tile[32][33];
+    const int p = blockIdx.z;
+    const int v0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+    const V* sp = s + (int64_t)p * n * n;
+    V* tp = sT + (int64_t)p * n * n;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int z = z0 + k, v = v0 + tx;
+        if (z < n && v < n) tile[k][tx] = sp[(int64_t)z * n + v];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int v = v0 + k, z = z0 + tx;
+        if (z < n && v < n) tp[(int64_t)v * n + z] = tile[tx][k];
+    }
+}
+
+template <typename T> struct ElemOps;
+template <> struct ElemOps<float> {
+    using V = float;
+    static __device__ __forceinline__ V zero() { return 0.f; }
+    static __device__ __forceinline__ V fma(float w, V a, V acc) { return acc + w * a; }
+    static __device__ __forceinline__ V scale(V a, float s) { return a * s; }
+};
+template <> struct ElemOps<double> {
+    using V = double;
+    static __device__ __forceinline__ V zero() { return 0.0; }
+    static __device__ __forceinline__ V fma(double w, V a, V acc) { return acc + w * a; }
+    static __device__ __forceinline__ V scale(V a, double s) { return a * s; }
+};
+template <typename T> struct ElemOps<c2<T>> {
+    using V = c2<T>;
+    static __device__ __forceinline__ V zero() { return mk<V>(0, 0); }
+    static __device__ __forceinline__ V fma(T w, V a, V acc) { return mk<V>(acc.x + w * a.x, acc.y + w * a.y); }
+    static __device__ __forceinline__ V scale(V a, T s) { return mk<V>(a.x * s, a.y * s); }
+};
+
+constexpr int kMergeMaxZPerLane = 8;  // n <= 512
+
+// E: element type (T or c2<T>), T: real type of the weights/angles.
+template <typename E, typename T>
+__global__ __launch_bounds__(256) void k_merge_gather(const E* __restrict__ sT, const T* __restrict__ angles,
+                                                     int P, int n, int XT, E* __restrict__ vol)
+{
+    using Ops = ElemOps<E>;
+    extern __shared__ unsigned char smem[];
+    // per wave: list of up to 256 (p*n + v, w) entries; then the output tile
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int* lidx = reinterpret_cast<int*>(smem) + wave * 256;
+    T* lw = reinterpret_cast<T*>(smem + 4 * 256 * sizeof(int)) + wave * 256;
+    E* tile = reinterpret_cast<E*>(smem + 4 * 256 * (sizeof(int) + sizeof(T)));  // [n][XT]
+    T* tw = reinterpret_cast<T*>(tile + (int64_t)n * XT);                       // [XT]
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * XT;
+    const T c = (T)(n / 2);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int xl = wave; xl < XT; xl += 4) {
+        const int x = x0 + xl;
+        const T dx = (T)x - c, dy = (T)y - c;
+        E acc[kMergeMaxZPerLane];
+#pragma unroll
+        for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
+        T wsum = 0;
+        for (int p0 = 0; p0 < P; p0 += 64) {
+            const int p = p0 + lane;
+            T cs = 0, sn = 0;
+            bool live = p < P;
+            if (live) {
+                const T th = angles[p];
+                cs = cos(th);
+                sn = sin(th);
+                const T dperp = -dx * sn + dy * cs;
+                live = fabs(dperp) < (T)1.4142135623730951;
+            }
+            const T s0 = dx * cs + dy * sn;
+            const int sb = (int)floor(s0) - 1;
+            int cnt_base = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int s = sb + k;
+                const int v = s + n / 2;
+                T w = 0;
+                if (live && v >= 0 && v < n) {
+                    const T ex = (T)s * cs - dx, ey = (T)s * sn - dy;
+                    const T wx = (T)1 - fabs(ex), wy = (T)1 - fabs(ey);
+                    if (wx > (T)0 && wy > (T)0) w = wx * wy;
+                }
+                const bool has = w > (T)0;
+                const uint64_t m = __ballot(has);
+                if (has) {
+                    const int pos = cnt_base + __popcll(m & lt);
+                    lidx[pos] = p * n + v;
+                    lw[pos] = w;
+                }
+                cnt_base += __popcll(m);
+            }
+            // the list is written and read by this wave only
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int e = 0; e < cnt_base; ++e) {
+                const int pv = lidx[e];
+                const T w = lw[e];
+                wsum += w;
+                const E* col = sT + (int64_t)pv * n;
+#pragma unroll
+                for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+                    const int z = lane + 64 * k;
+                    if (z < n) acc[k] = Ops::fma(w, col[z], acc[k]);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const T inv = wsum > (T)0 ? (T)1 / wsum : (T)0;
+#pragma unroll
+        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+            const int z = lane + 64 * k;
+            if (z < n) tile[(int64_t)z * XT + xl] = Ops::scale(acc[k], inv);
+        }
+        if (lane == 0) tw[xl] = wsum;
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < (int64_t)n * XT; i += blockDim.x) {
+        const int z = (int)(i / XT), xl = (int)(i % XT);
+        vol[((int64_t)z * n + y) * n + x0 + xl] = tile[i];
+    }
+}
+
+// R: slices[p][u][v] = bilinear sample of the plane z = u at the slice point.
+template <typename E, typename T>
+__global__ __launch_bounds__(256) void k_slice_extract(const E* __restrict__ vol, const T* __restrict__ angles,
+                                                      int P, int n, E* __restrict__ out)
+{
+    using Ops = ElemOps<E>;
+    const int64_t total = (int64_t)P * n * n;
+    const T c = (T)(n / 2);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int v = (int)(i % n);
+        const int u = (int)((i / n) % n);
+        const int p = (int)(i / ((int64_t)n * n));
+        const T th = angles[p];
+        const T px = c + ((T)v - c) * cos(th), py = c + ((T)v - c) * sin(th);
+        const T fx = floor(px), fy = floor(py);
+        const T ax = px - fx, ay = py - fy;
+        E acc = Ops::zero();
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dxx = 0; dxx < 2; ++dxx) {
+                const int xi = (int)fx + dxx, yi = (int)fy + dy;
+                const T w = (dxx ? ax : (T)1 - ax) * (dy ? ay : (T)1 - ay);
+                if (xi >= 0 && xi < n && yi >= 0 && yi < n && w != (T)0)
+                    acc = Ops::fma(w, vol[((int64_t)u * n + yi) * n + xi], acc);
+            }
+        out[i] = acc;
+    }
+}
+
+template <typename E, typename T>
+static void merge_launch(cpra_hip_ctx* h, const void* slices, void* vol, const void* angles, int n,
+                         int P)
+{
+    const size_t per = (size_t)P * n * n;
+    h->ws_misc.ensure(per * sizeof(E));
+    E* sT = static_cast<E*>(h->ws_misc.ptr);
+    hipLaunchKernelGGL((k_transpose_slices<E>), dim3((n + 31) / 32, (n + 31) / 32, P), dim3(256), 0,
+                       h->stream, static_cast<const E*>(slices), sT, n);
+    check_launch("k_transpose_slices");
+    int XT = n < 64 ? n : 64;
+    auto lds_of = [&](int xt) {
+        return 4 * 256 * (sizeof(int) + sizeof(T)) + (size_t)n * xt * sizeof(E) + xt * sizeof(T);
+    };
+    while (XT > 1 && lds_of(XT) > 96 * 1024) XT /= 2;
+    if (n % XT) fail(CPRA_HIP_ERR_UNSUPPORTED, "merge: n=%d not a multiple of the tile %d", n, XT);
+    const size_t lds = lds_of(XT);
+    hipLaunchKernelGGL((k_merge_gather<E, T>), dim3(n / XT, n), dim3(256), lds, h->stream, sT,
+                       static_cast<const T*>(angles), P, n, XT, static_cast<E*>(vol));
+    check_launch("k_merge_gather");
+}
+
+template <typename E, typename T>
+static void extract_launch(cpra_hip_ctx* h, const void* vol, void* slices, const void* angles, int n,
+                           int P)
+{
+    const int64_t total = (int64_t)P * n * n;
+    hipLaunchKernelGGL((k_slice_extract<E, T>), dim3(grid_1d(total, 256, 1 << 16)), dim3(256), 0,
+                       h->stream, static_cast<const E*>(vol), static_cast<const T*>(angles), P, n,
+                       static_cast<E*>(slices));
+    check_launch("k_slice_extract");
+}
+
+static void check_merge_geometry(uint64_t m, uint64_t n, uint64_t p, uint64_t l, const char* who)
+{
+    CPRA_ARG(p >= 1, "%s: p must be >= 1", who);
+    if (!(m == n && n == l))
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: the carousel geometry needs a cube (m == n == l)", who);
+    if (n < 2 || n > 64 * kMergeMaxZPerLane || (n & 1))
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: n must be even and <= %d", who, 64 * kMergeMaxZPerLane);
+}
+
+template <typename F> static int mguard(cpra_hip_ctx* h, F&& f)
+{
+    try {
+        return f();
+    } catch (const Status& s) {
+        if (h) h->last_error = s.msg;
+        return s.code;
+    } catch (const std::exception& e) {
+        if (h) h->last_error = e.what();
+        return CPRA_HIP_ERR_HIP;
+    }
+}
+
+}  // namespace cpra
+
+using namespace cpra;
+
+extern "C" {
+
+int cpra_hip_real2d_to_3d(cpra_hip_handle h, const void* slices, void* vol, const void* angles,
+                          uint64_t m, uint64_t n, uint64_t p, uint64_t l)
+{
+    return mguard(h, [&]() -> int {
+        CPRA_ARG(h && slices && vol && angles, "cpra_hip_real2d_to_3d: NULL argument");
+        check_merge_geometry(m, n, p, l, "cpra_hip_real2d_to_3d");
+        if (h->dtype == CPRA_HIP_F32) merge_launch<float, float>(h, slices, vol, angles, (int)n, (int)p);
+        else merge_launch<double, double>(h, slices, vol, angles, (int)n, (int)p);
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_complex2d_to_3d(cpra_hip_handle h, const void* slices, void* vol, const void* angles,
+                             uint64_t m, uint64_t n, uint64_t p, uint64_t l)
+{
+    return mguard(h, [&]() -> int {
+        CPRA_ARG(h && slices && vol && angles, "cpra_hip_complex2d_to_3d: NULL argument");
+        check_merge_geometry(m, n, p, l, "cpra_hip_complex2d_to_3d");
+        if (h->dtype == CPRA_HIP_F32) merge_launch<c2<float>, float>(h, slices, vol, angles, (int)n, (int)p);
+        else merge_launch<c2<double>, double>(h, slices, vol, angles, (int)n, (int)p);
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_slice_extract(cpra_hip_handle h, const void* vol, void* slices, const void* angles,
+                           uint64_t n, uint64_t p, int complex_data)
+{
+    return mguard(h, [&]() -> int {
+        CPRA_ARG(h && slices && vol && angles, "cpra_hip_slice_extract: NULL argument");
+        check_merge_geometry(n, n, p, n, "cpra_hip_slice_extract");
+        if (h->dtype == CPRA_HIP_F32) {
+            if (complex_data) extract_launch<c2<float>, float>(h, vol, slices, angles, (int)n, (int)p);
+            else extract_launch<float, float>(h, vol, slices, angles, (int)n, (int)p);
+        } else {
+            if (complex_data) extract_launch<c2<double>, double>(h, vol, slices, angles, (int)n, (int)p);
+            else extract_launch<double, double>(h, vol, slices, angles, (int)n, (int)p);
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+}  // extern "C"

@@ -8,8 +8,10 @@ reference would throw or abort on raise ``CpraHipError``.
 
 Arguments that the C++ interface takes as pointers accept either a torch
 tensor (its ``data_ptr()`` is passed; the tensor must live on the GPU) or a
-raw integer device address.  The handle launches on torch's current stream
-by default so that work is ordered with torch operations.
+raw integer device address, or a ``DeviceArray`` (device memory from the
+library's own allocator, used by the tests and benchmarks so that no second
+HIP runtime is loaded).  With ``use_torch_stream=True`` the handle launches
+on torch's current stream so that work is ordered with torch operations.
 
 The library must be built (``build.py``); nothing here falls back to a CPU
 implementation.
@@ -95,6 +97,12 @@ _SIGS = {
     "cpra_hip_shrinkwrap": (C.c_int, [_H, _P, _P, C.c_float, C.c_float, _P]),
     "cpra_hip_rfactor": (C.c_int, [_H, _P, _P, _U64, _P]),
     "cpra_hip_support_from_constraint": (C.c_int, [_H, _P, _P, _U64]),
+    "cpra_hip_real2d_to_3d": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
+    "cpra_hip_complex2d_to_3d": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
+    "cpra_hip_slice_extract": (C.c_int, [_H, _P, _P, _P, _U64, _U64, C.c_int]),
+    "cpra_hip_set_kernel_timing": (C.c_int, [_H, C.c_int]),
+    "cpra_hip_get_kernel_times": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                            C.c_int]),
 }
 
 
@@ -118,11 +126,87 @@ def lib() -> C.CDLL:
     return _lib
 
 
+class DeviceArray:
+    """Device buffer owned through the library's own allocator
+    (cpra_hip_malloc), with numpy upload/download.  Lets tests and benchmarks
+    run without importing torch, so only one HIP runtime lives in the
+    process."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+
+        self.shape = tuple(int(s) for s in (shape if hasattr(shape, "__len__") else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        rc = lib().cpra_hip_malloc(None, C.byref(p), max(self.nbytes, 1), 0)
+        if rc < 0:
+            raise CpraHipError(rc, lib().cpra_hip_last_error(None).decode())
+        self.ptr = p.value
+
+    @property
+    def size(self):
+        import numpy as np
+
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def numel(self):
+        return self.size
+
+    def data_ptr(self):
+        return self.ptr
+
+    def upload(self, a):
+        import numpy as np
+
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        rc = lib().cpra_hip_memcpy(None, self.ptr, a.ctypes.data, self.nbytes)
+        if rc < 0:
+            raise CpraHipError(rc, lib().cpra_hip_last_error(None).decode())
+        return self
+
+    def numpy(self):
+        import numpy as np
+
+        out = np.empty(self.shape, dtype=self.dtype)
+        rc = lib().cpra_hip_memcpy(None, out.ctypes.data, self.ptr, self.nbytes)
+        if rc < 0:
+            raise CpraHipError(rc, lib().cpra_hip_last_error(None).decode())
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cpra_hip_free(None, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_device(a) -> "DeviceArray":
+    import numpy as np
+
+    a = np.ascontiguousarray(a)
+    return DeviceArray(a.shape, a.dtype).upload(a)
+
+
+def zeros(shape, dtype) -> "DeviceArray":
+    import numpy as np
+
+    return to_device(np.zeros(shape, dtype=dtype))
+
+
 def _addr(x) -> Optional[int]:
     if x is None:
         return None
     if isinstance(x, int):
         return x
+    if isinstance(x, DeviceArray):
+        return x.ptr
     if hasattr(x, "data_ptr"):
         if hasattr(x, "is_cuda") and not x.is_cuda:
             raise CpraHipError(ERR_ARG, "tensor must be on the GPU")
@@ -147,7 +231,7 @@ class HipImpl:
     ``NewCpraImpl<T>(IMPL_TYPE::HIP, m, n, l, batch_size)``."""
 
     def __init__(self, m: int, n: int, l: int = 1, batch_size: int = 1, dtype: str = "float32",
-                 stream=None, use_torch_stream: bool = True):
+                 stream=None, use_torch_stream: bool = False):
         self.m, self.n, self.l, self.batch = int(m), int(n), int(l), int(batch_size)
         self.dtype = dtype
         code = F64 if dtype in ("float64", "double") else F32
@@ -159,12 +243,9 @@ class HipImpl:
         if stream is not None:
             self.set_stream(stream)
         elif use_torch_stream:
-            try:
-                import torch
-                if torch.cuda.is_available():
-                    self.set_stream(torch.cuda.current_stream().cuda_stream)
-            except ImportError:
-                pass
+            import torch
+
+            self.set_stream(torch.cuda.current_stream().cuda_stream)
 
     # -- plumbing ---------------------------------------------------------
     def _check(self, rc: int) -> bool:
@@ -239,6 +320,19 @@ class HipImpl:
     def Normalization(self, x, norm: float, num: int) -> bool:
         return self._check(lib().cpra_hip_normalize(self._h, _addr(x), norm, num))
 
+    def Real2DTo3DInterpolation(self, slices, vol, angles, m: int, n: int, p: int, l: int) -> bool:
+        return self._check(lib().cpra_hip_real2d_to_3d(self._h, _addr(slices), _addr(vol),
+                                                       _addr(angles), m, n, p, l))
+
+    def Complex2DTo3DInterpolation(self, slices, vol, angles, m: int, n: int, p: int,
+                                   l: int) -> bool:
+        return self._check(lib().cpra_hip_complex2d_to_3d(self._h, _addr(slices), _addr(vol),
+                                                          _addr(angles), m, n, p, l))
+
+    def slice_extract(self, vol, slices, angles, n: int, p: int, complex_data=True) -> bool:
+        return self._check(lib().cpra_hip_slice_extract(self._h, _addr(vol), _addr(slices),
+                                                        _addr(angles), n, p, int(complex_data)))
+
     def ConvergeError(self, old, new, m: int, n: int, l: int = 1, batch_size: int = 1):
         """Returns (ok, errors) with errors a numpy array of batch_size."""
         import numpy as np
@@ -275,6 +369,16 @@ class HipImpl:
         r = np.zeros(self.batch, dtype=np.float32)
         self._check(lib().cpra_hip_rfactor(self._h, _addr(x), _addr(amp), ab, r.ctypes.data))
         return r
+
+    def set_kernel_timing(self, enable: bool) -> bool:
+        return self._check(lib().cpra_hip_set_kernel_timing(self._h, int(enable)))
+
+    def kernel_times(self):
+        """{kind: (total_ms, launches)} for kind in column/row/shrinkwrap."""
+        ms = (C.c_double * 3)()
+        cnt = (C.c_int64 * 3)()
+        self._check(lib().cpra_hip_get_kernel_times(self._h, ms, cnt, 3))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(("column", "row", "shrinkwrap"))}
 
     def support_from_constraint(self, S, mask, num: int) -> bool:
         return self._check(lib().cpra_hip_support_from_constraint(self._h, _addr(S), _addr(mask),

@@ -66,11 +66,13 @@ int cpra_hip_set_stream(cpra_hip_handle h, void* stream);
 void* cpra_hip_get_stream(cpra_hip_handle h);
 
 /* Replaces CudaAllocator / CudaManagedAllocator::allocate / deallocate
- * (include/Allocator.hpp:18-52). managed != 0 -> hipMallocManaged. */
+ * (include/Allocator.hpp:18-52). managed != 0 -> hipMallocManaged.  h may be
+ * NULL (errors then go to the thread's last error). */
 int cpra_hip_malloc(cpra_hip_handle h, void** ptr, uint64_t bytes, int managed);
 int cpra_hip_free(cpra_hip_handle h, void* ptr);
 /* Replaces CudaImpl::Memcpy (include/CUDA_Impl.cuh:93-98): asynchronous,
- * direction inferred (hipMemcpyDefault), on the handle's stream. */
+ * direction inferred (hipMemcpyDefault), on the handle's stream; with
+ * h == NULL a synchronous copy on the null stream. */
 int cpra_hip_memcpy(cpra_hip_handle h, void* dst, const void* src, uint64_t bytes);
 /* Replaces CudaImpl::Sync (include/CUDA_Impl.cuh:99-103). */
 int cpra_hip_sync(cpra_hip_handle h);
@@ -130,6 +132,28 @@ int cpra_hip_normalize(cpra_hip_handle h, void* x, double norm, uint64_t num);
  * host_err is host memory with `batch` entries of T. Synchronises. */
 int cpra_hip_converge_error(cpra_hip_handle h, void* old_, const void* new_, void* host_err,
                             uint64_t m, uint64_t n, uint64_t l, uint64_t batch);
+
+/* ---- carousel 2D -> 3D merge (include/CPRA_Impl.hpp:54-55) --------------- */
+
+/* Gather-trilinear merge of p slices [p][m][n] (rows u along the rotation
+ * axis, columns v) into the volume [l][m][n] = [z][y][x] of a cube
+ * (m == n == l, even, <= 512): slice p at angle angles[p] (device array of T,
+ * radians) samples the point x = c + (v-c) cos th, y = c + (v-c) sin th,
+ * z = u with c = n/2; the volume is R^T(slices) / R^T(1) where R^T(1) > 0 and
+ * 0 elsewhere, R being trilinear sampling.  Replaces Real2DTo3DInterpolation
+ * (CUDA include/detail/CUDA_Impl.inl:174-197, a racy nearest-voxel scatter;
+ * a stub in the MKL backend, include/MKL_Impl.hpp:151). */
+int cpra_hip_real2d_to_3d(cpra_hip_handle h, const void* slices, void* vol, const void* angles,
+                          uint64_t m, uint64_t n, uint64_t p, uint64_t l);
+/* The same for complex<T> slices and volume (a stub returning true in both
+ * reference backends, include/MKL_Impl.hpp:152, include/CUDA_Impl.cuh:89). */
+int cpra_hip_complex2d_to_3d(cpra_hip_handle h, const void* slices, void* vol, const void* angles,
+                             uint64_t m, uint64_t n, uint64_t p, uint64_t l);
+/* Re-projection R: slices[p][u][v] = trilinear sample of the cube volume
+ * [n][n][n] at the slice points above (0 outside).  complex_data selects
+ * complex<T> or T elements.  No reference counterpart (north-star addition). */
+int cpra_hip_slice_extract(cpra_hip_handle h, const void* vol, void* slices, const void* angles,
+                           uint64_t n, uint64_t p, int complex_data);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,14 @@ int cpra_hip_rfactor(cpra_hip_handle h, const void* x, const float* amp, uint64_
 int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t* mask,
                                      uint64_t num);
 
+/* Per-kernel timing of the engines: when enabled, engine runs launch
+ * eagerly (no graph) with a hipEvent pair around every fused kernel on the
+ * stream it runs on; cpra_hip_get_kernel_times returns, per kernel kind
+ * (0 = column pass, 1 = row pass, 2 = one shrinkwrap update), the summed
+ * milliseconds and the launch count since the last enable. */
+int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable);
+int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,10 +10,6 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def to_dev(torch, a):
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
-
-
 def relerr(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
@@ -28,15 +24,14 @@ def synth(pkg):
 @pytest.mark.parametrize("n,P,starts", [(64, 2, 2), (256, 3, 1), (128, 1, 4), (192, 2, 1),
                                         (384, 1, 1), (512, 1, 1)])
 @pytest.mark.parametrize("rp", [1, 0])
-def test_hio_steps_match_oracle(hipmod, oracle, torch_gpu, synth, n, P, starts, rp):
-    torch = torch_gpu
+def test_hio_steps_match_oracle(hipmod, oracle, gpu, synth, n, P, starts, rp):
     data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, starts_per_pattern=starts,
                                config=70 + n)
     B = P * starts
     for iters in (1, 3):
-        x = to_dev(torch, data["x0"])
-        sup = to_dev(torch, data["support"])
-        amp = to_dev(torch, data["amp"])
+        x = hipmod.to_device(data["x0"])
+        sup = hipmod.to_device(data["support"])
+        amp = hipmod.to_device(data["amp"])
         with hipmod.HipImpl(n, n, 1, B) as h:
             h.hio_run(x, amp, sup, iters=iters, beta=0.9, real_positive=rp, dc_skip_zero=1,
                       sw_every=0, use_graph=0)
@@ -44,39 +39,37 @@ def test_hio_steps_match_oracle(hipmod, oracle, torch_gpu, synth, n, P, starts, 
         ref = data["x0"].astype(np.complex128)
         for _ in range(iters):
             ref = oracle.hio_step(ref, data["amp"], data["support"], 0.9, bool(rp), True)
-        assert relerr(x.cpu().numpy(), ref) < 1e-5 * iters, (n, iters)
+        assert relerr(x.numpy(), ref) < 1e-5 * iters, (n, iters)
 
 
-def test_error_reduction_mode(hipmod, oracle, torch_gpu, synth):
-    torch = torch_gpu
+def test_error_reduction_mode(hipmod, oracle, gpu, synth):
     data = synth.make_2d_batch(2, 128, 15, 20, 1e6, config=81)
-    x = to_dev(torch, data["x0"])
+    x = hipmod.to_device(data["x0"])
     with hipmod.HipImpl(128, 128, 1, 2) as h:
-        h.hio_run(x, to_dev(torch, data["amp"]), to_dev(torch, data["support"]), iters=1,
+        h.hio_run(x, hipmod.to_device(data["amp"]), hipmod.to_device(data["support"]), iters=1,
                   beta=0.9, real_positive=0, error_reduction=1, sw_every=0, use_graph=0)
         h.Sync()
     # ER = P_S P_M: complex mode keeps x' inside the support, zero outside
     pm = oracle.hio_step(data["x0"], data["amp"], np.ones_like(data["support"]), 0.0, False, True)
     ref = np.where(data["support"] > 0, pm, 0)
-    assert relerr(x.cpu().numpy(), ref) < 1e-5
+    assert relerr(x.numpy(), ref) < 1e-5
 
 
 @pytest.mark.parametrize("n,B", [(64, 3), (256, 2)])
-def test_shrinkwrap_matches_oracle(hipmod, oracle, torch_gpu, synth, n, B):
-    torch = torch_gpu
+def test_shrinkwrap_matches_oracle(hipmod, oracle, gpu, synth, n, B):
     data = synth.make_2d_batch(B, n, n * 0.12, n * 0.18, 1e6, config=90 + n)
     obj = data["obj"].astype(np.complex64)
     for sigma, frac in [(3.0, 0.1), (1.5, 0.2)]:
-        x = to_dev(torch, obj)
-        sup = torch.zeros((B, n, n), dtype=torch.uint8, device="cuda")
-        blur = torch.zeros((B, n, n), dtype=torch.float32, device="cuda")
+        x = hipmod.to_device(obj)
+        sup = hipmod.zeros((B, n, n), np.uint8)
+        blur = hipmod.zeros((B, n, n), np.float32)
         with hipmod.HipImpl(n, n, 1, B) as h:
             assert h.shrinkwrap(x, sup, sigma, frac, blur)
             h.Sync()
         s_ref, b_ref = oracle.shrinkwrap(obj, sigma, frac, want_blur=True)
-        b = blur.cpu().numpy()
+        b = blur.numpy()
         assert relerr(b, b_ref) < 1e-5
-        got = sup.cpu().numpy()
+        got = sup.numpy()
         thr = frac * b_ref.max(axis=(1, 2), keepdims=True)
         near = np.abs(b_ref - thr) <= 1e-5 * b_ref.max(axis=(1, 2), keepdims=True)
         assert np.array_equal(got[~near], s_ref[~near])
@@ -86,79 +79,73 @@ def test_shrinkwrap_matches_oracle(hipmod, oracle, torch_gpu, synth, n, B):
 
 
 @pytest.mark.parametrize("n,B", [(64, 4), (256, 2), (384, 1)])
-def test_rfactor_matches_oracle(hipmod, oracle, torch_gpu, synth, n, B):
-    torch = torch_gpu
+def test_rfactor_matches_oracle(hipmod, oracle, gpu, synth, n, B):
     data = synth.make_2d_batch(B, n, n * 0.12, n * 0.18, 1e6, config=100 + n)
     with hipmod.HipImpl(n, n, 1, B) as h:
-        r = h.rfactor(to_dev(torch, data["x0"]), to_dev(torch, data["amp"]))
-        r_obj = h.rfactor(to_dev(torch, data["obj"].astype(np.complex64)),
-                          to_dev(torch, data["amp"]))
+        r = h.rfactor(hipmod.to_device(data["x0"]), hipmod.to_device(data["amp"]))
+        r_obj = h.rfactor(hipmod.to_device(data["obj"].astype(np.complex64)),
+                          hipmod.to_device(data["amp"]))
     assert np.allclose(r, oracle.rfactor(data["x0"], data["amp"]), rtol=1e-4)
     assert np.allclose(r_obj, oracle.rfactor(data["obj"], data["amp"]), rtol=1e-3)
 
 
-def test_support_from_constraint_bit_exact(hipmod, torch_gpu):
-    torch = torch_gpu
+def test_support_from_constraint_bit_exact(hipmod, gpu):
     S = np.array([0.9, np.nextafter(np.float32(0.9), np.float32(1)), np.nan, 0, 1, 0.5],
                  dtype=np.float32)
-    m = torch.zeros(6, dtype=torch.uint8, device="cuda")
+    m = hipmod.zeros(6, np.uint8)
     with hipmod.HipImpl(1, 1, 1, 1) as h:
-        assert h.support_from_constraint(to_dev(torch, S), m, 6)
+        assert h.support_from_constraint(hipmod.to_device(S), m, 6)
         h.Sync()
-    assert m.cpu().numpy().tolist() == [0, 1, 1, 0, 1, 0]
+    assert m.numpy().tolist() == [0, 1, 1, 0, 1, 0]
 
 
-def test_hio_shrinkwrap_schedule_matches_oracle(hipmod, oracle, torch_gpu, synth):
-    torch = torch_gpu
+def test_hio_shrinkwrap_schedule_matches_oracle(hipmod, oracle, gpu, synth):
     n, P = 64, 3
     data = synth.make_2d_batch(P, n, 8, 12, 1e6, config=110)
     kw = dict(iters=40, beta=0.9, real_positive=1, dc_skip_zero=1, sw_every=10, sw_sigma0=3.0,
               sw_sigma1=1.5, sw_frac=0.1)
-    x = to_dev(torch, data["x0"])
-    sup = to_dev(torch, data["support"])
-    amp = to_dev(torch, data["amp"])
+    x = hipmod.to_device(data["x0"])
+    sup = hipmod.to_device(data["support"])
+    amp = hipmod.to_device(data["amp"])
     with hipmod.HipImpl(n, n, 1, P) as h:
         rf = h.hio_run(x, amp, sup, want_rfactor=True, use_graph=1, **kw)
     ref, ref_sup = oracle.hio(data["x0"], data["amp"], data["support"], **kw)
     rref = oracle.rfactor(ref, data["amp"])
     assert np.allclose(rf, rref, rtol=1e-3), (rf, rref)
-    assert np.allclose(oracle.rfactor(x.cpu().numpy(), data["amp"]), rref, rtol=1e-3)
+    assert np.allclose(oracle.rfactor(x.numpy(), data["amp"]), rref, rtol=1e-3)
 
 
-def test_graph_replays_match_eager(hipmod, torch_gpu, synth):
-    torch = torch_gpu
+def test_graph_replays_match_eager(hipmod, gpu, synth):
     n, P = 128, 4
     data = synth.make_2d_batch(P, n, 15, 20, 1e6, config=120)
     kw = dict(iters=12, beta=0.9, real_positive=1, dc_skip_zero=1, sw_every=4)
     outs = []
     for use_graph in (0, 1, 1):
-        x = to_dev(torch, data["x0"])
-        sup = to_dev(torch, data["support"])
-        amp = to_dev(torch, data["amp"])
+        x = hipmod.to_device(data["x0"])
+        sup = hipmod.to_device(data["support"])
+        amp = hipmod.to_device(data["amp"])
         with hipmod.HipImpl(n, n, 1, P) as h:
             h.hio_run(x, amp, sup, use_graph=use_graph, **kw)
             # a second run on the same handle replays the cached graph
             h.hio_run(x, amp, sup, use_graph=use_graph, **kw)
             h.Sync()
-        torch.cuda.synchronize()
-        outs.append((x.cpu().numpy(), sup.cpu().numpy()))
+        outs.append((x.numpy(), sup.numpy()))
     for x, s in outs[1:]:
         assert np.array_equal(x, outs[0][0])
         assert np.array_equal(s, outs[0][1])
 
 
-def test_engine_rejects_unsupported(hipmod, torch_gpu):
-    torch = torch_gpu
-    x = torch.zeros((100, 100), dtype=torch.complex64, device="cuda")
-    amp = torch.zeros((100, 100), dtype=torch.float32, device="cuda")
-    sup = torch.zeros((100, 100), dtype=torch.uint8, device="cuda")
+def test_engine_rejects_unsupported(hipmod, gpu):
+    x = hipmod.zeros((100, 100), np.complex64)
+    amp = hipmod.zeros((100, 100), np.float32)
+    sup = hipmod.zeros((100, 100), np.uint8)
     with hipmod.HipImpl(100, 100, 1, 1) as h:
         with pytest.raises(hipmod.CpraHipError) as e:
             h.hio_run(x, amp, sup, iters=1)
         assert e.value.code == hipmod.ERR_UNSUPPORTED
     with hipmod.HipImpl(64, 64, 1, 3) as h:
-        x = torch.zeros((3, 64, 64), dtype=torch.complex64, device="cuda")
+        x = hipmod.zeros((3, 64, 64), np.complex64)
         with pytest.raises(hipmod.CpraHipError) as e:
-            h.hio_run(x, torch.zeros((2, 64, 64), device="cuda"),
-                      torch.zeros((3, 64, 64), dtype=torch.uint8, device="cuda"), iters=1)
+            h.hio_run(x, hipmod.zeros((2, 64, 64), np.float32),
+                      hipmod.zeros((3, 64, 64), np.uint8), iters=1)
         assert e.value.code == hipmod.ERR_ARG

@@ -8,63 +8,56 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def to_dev(torch, a):
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
-
-
 def relerr(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
 # ------------------------------------------------------------------ reference tests
 
-def test_reference_initialize_range(hipmod, torch_gpu):
-    torch = torch_gpu
+def test_reference_initialize_range(hipmod, gpu):
     with hipmod.HipImpl(1, 1, 1, 1) as h:
-        x = torch.empty(1000, dtype=torch.float32, device="cuda")
+        x = hipmod.zeros(1000, np.float32)
         assert h.Initialize(x, 1000, seed=5)
         h.Sync()
-        v = x.cpu().numpy()
+        v = x.numpy()
         assert ((v >= 0) & (v <= 1)).all() and v.std() > 0.2
         assert h.Initialize(x, 0) is False
 
 
-def test_reference_norm_merge(hipmod, torch_gpu):
+def test_reference_norm_merge(hipmod, gpu):
     # test/unitest_mkl.cpp:15-40 and test/unitest_cuda.cu:14-41
-    torch = torch_gpu
     with hipmod.HipImpl(1, 1, 1, 1) as h:
-        x = torch.full((1000,), 10 + 10j, dtype=torch.complex64, device="cuda")
+        x = hipmod.to_device(np.full(1000, 10 + 10j, dtype=np.complex64))
         assert h.MergeAddData(x, x, 0.5, 0.5, 1000)
         h.Sync()
-        assert (x.cpu() == 10 + 10j).all()
+        assert (x.numpy() == 10 + 10j).all()
         assert h.Normalization(x, 10.0, 1000)
         h.Sync()
-        assert (x.cpu() == 1 + 1j).all()
-        y = torch.full((1000,), 10 + 10j, dtype=torch.complex64, device="cuda")
+        assert (x.numpy() == 1 + 1j).all()
+        y = hipmod.to_device(np.full(1000, 10 + 10j, dtype=np.complex64))
         assert h.MergeAddData(y, y, 1.0, 1.0, 1000)
         assert h.Normalization(y, 20.0, 1000)
         h.Sync()
-        assert (y.cpu() == 1 + 1j).all()
+        assert (y.numpy() == 1 + 1j).all()
         assert h.MergeAddData(y, x, 0.0, 2.0, 1000) is False  # alpha == 0: no-op
         h.Sync()
-        assert (x.cpu() == 1 + 1j).all()
+        assert (x.numpy() == 1 + 1j).all()
 
 
-def test_reference_2d_dft_known_answer_double(hipmod, torch_gpu):
+def test_reference_2d_dft_known_answer_double(hipmod, gpu):
     # test/unitest_mkl.cpp:42-71 (double 10x10; generic-length FFT path)
-    torch = torch_gpu
     with hipmod.HipImpl(10, 10, 1, 1, dtype="float64") as h:
-        x = torch.ones((10, 10), dtype=torch.complex128, device="cuda")
+        x = hipmod.to_device(np.ones((10, 10), dtype=np.complex128))
         assert h.Forward2D(x)
         h.Sync()
-        y = x.cpu().numpy()
+        y = x.numpy()
         expect = np.zeros((10, 10))
         expect[5, 5] = 100
         assert np.abs(y.real - expect).max() < 1e-13
         assert np.abs(y.imag).max() < 1e-13
         assert h.Backward2D(x)
         h.Sync()
-        z = x.cpu().numpy()
+        z = x.numpy()
         assert np.abs(z.real - 1).max() < 1e-14 and np.abs(z.imag).max() < 1e-14
 
 
@@ -72,31 +65,28 @@ def test_reference_2d_dft_known_answer_double(hipmod, torch_gpu):
 
 @pytest.mark.parametrize("M,N,B", [(256, 256, 3), (10, 14, 2), (128, 64, 1)])
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
-def test_fftshift2d_bit_exact(hipmod, oracle, torch_gpu, M, N, B, dtype):
-    torch = torch_gpu
+def test_fftshift2d_bit_exact(hipmod, oracle, gpu, M, N, B, dtype):
     cd = np.complex64 if dtype == "float32" else np.complex128
     x = (np.random.default_rng(M + N).standard_normal((B, M, N)) * (1 + 1j)).astype(cd)
-    d = to_dev(torch, x)
+    d = hipmod.to_device(x)
     with hipmod.HipImpl(M, N, 1, B, dtype=dtype) as h:
         assert h.FFTShift2D(d, M, N, B)
         h.Sync()
-    assert np.array_equal(d.cpu().numpy(), oracle.fftshift2d(x, M, N, B))
+    assert np.array_equal(d.numpy(), oracle.fftshift2d(x, M, N, B))
 
 
 @pytest.mark.parametrize("L,M,N,B", [(8, 6, 4, 2), (16, 16, 16, 1)])
-def test_fftshift3d_bit_exact(hipmod, oracle, torch_gpu, L, M, N, B):
-    torch = torch_gpu
+def test_fftshift3d_bit_exact(hipmod, oracle, gpu, L, M, N, B):
     x = np.random.default_rng(1).standard_normal((B, L, M, N)).astype(np.complex64)
-    d = to_dev(torch, x)
+    d = hipmod.to_device(x)
     with hipmod.HipImpl(M, N, L, B) as h:
         assert h.FFTShift3D(d, M, N, L, B)
         h.Sync()
-    assert np.array_equal(d.cpu().numpy(), oracle.fftshift3d(x, M, N, L, B))
+    assert np.array_equal(d.numpy(), oracle.fftshift3d(x, M, N, L, B))
 
 
-def test_odd_shift_rejected(hipmod, torch_gpu):
-    torch = torch_gpu
-    d = torch.zeros(9 * 8, dtype=torch.complex64, device="cuda")
+def test_odd_shift_rejected(hipmod, gpu):
+    d = hipmod.zeros(9 * 8, np.complex64)
     with hipmod.HipImpl(9, 8, 1, 1) as h:
         with pytest.raises(hipmod.CpraHipError) as e:
             h.FFTShift2D(d, 9, 8, 1)
@@ -105,8 +95,7 @@ def test_odd_shift_rejected(hipmod, torch_gpu):
 
 # ------------------------------------------------------------------ pointwise
 
-def test_space_constraint_bit_exact(hipmod, oracle, torch_gpu):
-    torch = torch_gpu
+def test_space_constraint_bit_exact(hipmod, oracle, gpu):
     rng = np.random.default_rng(2)
     per, B = 4096, 3
     x = (rng.standard_normal(per * B) + 1j * rng.standard_normal(per * B)).astype(np.complex64)
@@ -114,16 +103,15 @@ def test_space_constraint_bit_exact(hipmod, oracle, torch_gpu):
     S[:4] = [0.9, np.nextafter(np.float32(0.9), np.float32(1)), np.nan, 0.0]
     for mode, ref in [(hipmod.SPACE_MKL, oracle.space_constraint_f32),
                       (hipmod.SPACE_POSITIVE, oracle.space_constraint_pos_f32)]:
-        d = to_dev(torch, x)
+        d = hipmod.to_device(x)
         with hipmod.HipImpl(1, 1, 1, 1) as h:
-            assert h.SpaceConstraint(d, to_dev(torch, S), per * B, B, mode)
+            assert h.SpaceConstraint(d, hipmod.to_device(S), per * B, B, mode)
             h.Sync()
         e, _ = ref(x, S, B)
-        assert np.array_equal(d.cpu().numpy().view(np.uint32), e.view(np.uint32)), mode
+        assert np.array_equal(d.numpy().view(np.uint32), e.view(np.uint32)), mode
 
 
-def test_data_constraints(hipmod, oracle, torch_gpu):
-    torch = torch_gpu
+def test_data_constraints(hipmod, oracle, gpu):
     rng = np.random.default_rng(3)
     per, B = 5000, 2
     x = (rng.standard_normal(per * B) + 1j * rng.standard_normal(per * B)).astype(np.complex64)
@@ -131,34 +119,33 @@ def test_data_constraints(hipmod, oracle, torch_gpu):
     A = np.abs(rng.standard_normal(per)).astype(np.float32)
     A[5:50] = 0
     for skip in (True, False):
-        d = to_dev(torch, x)
+        d = hipmod.to_device(x)
         with hipmod.HipImpl(1, 1, 1, 1) as h:
-            assert h.DataConstraint(d, to_dev(torch, A), per * B, B, skip_zero=skip)
+            assert h.DataConstraint(d, hipmod.to_device(A), per * B, B, skip_zero=skip)
             h.Sync()
         e, _ = oracle.data_constraint_f32(x, A, B, skip_zero=skip)
-        assert relerr(d.cpu().numpy(), e) < 1e-6
+        assert relerr(d.numpy(), e) < 1e-6
     Cc = (rng.standard_normal(per) + 1j * rng.standard_normal(per)).astype(np.complex64)
     Cc[::7] = 0
-    d = to_dev(torch, x)
+    d = hipmod.to_device(x)
     with hipmod.HipImpl(1, 1, 1, 1) as h:
-        assert h.DataConstraint(d, to_dev(torch, Cc), per * B, B)
+        assert h.DataConstraint(d, hipmod.to_device(Cc), per * B, B)
         h.Sync()
     e, _ = oracle.data_constraint_cplx_f32(x, Cc, B)
-    assert np.array_equal(d.cpu().numpy(), e)
+    assert np.array_equal(d.numpy(), e)
 
 
-def test_converge_error(hipmod, oracle, torch_gpu):
-    torch = torch_gpu
+def test_converge_error(hipmod, oracle, gpu):
     rng = np.random.default_rng(4)
     m, n, l, B = 32, 16, 2, 3
     old = (rng.standard_normal((B, l, m, n)) + 1j * rng.standard_normal((B, l, m, n))).astype(np.complex64)
     new = (rng.standard_normal((B, l, m, n)) + 1j * rng.standard_normal((B, l, m, n))).astype(np.complex64)
-    do, dn = to_dev(torch, old), to_dev(torch, new)
+    do, dn = hipmod.to_device(old), hipmod.to_device(new)
     with hipmod.HipImpl(m, n, l, B) as h:
         ok, err = h.ConvergeError(do, dn, m, n, l, B)
     e, diff = oracle.converge_error(old, new, m, n, l, B)
     assert ok and np.allclose(err, e, rtol=1e-5)
-    assert relerr(do.cpu().numpy(), diff) < 1e-6
+    assert relerr(do.numpy(), diff) < 1e-6
 
 
 # ------------------------------------------------------------------ FFTs
@@ -169,82 +156,78 @@ FFT2_CASES = [(64, 64, 4), (128, 128, 2), (192, 192, 2), (256, 256, 3), (384, 38
 
 
 @pytest.mark.parametrize("M,N,B", FFT2_CASES)
-def test_forward_backward_2d_f32(hipmod, oracle, torch_gpu, M, N, B):
-    torch = torch_gpu
+def test_forward_backward_2d_f32(hipmod, oracle, gpu, M, N, B):
     rng = np.random.default_rng(M * 7 + N)
     x = (rng.standard_normal((B, M, N)) + 1j * rng.standard_normal((B, M, N))).astype(np.complex64)
-    d = to_dev(torch, x)
+    d = hipmod.to_device(x)
     with hipmod.HipImpl(M, N, 1, B) as h:
         assert h.Forward2D(d)
         h.Sync()
-        y = d.cpu().numpy()
+        y = d.numpy()
         assert relerr(y, oracle.forward2d(x)) < 1e-5, "forward"
         assert h.Backward2D(d)
         h.Sync()
-    assert relerr(d.cpu().numpy(), x.astype(np.complex128)) < 1e-5, "round trip"
-    d2 = to_dev(torch, x)
+    assert relerr(d.numpy(), x.astype(np.complex128)) < 1e-5, "round trip"
+    d2 = hipmod.to_device(x)
     with hipmod.HipImpl(M, N, 1, B) as h:
         h.Backward2D(d2)
         h.Sync()
-    assert relerr(d2.cpu().numpy(), oracle.backward2d(x)) < 1e-5, "backward"
+    assert relerr(d2.numpy(), oracle.backward2d(x)) < 1e-5, "backward"
 
 
 @pytest.mark.parametrize("M,N,B", [(64, 64, 2), (256, 256, 1), (10, 10, 1), (128, 192, 1)])
-def test_forward_backward_2d_f64(hipmod, oracle, torch_gpu, M, N, B):
-    torch = torch_gpu
+def test_forward_backward_2d_f64(hipmod, oracle, gpu, M, N, B):
     rng = np.random.default_rng(M + 3 * N)
     x = rng.standard_normal((B, M, N)) + 1j * rng.standard_normal((B, M, N))
-    d = to_dev(torch, x)
+    d = hipmod.to_device(x)
     with hipmod.HipImpl(M, N, 1, B, dtype="float64") as h:
         assert h.Forward2D(d)
         h.Sync()
-        assert relerr(d.cpu().numpy(), oracle.forward2d(x)) < 1e-13
+        assert relerr(d.numpy(), oracle.forward2d(x)) < 1e-13
         assert h.Backward2D(d)
         h.Sync()
-    assert relerr(d.cpu().numpy(), x) < 1e-13
+    assert relerr(d.numpy(), x) < 1e-13
 
 
 @pytest.mark.parametrize("L,M,N,B", [(64, 64, 64, 1), (128, 128, 128, 1), (16, 16, 16, 2),
                                      (32, 64, 128, 1)])
-def test_forward_backward_3d_f32(hipmod, oracle, torch_gpu, L, M, N, B):
-    torch = torch_gpu
+def test_forward_backward_3d_f32(hipmod, oracle, gpu, L, M, N, B):
     rng = np.random.default_rng(L + M + N)
     x = (rng.standard_normal((B, L, M, N)) + 1j * rng.standard_normal((B, L, M, N))).astype(np.complex64)
-    d = to_dev(torch, x)
+    d = hipmod.to_device(x)
     with hipmod.HipImpl(M, N, L, B) as h:
         assert h.Forward3D(d)
         h.Sync()
-        assert relerr(d.cpu().numpy(), oracle.forward3d(x)) < 1e-5
+        assert relerr(d.numpy(), oracle.forward3d(x)) < 1e-5
         assert h.Backward3D(d)
         h.Sync()
-    assert relerr(d.cpu().numpy(), x.astype(np.complex128)) < 1e-5
+    assert relerr(d.numpy(), x.astype(np.complex128)) < 1e-5
 
 
-def test_fft_known_answers_f32(hipmod, torch_gpu):
-    torch = torch_gpu
+def test_fft_known_answers_f32(hipmod, gpu):
     M = N = 256
     with hipmod.HipImpl(M, N, 1, 1) as h:
         # constant -> impulse of M*N at the centre (DC at (M/2, N/2))
-        d = torch.ones((M, N), dtype=torch.complex64, device="cuda")
+        d = hipmod.to_device(np.ones((M, N), dtype=np.complex64))
         h.Forward2D(d)
         h.Sync()
-        y = d.cpu().numpy()
+        y = d.numpy()
         assert abs(y[M // 2, N // 2] - M * N) < 1e-2
         y[M // 2, N // 2] = 0
         assert np.abs(y).max() < 1e-2
         # centred impulse -> constant 1
-        d = torch.zeros((M, N), dtype=torch.complex64, device="cuda")
+        d = hipmod.zeros((M, N), np.complex64)
         d[M // 2, N // 2] = 1
         h.Forward2D(d)
         h.Sync()
-        assert np.abs(d.cpu().numpy() - 1).max() < 1e-6
+        assert np.abs(d.numpy() - 1).max() < 1e-6
         # single frequency: cos along columns -> two impulses at N/2 +- k
         k = 5
         c = np.arange(N)
         row = np.cos(2 * np.pi * k * (c - N // 2) / N).astype(np.complex64)
-        d = torch.from_numpy(np.tile(row, (M, 1))).cuda()
+        d = hipmod.to_device(np.tile(row, (M, 1)))
         h.Forward2D(d)
         h.Sync()
-        y = d.cpu().numpy()
+        y = d.numpy()
         assert abs(y[M // 2, N // 2 + k] - M * N / 2) < 0.05
         assert abs(y[M // 2, N // 2 - k] - M * N / 2) < 0.05
