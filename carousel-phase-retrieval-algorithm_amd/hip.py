@@ -153,6 +153,9 @@ class DeviceArray:
     def numel(self):
         return self.size
 
+    def is_complex(self):
+        return self.dtype.kind == "c"
+
     def data_ptr(self):
         return self.ptr
 

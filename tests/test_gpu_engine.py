@@ -21,25 +21,39 @@ def synth(pkg):
     return synth
 
 
+def boundary_pixels(oracle, x, data, rp):
+    """Pixels where an HIO decision (support and, in real-positive mode,
+    Re P_M x >= 0) lies within FP32 rounding of its threshold."""
+    pm = oracle.hio_step(x, data["amp"], np.ones_like(data["support"]), 0.0, False, True)
+    if not rp:
+        return np.zeros(pm.shape, dtype=bool)
+    scale = np.abs(pm).max(axis=(-2, -1), keepdims=True)
+    return (data["support"] > 0) & (np.abs(pm.real) < 1e-5 * scale)
+
+
 @pytest.mark.parametrize("n,P,starts", [(64, 2, 2), (256, 3, 1), (128, 1, 4), (192, 2, 1),
                                         (384, 1, 1), (512, 1, 1)])
 @pytest.mark.parametrize("rp", [1, 0])
 def test_hio_steps_match_oracle(hipmod, oracle, gpu, synth, n, P, starts, rp):
+    """Each GPU iteration against the FP64 oracle's iteration from the same
+    (GPU) state: 1e-5 relative L2 outside decision-boundary pixels."""
     data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, starts_per_pattern=starts,
                                config=70 + n)
     B = P * starts
-    for iters in (1, 3):
-        x = hipmod.to_device(data["x0"])
-        sup = hipmod.to_device(data["support"])
-        amp = hipmod.to_device(data["amp"])
-        with hipmod.HipImpl(n, n, 1, B) as h:
-            h.hio_run(x, amp, sup, iters=iters, beta=0.9, real_positive=rp, dc_skip_zero=1,
+    x = hipmod.to_device(data["x0"])
+    sup = hipmod.to_device(data["support"])
+    amp = hipmod.to_device(data["amp"])
+    prev = data["x0"]
+    with hipmod.HipImpl(n, n, 1, B) as h:
+        for step in range(3):
+            h.hio_run(x, amp, sup, iters=1, beta=0.9, real_positive=rp, dc_skip_zero=1,
                       sw_every=0, use_graph=0)
             h.Sync()
-        ref = data["x0"].astype(np.complex128)
-        for _ in range(iters):
-            ref = oracle.hio_step(ref, data["amp"], data["support"], 0.9, bool(rp), True)
-        assert relerr(x.numpy(), ref) < 1e-5 * iters, (n, iters)
+            got = x.numpy()
+            ref = oracle.hio_step(prev, data["amp"], data["support"], 0.9, bool(rp), True)
+            keep = ~boundary_pixels(oracle, prev, data, rp)
+            assert relerr(got[keep], ref[keep]) < 1e-5, (n, step)
+            prev = got
 
 
 def test_error_reduction_mode(hipmod, oracle, gpu, synth):
@@ -81,12 +95,16 @@ def test_shrinkwrap_matches_oracle(hipmod, oracle, gpu, synth, n, B):
 @pytest.mark.parametrize("n,B", [(64, 4), (256, 2), (384, 1)])
 def test_rfactor_matches_oracle(hipmod, oracle, gpu, synth, n, B):
     data = synth.make_2d_batch(B, n, n * 0.12, n * 0.18, 1e6, config=100 + n)
+    rng = np.random.default_rng(n)
+    obj = data["obj"].astype(np.complex64)
+    noisy = (data["obj"] + 0.05 * rng.standard_normal(data["obj"].shape)).astype(np.complex64)
     with hipmod.HipImpl(n, n, 1, B) as h:
-        r = h.rfactor(hipmod.to_device(data["x0"]), hipmod.to_device(data["amp"]))
-        r_obj = h.rfactor(hipmod.to_device(data["obj"].astype(np.complex64)),
-                          hipmod.to_device(data["amp"]))
-    assert np.allclose(r, oracle.rfactor(data["x0"], data["amp"]), rtol=1e-4)
-    assert np.allclose(r_obj, oracle.rfactor(data["obj"], data["amp"]), rtol=1e-3)
+        r_obj = h.rfactor(hipmod.to_device(obj), hipmod.to_device(data["amp"]))
+        r_noisy = h.rfactor(hipmod.to_device(noisy), hipmod.to_device(data["amp"]))
+    # the object itself: R is set by the Poisson noise of A
+    assert np.allclose(r_obj, oracle.rfactor(obj, data["amp"]), rtol=1e-5)
+    assert np.allclose(r_noisy, oracle.rfactor(noisy, data["amp"]), rtol=1e-5)
+    assert np.all(r_noisy > r_obj)
 
 
 def test_support_from_constraint_bit_exact(hipmod, gpu):
@@ -100,6 +118,11 @@ def test_support_from_constraint_bit_exact(hipmod, gpu):
 
 
 def test_hio_shrinkwrap_schedule_matches_oracle(hipmod, oracle, gpu, synth):
+    """Full schedule (40 HIO iterations, shrinkwrap every 10, graph).  HIO with
+    positivity and shrinkwrap is dynamically unstable: the FP64 oracle run
+    against itself from inputs perturbed at FP32 rounding level (6e-8) already
+    differs by several percent in R after 40 iterations.  The GPU R-factor must
+    lie within that spread (x3) of the unperturbed FP64 run, plus 1e-3 relative."""
     n, P = 64, 3
     data = synth.make_2d_batch(P, n, 8, 12, 1e6, config=110)
     kw = dict(iters=40, beta=0.9, real_positive=1, dc_skip_zero=1, sw_every=10, sw_sigma0=3.0,
@@ -109,10 +132,18 @@ def test_hio_shrinkwrap_schedule_matches_oracle(hipmod, oracle, gpu, synth):
     amp = hipmod.to_device(data["amp"])
     with hipmod.HipImpl(n, n, 1, P) as h:
         rf = h.hio_run(x, amp, sup, want_rfactor=True, use_graph=1, **kw)
-    ref, ref_sup = oracle.hio(data["x0"], data["amp"], data["support"], **kw)
+    ref, _ = oracle.hio(data["x0"], data["amp"], data["support"], **kw)
     rref = oracle.rfactor(ref, data["amp"])
-    assert np.allclose(rf, rref, rtol=1e-3), (rf, rref)
-    assert np.allclose(oracle.rfactor(x.numpy(), data["amp"]), rref, rtol=1e-3)
+    spread = np.zeros(P)
+    for seed in (1, 2):
+        rng = np.random.default_rng(seed)
+        xp = data["x0"].astype(complex) * (1 + 6e-8 * rng.standard_normal(data["x0"].shape))
+        pert, _ = oracle.hio(xp, data["amp"], data["support"], **kw)
+        spread = np.maximum(spread, np.abs(oracle.rfactor(pert, data["amp"]) - rref))
+    tol = 3 * spread + 1e-3 * rref
+    assert np.all(np.abs(rf - rref) <= tol), (rf, rref, spread)
+    # the R-factor the engine reports is the R-factor of the x it returns
+    assert np.allclose(oracle.rfactor(x.numpy(), data["amp"]), rf, rtol=1e-4)
 
 
 def test_graph_replays_match_eager(hipmod, gpu, synth):

@@ -216,8 +216,9 @@ def test_fft_known_answers_f32(hipmod, gpu):
         y[M // 2, N // 2] = 0
         assert np.abs(y).max() < 1e-2
         # centred impulse -> constant 1
-        d = hipmod.zeros((M, N), np.complex64)
-        d[M // 2, N // 2] = 1
+        imp = np.zeros((M, N), dtype=np.complex64)
+        imp[M // 2, N // 2] = 1
+        d = hipmod.to_device(imp)
         h.Forward2D(d)
         h.Sync()
         assert np.abs(d.numpy() - 1).max() < 1e-6
