@@ -20,282 +20,16 @@
 // epilogues: |x| -> row DFT; column DFT, x Gaussian transfer, column IDFT;
 // row IDFT -> blurred image + per-pattern max; threshold -> mask.
 #include <cmath>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/cpra_hip_engine.h"
 #include "common.hpp"
 #include "plans.hpp"
+#include "engine_kernels.hpp"
 
 namespace cpra {
-
-// Float <-> order-preserving unsigned key for atomicMax on floats.
-__device__ __forceinline__ unsigned f2key(float f)
-{
-    const unsigned u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float key2f(unsigned k)
-{
-    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
-}
-
-enum ColMode { COL_MOD = 0, COL_MOD_RF = 1, COL_BLUR = 2 };
-
-// Column kernel.  Block = C consecutive columns of one pattern (N % C == 0).
-// Y and Z may alias (each block reads its columns fully before writing).
-template <int M, int MODE>
-__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
-    const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
-    float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c)
-{
-    using V = c2<float>;
-    using PL = typename SizeCfg<M>::P;
-    constexpr int E = PL::E, TT = PL::T, C = SizeCfg<M>::C;
-    __shared__ V lds[M * C];
-    __shared__ float red[2][SizeCfg<M>::TPB];
-    const int tid = threadIdx.x;
-    const int c = tid % C, t = tid / C;
-    const int64_t line = (int64_t)blockIdx.x * C + c;
-    const int64_t b = line / N;
-    const int col = (int)(line % N);
-    const int64_t base = b * (int64_t)M * N + col;
-    V v[E];
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        v[m] = Y[base + (int64_t)(t + PL::rot(m) * TT) * N];
-    });
-    LdsCol<V, C> acc{lds + c};
-    fft_line<PL, -1>(v, t, g_tw<M>(), acc);
-    float rnum = 0.f, rden = 0.f;
-    if constexpr (MODE == COL_BLUR) {
-        const float dc = (float)(col - N / 2);
-        const float gc = gcoef_c * dc * dc;
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            const int row = t + PL::rot(m) * TT;
-            const float dr = (float)(row - M / 2);
-            const float g = expf(gcoef_r * dr * dr + gc) * scale;
-            v[m] = cscale(v[m], g);
-        });
-    } else {
-        const float* Ab = A + (b % amp_batch) * (int64_t)M * N + col;
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            const int row = t + PL::rot(m) * TT;
-            const float a = Ab[(int64_t)row * N];
-            V x = v[m];
-            const float m2 = x.x * x.x + x.y * x.y;
-            if constexpr (MODE == COL_MOD_RF) {
-                if (a > 0.f) {
-                    rnum += fabsf(sqrtf(m2) - a);
-                    rden += a;
-                }
-            }
-            float s = scale;
-            if (!(skip_zero && a == 0.f) && m2 > 0.f) s = scale * a / sqrtf(m2);
-            v[m] = cscale(x, s);
-        });
-    }
-    fft_line<PL, +1>(v, t, g_tw<M>(), acc);
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        Z[base + (int64_t)(t + PL::rot(m) * TT) * N] = v[m];
-    });
-    if constexpr (MODE == COL_MOD_RF) {
-        red[0][tid] = rnum;
-        red[1][tid] = rden;
-        __syncthreads();
-        for (int s = SizeCfg<M>::TPB / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                red[0][tid] += red[0][tid + s];
-                red[1][tid] += red[1][tid + s];
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            rf_part[2 * blockIdx.x] = red[0][0];
-            rf_part[2 * blockIdx.x + 1] = red[1][0];
-        }
-    }
-}
-
-enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
-
-// Row kernel of the HIO iteration (Fienup 1982; see orc_hio in the oracle).
-template <int N, int MODE>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
-    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
-    int64_t nlines, float beta, float invN, int real_positive)
-{
-    using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
-    constexpr int SPAN = LdsRow<V, N>::span();
-    __shared__ V lds[LPB * SPAN];
-    const int tid = threadIdx.x;
-    const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    const bool valid = line < nlines;
-    const int64_t base = (valid ? line : 0) * N;
-    LdsRow<V, N> acc{lds + ll * SPAN};
-    const V* tw = g_tw<N>();
-    V xp[E], xo[E];
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        xp[m] = valid ? Z[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
-    });
-    fft_line<PL, +1>(xp, t, tw, acc);
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        xo[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
-    });
-    fft_line<PL, +1>(xo, t, tw, acc);
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const bool s = valid ? mask[base + t + PL::rot(m) * TT] != 0 : false;
-        const V p = xp[m];
-        const bool ok = s && (!real_positive || p.x >= 0.f);
-        V r;
-        if (ok) {
-            r = real_positive ? mk<V>(p.x, 0.f) : p;
-        } else if constexpr (MODE == ROW_ER) {
-            r = mk<V>(0.f, 0.f);
-        } else {
-            r = mk<V>(xo[m].x * invN - beta * p.x, xo[m].y * invN - beta * p.y);
-        }
-        xo[m] = r;
-    });
-    fft_line<PL, -1>(xo, t, tw, acc);
-    if (valid) {
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            Y[base + t + PL::rot(m) * TT] = xo[m];
-        });
-    }
-}
-
-// Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
-// read directly.
-template <int N, bool FROM_STATE>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __restrict__ Y,
-                                                           c2<float>* __restrict__ W,
-                                                           int64_t nlines, float invN)
-{
-    using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
-    constexpr int SPAN = LdsRow<V, N>::span();
-    __shared__ V lds[LPB * SPAN];
-    const int tid = threadIdx.x;
-    const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    const bool valid = line < nlines;
-    const int64_t base = (valid ? line : 0) * N;
-    LdsRow<V, N> acc{lds + ll * SPAN};
-    const V* tw = g_tw<N>();
-    V v[E];
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
-    });
-    if constexpr (FROM_STATE) fft_line<PL, +1>(v, t, tw, acc);
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const float sc = FROM_STATE ? invN : 1.f;
-        const float a = sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc;
-        v[m] = mk<V>(a, 0.f);
-    });
-    fft_line<PL, -1>(v, t, tw, acc);
-    if (valid) {
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            W[base + t + PL::rot(m) * TT] = v[m];
-        });
-    }
-}
-
-// Shrinkwrap, row stage 3: blurred image b = Re IDFT_r(W) and per-pattern max.
-template <int N>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_blurmax(const c2<float>* __restrict__ W,
-                                                               float* __restrict__ bl,
-                                                               unsigned* __restrict__ keys,
-                                                               int64_t nlines, int M)
-{
-    using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
-    constexpr int SPAN = LdsRow<V, N>::span();
-    __shared__ V lds[LPB * SPAN];
-    __shared__ unsigned bmax[LPB];
-    const int tid = threadIdx.x;
-    const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    const bool valid = line < nlines;
-    const int64_t base = (valid ? line : 0) * N;
-    LdsRow<V, N> acc{lds + ll * SPAN};
-    const V* tw = g_tw<N>();
-    V v[E];
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        v[m] = valid ? W[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
-    });
-    fft_line<PL, +1>(v, t, tw, acc);
-    float mx = -INFINITY;
-    if (valid) {
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            bl[base + t + PL::rot(m) * TT] = v[m].x;
-            mx = fmaxf(mx, v[m].x);
-        });
-    }
-    if (tid < LPB) bmax[tid] = 0u;
-    __syncthreads();
-    if (valid) atomicMax(&bmax[ll], f2key(mx));
-    __syncthreads();
-    if (tid < LPB) {
-        const int64_t ln = (int64_t)blockIdx.x * LPB + tid;
-        if (ln < nlines) atomicMax(&keys[ln / M], bmax[tid]);
-    }
-}
-
-__global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
-// mask = b >= frac * max_pattern(b)
-__global__ void k_sw_thresh(const float* __restrict__ bl, const unsigned* __restrict__ keys,
-                            uint8_t* __restrict__ mask, int64_t per, int64_t total, float frac)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const float thr = frac * key2f(keys[i / per]);
-        mask[i] = bl[i] >= thr ? 1 : 0;
-    }
-}
-
-// R-factor: sum the per-block partials of each pattern (fixed order, double).
-__global__ void k_rf_fin(const float* __restrict__ part, int blocks_per_pattern, int64_t B,
-                         float* __restrict__ rf)
-{
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    double num = 0.0, den = 0.0;
-    for (int k = 0; k < blocks_per_pattern; ++k) {
-        num += part[2 * (b * blocks_per_pattern + k)];
-        den += part[2 * (b * blocks_per_pattern + k) + 1];
-    }
-    rf[b] = den > 0.0 ? (float)(num / den) : 0.f;
-}
-
-__global__ void k_u8_from_S(const float* __restrict__ S, uint8_t* __restrict__ mask, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        mask[i] = (double)S[i] < 0.9 ? 0 : 1;
-}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -365,34 +99,65 @@ void engine_check_geometry(cpra_hip_ctx* h, const char* who)
         fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: N=%d not a multiple of the column tile %d", who, N, col_tile_rt(M));
 }
 
-template <int M>
-static void launch_cols_m(hipStream_t st, const Geo& g, int mode, const void* Y, void* Z,
-                          const float* A, int64_t amp_batch, float scale, int skip,
-                          float* rf_part, float gr, float gc)
+// Column-kernel variant chosen per handle (the autotuner may change it):
+// thread mapping (MAP_BLOCK / MAP_WAVE) and XCD-aware tile order.
+struct ColVariant {
+    int map = MAP_BLOCK;
+    int xcd = 1;
+};
+
+template <int M, int MODE>
+static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, const void* Y,
+                           void* Z, const float* A, int64_t amp_batch, float scale, int skip,
+                           float* rf_part, float gr, float gc)
 {
     const unsigned grid = (unsigned)(g.B * g.N / SizeCfg<M>::C);
+    const int xcd = (cv.xcd && grid % 8 == 0) ? 1 : 0;
     const dim3 blk(SizeCfg<M>::TPB);
     auto* y = static_cast<const c2<float>*>(Y);
     auto* z = static_cast<c2<float>*>(Z);
-    if (mode == COL_MOD)
-        hipLaunchKernelGGL((k_eng_cols<M, COL_MOD>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc);
-    else if (mode == COL_MOD_RF)
-        hipLaunchKernelGGL((k_eng_cols<M, COL_MOD_RF>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc);
+    if (cv.map == MAP_WAVE)
+        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_WAVE>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd);
     else
-        hipLaunchKernelGGL((k_eng_cols<M, COL_BLUR>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc);
+        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_BLOCK>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd);
 }
 
-static void launch_cols(hipStream_t st, const Geo& g, int mode, const void* Y, void* Z,
-                        const float* A, int64_t amp_batch, float scale, int skip, float* rf_part,
-                        float gr = 0.f, float gc = 0.f)
+template <int M>
+static void launch_cols_m(hipStream_t st, const Geo& g, const ColVariant& cv, int mode,
+                          const void* Y, void* Z, const float* A, int64_t amp_batch, float scale,
+                          int skip, float* rf_part, float gr, float gc)
 {
+    if (mode == COL_MOD)
+        launch_cols_mm<M, COL_MOD>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc);
+    else if (mode == COL_MOD_RF)
+        launch_cols_mm<M, COL_MOD_RF>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc);
+    else
+        launch_cols_mm<M, COL_BLUR>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc);
+}
+
+static ColVariant col_variant(const cpra_hip_ctx* h, int M)
+{
+    ColVariant cv;
+    auto it = h->tuning.find(((uint64_t)1 << 32) | (uint64_t)M);
+    if (it != h->tuning.end()) {
+        cv.map = it->second & 1;
+        cv.xcd = (it->second >> 1) & 1;
+    }
+    return cv;
+}
+
+static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode, const void* Y,
+                        void* Z, const float* A, int64_t amp_batch, float scale, int skip,
+                        float* rf_part, float gr = 0.f, float gc = 0.f)
+{
+    const ColVariant cv = col_variant(h, g.M);
     switch (g.M) {
-    case 64: launch_cols_m<64>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 128: launch_cols_m<128>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 192: launch_cols_m<192>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 256: launch_cols_m<256>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 384: launch_cols_m<384>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 512: launch_cols_m<512>(st, g, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 64: launch_cols_m<64>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 128: launch_cols_m<128>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 192: launch_cols_m<192>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 256: launch_cols_m<256>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 384: launch_cols_m<384>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 512: launch_cols_m<512>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
     }
     check_launch("k_eng_cols");
 }
@@ -402,14 +167,21 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
                               const uint8_t* mask, float beta, int rp, int er)
 {
     const int64_t lines = g.B * g.M;
-    const unsigned grid = (unsigned)((lines + SizeCfg<N>::LPB - 1) / SizeCfg<N>::LPB);
+    constexpr int LPB = SizeCfg<N>::LPB;
+    const unsigned grid = (unsigned)((lines + LPB - 1) / LPB);
     auto* z = static_cast<const c2<float>*>(Z);
     auto* y = static_cast<c2<float>*>(Y);
     const float invN = 1.0f / (float)N;
-    if (er)
-        hipLaunchKernelGGL((k_eng_rows_hio<N, ROW_ER>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, lines, beta, invN, rp);
-    else
-        hipLaunchKernelGGL((k_eng_rows_hio<N, ROW_HIO>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, lines, beta, invN, rp);
+    const dim3 blk(SizeCfg<N>::TPB);
+    const bool exact = lines % LPB == 0;
+#define CPRA_ROWS(MODE, EX) \
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp)
+    if (er) {
+        if (exact) CPRA_ROWS(ROW_ER, true); else CPRA_ROWS(ROW_ER, false);
+    } else {
+        if (exact) CPRA_ROWS(ROW_HIO, true); else CPRA_ROWS(ROW_HIO, false);
+    }
+#undef CPRA_ROWS
 }
 
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
@@ -481,7 +253,7 @@ static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, 
     const float gr = (float)(k / ((double)g.M * g.M));
     const float gc = (float)(k / ((double)g.N * g.N));
     const float scale = (float)(1.0 / ((double)g.M * g.N));
-    launch_cols(st, g, COL_BLUR, W, W, nullptr, 1, scale, 0, nullptr, gr, gc);
+    launch_cols(h, st, g, COL_BLUR, W, W, nullptr, 1, scale, 0, nullptr, gr, gc);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(g.B, 256)), dim3(256), 0, st, keys, 0u, g.B);
     launch_sw_rows(st, g, 3, false, W, nullptr, bl, keys);
     hipLaunchKernelGGL(k_sw_thresh, dim3(grid_1d(total, 256, 65536)), dim3(256), 0, st, bl, keys, mask, px, total, frac);
@@ -497,7 +269,7 @@ static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, co
     h->ws_part.ensure(sizeof(float) * 2 * (size_t)bpp * g.B);
     float* part = static_cast<float*>(h->ws_part.ptr);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
-    launch_cols(st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part);
+    launch_cols(h, st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part);
     hipLaunchKernelGGL(k_rf_fin, dim3(grid_1d(g.B, 64)), dim3(64), 0, st, part, bpp, g.B, rf_dev);
     check_launch("k_rf_fin");
 }
@@ -548,7 +320,7 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
     const Geo g = geo_of(h);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
     if (kt) kt->begin(h, st, KT_COL);
-    launch_cols(st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr);
+    launch_cols(h, st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr);
     if (kt) kt->end(h, st);
     if (kt) kt->begin(h, st, KT_ROW);
     launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction);
@@ -759,6 +531,99 @@ int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* coun
         for (int i = 0; i < n; ++i) {
             ms_total[i] = h->kt_ms[i];
             counts[i] = h->kt_n[i];
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+// Autotuner: times every column-kernel variant (thread mapping x XCD tile
+// order) on the caller's own data for `iters` eager HIO iterations each, keeps
+// the fastest in the handle, restores x and the support, and writes a report
+// "key=value;..." (the choice plus the average column-kernel time of every
+// candidate in microseconds).
+int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                         uint8_t* support, int iters, char* report, uint64_t report_len)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && x && amp && support, "cpra_hip_autotune_2d: NULL argument");
+        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0,
+                 "cpra_hip_autotune_2d: amp_batch must divide batch");
+        CPRA_ARG(iters >= 1 && iters <= 1000, "cpra_hip_autotune_2d: iters out of range");
+        engine_check_geometry(h, "cpra_hip_autotune_2d");
+        const Geo g = geo_of(h);
+        const int64_t total = (int64_t)g.M * g.N * g.B;
+        h->ws_backup.ensure(sizeof(c2<float>) * total + (size_t)total);
+        c2<float>* bx = static_cast<c2<float>*>(h->ws_backup.ptr);
+        uint8_t* bs = reinterpret_cast<uint8_t*>(bx + total);
+        hipStream_t st = h->stream;
+        CPRA_CHECK_HIP(hipMemcpyAsync(bx, x, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
+        CPRA_CHECK_HIP(hipMemcpyAsync(bs, support, (size_t)total, hipMemcpyDeviceToDevice, st));
+        cpra_hip_hio_params p;
+        cpra_hip_hio_default_params(&p);
+        p.iters = iters;
+        p.sw_every = 0;
+        p.use_graph = 0;
+        const uint64_t key = ((uint64_t)1 << 32) | (uint64_t)g.M;
+        std::string rep;
+        double best = 1e30;
+        int best_v = 0;
+        const bool saved_timing = h->timing;
+        for (int v = 0; v < 4; ++v) {
+            h->tuning[key] = v;
+            h->timing = true;
+            for (int i = 0; i < 8; ++i) { h->kt_ms[i] = 0; h->kt_n[i] = 0; }
+            // one untimed warm-up iteration per candidate
+            cpra_hip_hio_params w = p;
+            w.iters = 1;
+            h->timing = false;
+            int rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &w, nullptr);
+            if (rc < 0) return rc;
+            h->timing = true;
+            rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &p, nullptr);
+            if (rc < 0) return rc;
+            const double us = 1e3 * h->kt_ms[0] / (double)(h->kt_n[0] > 0 ? h->kt_n[0] : 1);
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), "col_v%d_us=%.2f;", v, us);
+            rep += buf;
+            if (us < best) {
+                best = us;
+                best_v = v;
+            }
+            CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
+        }
+        h->timing = saved_timing;
+        h->tuning[key] = best_v;
+        CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
+        CPRA_CHECK_HIP(hipStreamSynchronize(st));
+        char head[128];
+        std::snprintf(head, sizeof(head), "col_%d=%d;col_map=%s;col_xcd=%d;", g.M, best_v,
+                      (best_v & 1) ? "wave" : "block", (best_v >> 1) & 1);
+        rep = head + rep;
+        if (report && report_len) {
+            std::snprintf(report, report_len, "%s", rep.c_str());
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+// Applies "col_<M>=<variant>;" entries of a previous autotune report (other
+// keys are ignored) and writes the current choices back when out != NULL.
+int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && spec, "cpra_hip_set_tuning: NULL argument");
+        const char* p = spec;
+        while (*p) {
+            int m = 0, v = 0, used = 0;
+            if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
+                CPRA_ARG(v >= 0 && v < 4, "cpra_hip_set_tuning: variant %d out of range", v);
+                h->tuning[((uint64_t)1 << 32) | (uint64_t)m] = v;
+                p += used;
+            } else {
+                const char* semi = std::strchr(p, ';');
+                if (!semi) break;
+                p = semi + 1;
+            }
         }
         return CPRA_HIP_OK;
     });

@@ -1,0 +1,347 @@
+// engine_kernels.hpp -- device kernels of the fused 2D engine (engine2d.hip).
+//
+// Kept in a header so the tuning variants (column-thread mapping, XCD-aware
+// tile order, exact-grid specialisation) are instantiated from one source by
+// the engine and its autotuner.
+#pragma once
+#include "fft_core.hpp"
+#include "plans.hpp"
+
+namespace cpra {
+
+// Float <-> order-preserving unsigned key for atomicMax on floats.
+__device__ __forceinline__ unsigned f2key(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(unsigned k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  With remapping, XCD x runs the contiguous tile range
+// [x*G/8, (x+1)*G/8) in order, so neighbouring tiles that share cache lines
+// (the 4-byte amplitude rows of adjacent column tiles) meet in one L2.
+__device__ __forceinline__ unsigned tile_index(int xcd_remap)
+{
+    const unsigned b = blockIdx.x;
+    if (!xcd_remap) return b;
+    return (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+}
+
+enum ColMode { COL_MOD = 0, COL_MOD_RF = 1, COL_BLUR = 2 };
+enum ColMap { MAP_BLOCK = 0, MAP_WAVE = 1 };
+
+// Thread <-> (column, t) mapping of the column kernels.  A block owns C
+// consecutive columns of one pattern and has C*T = 256 threads.
+//  MAP_BLOCK: lane-consecutive columns across the whole block (a 16-column
+//             row segment is one 128-byte line per load); the column FFT's
+//             LDS exchange spans the block's waves -> workgroup barriers.
+//  MAP_WAVE:  each wave owns CW = 64/T whole columns, so the exchange stays
+//             inside the wave (no workgroup barrier) at the price of 32-byte
+//             row segments per wave load.
+template <int M, int MAP> struct ColGeom {
+    using PL = typename SizeCfg<M>::P;
+    static constexpr int T = PL::T;
+    static constexpr int C = SizeCfg<M>::C;
+    static constexpr int CW = 64 / T;
+    static_assert(MAP == MAP_BLOCK || C == 4 * CW, "MAP_WAVE needs 4 waves of CW columns");
+    __device__ static int col(int tid)
+    {
+        if constexpr (MAP == MAP_BLOCK) return tid % C;
+        else return (tid >> 6) * CW + (tid & 63) % CW;
+    }
+    __device__ static int t(int tid)
+    {
+        if constexpr (MAP == MAP_BLOCK) return tid / C;
+        else return (tid & 63) / CW;
+    }
+};
+
+// Column kernel: centred column DFT of the row-spectral state Y, then
+//  COL_MOD / COL_MOD_RF: modulus projection with A (scale 1/(MN)); RF also
+//    accumulates the R-factor sums of the incoming iterate per block;
+//  COL_BLUR: multiply by the centred Gaussian transfer function (scale 1/(MN));
+// then the inverse column DFT into Z.  Y and Z may alias (each block reads
+// its columns fully before writing them).
+template <int M, int MODE, int MAP>
+__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
+    const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
+    float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
+    int xcd_remap)
+{
+    using V = c2<float>;
+    using G = ColGeom<M, MAP>;
+    using PL = typename G::PL;
+    constexpr int E = PL::E, TT = PL::T, C = G::C;
+    __shared__ V lds[M * C];
+    __shared__ float red[2][SizeCfg<M>::TPB];
+    const int tid = threadIdx.x;
+    const int c = G::col(tid), t = G::t(tid);
+    const unsigned tile = tile_index(xcd_remap);
+    const int64_t line = (int64_t)tile * C + c;
+    const int64_t b = line / N;
+    const int col = (int)(line % N);
+    const int64_t base = b * (int64_t)M * N + col;
+    V v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = Y[base + (int64_t)(t + PL::rot(m) * TT) * N];
+    });
+    float rnum = 0.f, rden = 0.f;
+    auto body = [&](auto& acc) {
+        fft_line<PL, -1>(v, t, g_tw<M>(), acc);
+        if constexpr (MODE == COL_BLUR) {
+            const float dc = (float)(col - N / 2);
+            const float gc = gcoef_c * dc * dc;
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                const int row = t + PL::rot(m) * TT;
+                const float dr = (float)(row - M / 2);
+                const float g = expf(gcoef_r * dr * dr + gc) * scale;
+                v[m] = cscale(v[m], g);
+            });
+        } else {
+            const float* Ab = A + (b % amp_batch) * (int64_t)M * N + col;
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                const int row = t + PL::rot(m) * TT;
+                const float a = Ab[(int64_t)row * N];
+                const V x = v[m];
+                const float m2 = x.x * x.x + x.y * x.y;
+                if constexpr (MODE == COL_MOD_RF) {
+                    if (a > 0.f) {
+                        rnum += fabsf(sqrtf(m2) - a);
+                        rden += a;
+                    }
+                }
+                // x * A/|x| (the reference's polar(A, arg x), MKL_Impl.inl:102-103)
+                float s = scale;
+                if (!(skip_zero && a == 0.f) && m2 > 0.f) s = scale * a * rsqrtf(m2);
+                v[m] = cscale(x, s);
+            });
+        }
+        fft_line<PL, +1>(v, t, g_tw<M>(), acc);
+    };
+    if constexpr (MAP == MAP_BLOCK) {
+        LdsCol<V, C, false> acc{lds + c};
+        body(acc);
+    } else {
+        const int wave = tid >> 6, cw = (tid & 63) % G::CW;
+        LdsCol<V, G::CW, true> acc{lds + wave * M * G::CW + cw};
+        body(acc);
+    }
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        Z[base + (int64_t)(t + PL::rot(m) * TT) * N] = v[m];
+    });
+    if constexpr (MODE == COL_MOD_RF) {
+        red[0][tid] = rnum;
+        red[1][tid] = rden;
+        __syncthreads();
+        for (int s = SizeCfg<M>::TPB / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                red[0][tid] += red[0][tid + s];
+                red[1][tid] += red[1][tid + s];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            rf_part[2 * tile] = red[0][0];
+            rf_part[2 * tile + 1] = red[1][0];
+        }
+    }
+}
+
+enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
+
+// Row kernel of the HIO iteration (Fienup 1982; orc_hio in the oracle):
+// x' = IDFT_r(Z), x = IDFT_r(Y)/N, HIO update with the support (and
+// positivity), Y <- DFT_r(x_new).  A line's T threads are in one wave, so the
+// LDS exchanges use wave-scope ordering.  EXACT: the grid covers the lines
+// exactly (no bounds checks).
+template <int N, int MODE, bool EXACT>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
+    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
+    int64_t nlines, float beta, float invN, int real_positive)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = EXACT || line < nlines;
+    const int64_t base = (valid ? line : 0) * N;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    V xp[E], xo[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        xp[m] = valid ? Z[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+        xo[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(xp, t, tw, acc);
+    fft_line<PL, +1>(xo, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const bool s = valid && mask[base + t + PL::rot(m) * TT] != 0;
+        const V p = xp[m];
+        const bool ok = s && (!real_positive || p.x >= 0.f);
+        V r;
+        if constexpr (MODE == ROW_ER) {
+            r = ok ? (real_positive ? mk<V>(p.x, 0.f) : p) : mk<V>(0.f, 0.f);
+        } else {
+            const V o = mk<V>(xo[m].x * invN - beta * p.x, xo[m].y * invN - beta * p.y);
+            r = ok ? (real_positive ? mk<V>(p.x, 0.f) : p) : o;
+        }
+        xo[m] = r;
+    });
+    fft_line<PL, -1>(xo, t, tw, acc);
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            Y[base + t + PL::rot(m) * TT] = xo[m];
+        });
+    }
+}
+
+// Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
+// read directly.
+template <int N, bool FROM_STATE>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __restrict__ Y,
+                                                           c2<float>* __restrict__ W,
+                                                           int64_t nlines, float invN)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int64_t base = (valid ? line : 0) * N;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    V v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    if constexpr (FROM_STATE) fft_line<PL, +1>(v, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float sc = FROM_STATE ? invN : 1.f;
+        const float a = sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc;
+        v[m] = mk<V>(a, 0.f);
+    });
+    fft_line<PL, -1>(v, t, tw, acc);
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            W[base + t + PL::rot(m) * TT] = v[m];
+        });
+    }
+}
+
+// Shrinkwrap, row stage 3: blurred image b = Re IDFT_r(W) and per-pattern
+// maximum (reduced in the wave, then per block, then one atomic per block
+// when the block's lines belong to one pattern).
+template <int N>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_blurmax(const c2<float>* __restrict__ W,
+                                                               float* __restrict__ bl,
+                                                               unsigned* __restrict__ keys,
+                                                               int64_t nlines, int M)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    __shared__ unsigned bmax[LPB];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const int64_t base = (valid ? line : 0) * N;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    V v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = valid ? W[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(v, t, tw, acc);
+    float mx = -INFINITY;
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            bl[base + t + PL::rot(m) * TT] = v[m].x;
+            mx = fmaxf(mx, v[m].x);
+        });
+    }
+    // max over the line's T lanes (consecutive lanes of one wave)
+    for (int off = TT / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if (t == 0) bmax[ll] = valid ? f2key(mx) : 0u;
+    __syncthreads();
+    if (tid == 0) {
+        const int64_t first = (int64_t)blockIdx.x * LPB;
+        const int64_t last = first + LPB - 1 < nlines ? first + LPB - 1 : nlines - 1;
+        if (first / M == last / M) {
+            unsigned k = 0u;
+            for (int i = 0; i < LPB; ++i) k = bmax[i] > k ? bmax[i] : k;
+            atomicMax(&keys[first / M], k);
+        } else {
+            for (int i = 0; i < LPB && first + i < nlines; ++i)
+                atomicMax(&keys[(first + i) / M], bmax[i]);
+        }
+    }
+}
+
+__global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// mask = b >= frac * max_pattern(b)
+__global__ void k_sw_thresh(const float* __restrict__ bl, const unsigned* __restrict__ keys,
+                            uint8_t* __restrict__ mask, int64_t per, int64_t total, float frac)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float thr = frac * key2f(keys[i / per]);
+        mask[i] = bl[i] >= thr ? 1 : 0;
+    }
+}
+
+// R-factor: sum the per-block partials of each pattern (fixed order, double).
+__global__ void k_rf_fin(const float* __restrict__ part, int blocks_per_pattern, int64_t B,
+                         float* __restrict__ rf)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < blocks_per_pattern; ++k) {
+        const int64_t tile = b * blocks_per_pattern + k;
+        num += part[2 * tile];
+        den += part[2 * tile + 1];
+    }
+    rf[b] = den > 0.0 ? (float)(num / den) : 0.f;
+}
+
+__global__ void k_u8_from_S(const float* __restrict__ S, uint8_t* __restrict__ mask, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        mask[i] = (double)S[i] < 0.9 ? 0 : 1;
+}
+
+}  // namespace cpra

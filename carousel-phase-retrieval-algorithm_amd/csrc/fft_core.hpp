@@ -369,23 +369,44 @@ CPRA_HD void fft_line(V (&v)[PL::E], int t, const V* __restrict__ tw, Acc& acc)
     });
 }
 
+// Orders this wave's LDS writes before its later LDS reads (and the reads
+// before later writes).  Enough when every lane that exchanges data through
+// an LDS region belongs to one wave: a wave's LDS instructions complete in
+// issue order, so only compiler reordering has to be prevented.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS accessor for contiguous-line kernels: one region of N + N/16 values per
 // line; the +1-per-16 padding keeps the radix-16 scatter conflict-free.
-template <typename V, int N> struct LdsRow {
+// WAVE: the T threads of a line sit in one wave (64 % T == 0), so the
+// exchange needs no workgroup barrier.
+template <typename V, int N, bool WAVE = false> struct LdsRow {
     V* base;
     static constexpr int span() { return N + N / 16; }
     CPRA_HD void st(int i, V x) { base[i + (i >> 4)] = x; }
     CPRA_HD V ld(int i) const { return base[i + (i >> 4)]; }
-    __device__ __forceinline__ void sync() { __syncthreads(); }
+    __device__ __forceinline__ void sync()
+    {
+        if constexpr (WAVE) wave_lds_sync();
+        else __syncthreads();
+    }
 };
 
 // LDS accessor for strided-line kernels: element-major [N][C] so that the C
 // lanes that share an element index touch consecutive words.
-template <typename V, int C> struct LdsCol {
+template <typename V, int C, bool WAVE = false> struct LdsCol {
     V* base;  // base + line
     CPRA_HD void st(int i, V x) { base[i * C] = x; }
     CPRA_HD V ld(int i) const { return base[i * C]; }
-    __device__ __forceinline__ void sync() { __syncthreads(); }
+    __device__ __forceinline__ void sync()
+    {
+        if constexpr (WAVE) wave_lds_sync();
+        else __syncthreads();
+    }
 };
 
 }  // namespace cpra

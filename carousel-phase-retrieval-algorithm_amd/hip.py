@@ -101,6 +101,8 @@ _SIGS = {
     "cpra_hip_complex2d_to_3d": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
     "cpra_hip_slice_extract": (C.c_int, [_H, _P, _P, _P, _U64, _U64, C.c_int]),
     "cpra_hip_set_kernel_timing": (C.c_int, [_H, C.c_int]),
+    "cpra_hip_autotune_2d": (C.c_int, [_H, _P, _P, _U64, _P, C.c_int, C.c_char_p, _U64]),
+    "cpra_hip_set_tuning": (C.c_int, [_H, C.c_char_p]),
     "cpra_hip_get_kernel_times": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                             C.c_int]),
 }
@@ -372,6 +374,18 @@ class HipImpl:
         r = np.zeros(self.batch, dtype=np.float32)
         self._check(lib().cpra_hip_rfactor(self._h, _addr(x), _addr(amp), ab, r.ctypes.data))
         return r
+
+    def autotune(self, x, amp, support, iters: int = 10, amp_batch=None) -> str:
+        """Times the engine's column-kernel variants on this data, keeps the
+        fastest for this handle and returns the report string."""
+        ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n)
+        buf = C.create_string_buffer(1024)
+        self._check(lib().cpra_hip_autotune_2d(self._h, _addr(x), _addr(amp), ab,
+                                               _addr(support), iters, buf, 1024))
+        return buf.value.decode()
+
+    def set_tuning(self, spec: str) -> bool:
+        return self._check(lib().cpra_hip_set_tuning(self._h, spec.encode()))
 
     def set_kernel_timing(self, enable: bool) -> bool:
         return self._check(lib().cpra_hip_set_kernel_timing(self._h, int(enable)))

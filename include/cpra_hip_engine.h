@@ -60,6 +60,16 @@ int cpra_hip_rfactor(cpra_hip_handle h, const void* x, const float* amp, uint64_
 int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t* mask,
                                      uint64_t num);
 
+/* Autotuner (north-star addition): times each column-kernel variant
+ * (thread mapping x XCD-aware tile order) for `iters` eager HIO iterations on
+ * the caller's data, keeps the fastest for this handle, restores x and the
+ * support, and writes "col_<m>=<v>;col_map=..;col_xcd=..;col_v<k>_us=..;"
+ * into report (may be NULL). */
+int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                         uint8_t* support, int iters, char* report, uint64_t report_len);
+/* Re-applies the "col_<m>=<v>;" choices of an autotune report. */
+int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
+
 /* Per-kernel timing of the engines: when enabled, engine runs launch
  * eagerly (no graph) with a hipEvent pair around every fused kernel on the
  * stream it runs on; cpra_hip_get_kernel_times returns, per kernel kind

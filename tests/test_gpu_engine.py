@@ -180,3 +180,30 @@ def test_engine_rejects_unsupported(hipmod, gpu):
             h.hio_run(x, hipmod.zeros((2, 64, 64), np.float32),
                       hipmod.zeros((3, 64, 64), np.uint8), iters=1)
         assert e.value.code == hipmod.ERR_ARG
+
+
+def test_autotune_picks_variant_and_restores_state(hipmod, oracle, gpu, synth):
+    n, P = 128, 2
+    data = synth.make_2d_batch(P, n, 15, 20, 1e6, config=130)
+    x = hipmod.to_device(data["x0"])
+    sup = hipmod.to_device(data["support"])
+    amp = hipmod.to_device(data["amp"])
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        rep = h.autotune(x, amp, sup, iters=3)
+        assert np.array_equal(x.numpy(), data["x0"]) and np.array_equal(sup.numpy(),
+                                                                         data["support"])
+        fields = dict(kv.split("=") for kv in rep.strip(";").split(";"))
+        assert fields[f"col_{n}"] in {"0", "1", "2", "3"}
+        assert all(float(fields[f"col_v{v}_us"]) > 0 for v in range(4))
+        # every variant computes the same iteration (to FP32 rounding)
+        outs = []
+        for v in range(4):
+            assert h.set_tuning(f"col_{n}={v};")
+            xv = hipmod.to_device(data["x0"])
+            h.hio_run(xv, amp, sup, iters=2, beta=0.9, real_positive=0, sw_every=0, use_graph=0)
+            h.Sync()
+            outs.append(xv.numpy())
+        for o in outs[1:]:
+            assert np.linalg.norm(o - outs[0]) / np.linalg.norm(outs[0]) < 1e-6
+        with pytest.raises(hipmod.CpraHipError):
+            h.set_tuning(f"col_{n}=9;")
