@@ -80,16 +80,30 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
     __shared__ float red[2][SizeCfg<M>::TPB];
     const int tid = threadIdx.x;
     const int c = G::col(tid), t = G::t(tid);
+    // block-uniform pattern and first column; per-lane 32-bit offsets
     const unsigned tile = tile_index(xcd_remap);
-    const int64_t line = (int64_t)tile * C + c;
-    const int64_t b = line / N;
-    const int col = (int)(line % N);
-    const int64_t base = b * (int64_t)M * N + col;
+    const unsigned tiles_per_pattern = (unsigned)N / C;
+    const unsigned b = tile / tiles_per_pattern;
+    const int col0 = (int)(tile - b * tiles_per_pattern) * C;
+    const int col = col0 + c;
+    const size_t pat = (size_t)M * (size_t)N;
+    const c2<float>* Yp = Y + b * pat + col0;
+    c2<float>* Zp = Z + b * pat + col0;
     V v[E];
+    float av[E];
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        v[m] = Y[base + (int64_t)(t + PL::rot(m) * TT) * N];
+        v[m] = Yp[(t + PL::rot(m) * TT) * N + c];
     });
+    if constexpr (MODE != COL_BLUR) {
+        // the amplitudes are needed after the forward FFT; load them now so
+        // their latency overlaps the transform
+        const float* Ap = A + (size_t)(b % (unsigned)amp_batch) * pat + col0;
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            av[m] = Ap[(t + PL::rot(m) * TT) * N + c];
+        });
+    }
     float rnum = 0.f, rden = 0.f;
     auto body = [&](auto& acc) {
         fft_line<PL, -1>(v, t, g_tw<M>(), acc);
@@ -104,11 +118,9 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
                 v[m] = cscale(v[m], g);
             });
         } else {
-            const float* Ab = A + (b % amp_batch) * (int64_t)M * N + col;
             sfor<E>([&](auto mc) {
                 constexpr int m = decltype(mc)::value;
-                const int row = t + PL::rot(m) * TT;
-                const float a = Ab[(int64_t)row * N];
+                const float a = av[m];
                 const V x = v[m];
                 const float m2 = x.x * x.x + x.y * x.y;
                 if constexpr (MODE == COL_MOD_RF) {
@@ -135,7 +147,7 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
     }
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        Z[base + (int64_t)(t + PL::rot(m) * TT) * N] = v[m];
+        Zp[(t + PL::rot(m) * TT) * N + c] = v[m];
     });
     if constexpr (MODE == COL_MOD_RF) {
         red[0][tid] = rnum;
@@ -176,20 +188,28 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     const int t = tid % TT, ll = tid / TT;
     const int64_t line = (int64_t)blockIdx.x * LPB + ll;
     const bool valid = EXACT || line < nlines;
-    const int64_t base = (valid ? line : 0) * N;
+    // block-uniform base, per-lane 32-bit offsets
+    const size_t bbase = (size_t)blockIdx.x * LPB * N;
+    const int lo = ll * N + t;
+    const c2<float>* Zb = Z + bbase;
+    c2<float>* Yb = Y + bbase;
+    const uint8_t* Sb = mask + bbase;
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
     V xp[E], xo[E];
+    uint8_t sm[E];
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        xp[m] = valid ? Z[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
-        xo[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+        const int o = lo + PL::rot(m) * TT;
+        xp[m] = valid ? Zb[o] : mk<V>(0, 0);
+        xo[m] = valid ? Yb[o] : mk<V>(0, 0);
+        sm[m] = valid ? Sb[o] : (uint8_t)0;
     });
     fft_line<PL, +1>(xp, t, tw, acc);
     fft_line<PL, +1>(xo, t, tw, acc);
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const bool s = valid && mask[base + t + PL::rot(m) * TT] != 0;
+        const bool s = sm[m] != 0;
         const V p = xp[m];
         const bool ok = s && (!real_positive || p.x >= 0.f);
         V r;
@@ -205,7 +225,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     if (valid) {
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            Y[base + t + PL::rot(m) * TT] = xo[m];
+            Yb[lo + PL::rot(m) * TT] = xo[m];
         });
     }
 }
