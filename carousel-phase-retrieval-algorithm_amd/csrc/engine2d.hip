@@ -42,6 +42,8 @@ struct EngineGraph {
     const void* amp = nullptr;
     const void* mask = nullptr;
     const void* ws = nullptr;
+    const void* ws_rows = nullptr;
+    bool sparse = true;
     uint64_t amp_batch = 0;
     int iters = 0;
     float beta = 0.f;
@@ -106,10 +108,11 @@ struct ColVariant {
     int xcd = 1;
 };
 
-template <int M, int MODE>
+template <int M, int MODE, bool SP>
 static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, const void* Y,
                            void* Z, const float* A, int64_t amp_batch, float scale, int skip,
-                           float* rf_part, float gr, float gc)
+                           float* rf_part, float gr, float gc, const unsigned* rowflag,
+                           float beta_n, int er)
 {
     const unsigned grid = (unsigned)(g.B * g.N / SizeCfg<M>::C);
     const int xcd = (cv.xcd && grid % 8 == 0) ? 1 : 0;
@@ -117,22 +120,25 @@ static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, c
     auto* y = static_cast<const c2<float>*>(Y);
     auto* z = static_cast<c2<float>*>(Z);
     if (cv.map == MAP_WAVE)
-        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_WAVE>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd);
+        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_WAVE, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er);
     else
-        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_BLOCK>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd);
+        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_BLOCK, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er);
 }
 
 template <int M>
 static void launch_cols_m(hipStream_t st, const Geo& g, const ColVariant& cv, int mode,
                           const void* Y, void* Z, const float* A, int64_t amp_batch, float scale,
-                          int skip, float* rf_part, float gr, float gc)
+                          int skip, float* rf_part, float gr, float gc, const unsigned* rowflag,
+                          float beta_n, int er)
 {
-    if (mode == COL_MOD)
-        launch_cols_mm<M, COL_MOD>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc);
+    if (mode == COL_MOD && rowflag)
+        launch_cols_mm<M, COL_MOD, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er);
+    else if (mode == COL_MOD)
+        launch_cols_mm<M, COL_MOD, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0);
     else if (mode == COL_MOD_RF)
-        launch_cols_mm<M, COL_MOD_RF>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc);
+        launch_cols_mm<M, COL_MOD_RF, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0);
     else
-        launch_cols_mm<M, COL_BLUR>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc);
+        launch_cols_mm<M, COL_BLUR, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0);
 }
 
 static ColVariant col_variant(const cpra_hip_ctx* h, int M)
@@ -148,17 +154,20 @@ static ColVariant col_variant(const cpra_hip_ctx* h, int M)
 
 static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode, const void* Y,
                         void* Z, const float* A, int64_t amp_batch, float scale, int skip,
-                        float* rf_part, float gr = 0.f, float gc = 0.f)
+                        float* rf_part, float gr = 0.f, float gc = 0.f,
+                        const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0)
 {
     const ColVariant cv = col_variant(h, g.M);
+#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er)
     switch (g.M) {
-    case 64: launch_cols_m<64>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 128: launch_cols_m<128>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 192: launch_cols_m<192>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 256: launch_cols_m<256>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 384: launch_cols_m<384>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
-    case 512: launch_cols_m<512>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc); break;
+    case 64: CPRA_COLS(64); break;
+    case 128: CPRA_COLS(128); break;
+    case 192: CPRA_COLS(192); break;
+    case 256: CPRA_COLS(256); break;
+    case 384: CPRA_COLS(384); break;
+    case 512: CPRA_COLS(512); break;
     }
+#undef CPRA_COLS
     check_launch("k_eng_cols");
 }
 
@@ -182,6 +191,38 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
         if (exact) CPRA_ROWS(ROW_HIO, true); else CPRA_ROWS(ROW_HIO, false);
     }
 #undef CPRA_ROWS
+}
+
+template <int N>
+static void launch_rows_list_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
+                               const uint8_t* mask, const int* rows, const int* count, float beta,
+                               int rp, int er)
+{
+    constexpr int LPB = SizeCfg<N>::LPB;
+    int64_t blocks = (g.B * g.M + LPB - 1) / LPB;
+    if (blocks > 4096) blocks = 4096;
+    auto* z = static_cast<const c2<float>*>(Z);
+    auto* y = static_cast<c2<float>*>(Y);
+    const float invN = 1.0f / (float)N;
+    if (er)
+        hipLaunchKernelGGL((k_eng_rows_hio_list<N, ROW_ER>), dim3((unsigned)blocks), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, rows, count, beta, invN, rp);
+    else
+        hipLaunchKernelGGL((k_eng_rows_hio_list<N, ROW_HIO>), dim3((unsigned)blocks), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, rows, count, beta, invN, rp);
+}
+
+static void launch_rows_list(hipStream_t st, const Geo& g, const void* Z, void* Y,
+                             const uint8_t* mask, const int* rows, const int* count, float beta,
+                             int rp, int er)
+{
+    switch (g.N) {
+    case 64: launch_rows_list_n<64>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
+    case 128: launch_rows_list_n<128>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
+    case 192: launch_rows_list_n<192>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
+    case 256: launch_rows_list_n<256>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
+    case 384: launch_rows_list_n<384>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
+    case 512: launch_rows_list_n<512>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
+    }
+    check_launch("k_eng_rows_hio_list");
 }
 
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
@@ -313,17 +354,62 @@ struct KernelTimer {
 
 enum KernelKind { KT_COL = 0, KT_ROW = 1, KT_SW = 2 };
 
+// Row flags + compacted list of the rows that hold support pixels.
+static bool sparse_rows_enabled(const cpra_hip_ctx* h)
+{
+    auto it = h->tuning.find((uint64_t)2 << 32);
+    return it == h->tuning.end() || it->second != 0;
+}
+
+struct RowList {
+    unsigned* bits;  // per pattern M/32 words: bit r = row r holds support pixels
+    int* rows;       // compacted global row indices b*M + r
+    int* count;
+};
+
+static RowList row_list(cpra_hip_ctx* h)
+{
+    const Geo g = geo_of(h);
+    const size_t nrows = (size_t)g.B * g.M;
+    const size_t words = (nrows + 31) / 32;
+    h->ws_rows.ensure(4 * words + 4 * nrows + 256);
+    RowList r;
+    r.bits = static_cast<unsigned*>(h->ws_rows.ptr);
+    r.rows = reinterpret_cast<int*>(r.bits + words);
+    r.count = r.rows + nrows;
+    return r;
+}
+
+static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* mask)
+{
+    const Geo g = geo_of(h);
+    const RowList r = row_list(h);
+    const int64_t nrows = g.B * g.M;
+    const int64_t words = (nrows + 31) / 32;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(words, 256, 4096)), dim3(256), 0, st, r.bits, 0u, words);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(r.count), 0u, (int64_t)1);
+    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)((nrows + 15) / 16)), dim3(256), 0, st, mask, nrows, g.N, r.bits, r.rows, r.count);
+    check_launch("k_support_rows");
+}
+
 static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
                           int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p,
                           KernelTimer* kt = nullptr)
 {
     const Geo g = geo_of(h);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
+    const bool sparse = sparse_rows_enabled(h);
+    const RowList rl = sparse ? row_list(h) : RowList{nullptr, nullptr, nullptr};
     if (kt) kt->begin(h, st, KT_COL);
-    launch_cols(h, st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr);
+    launch_cols(h, st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr,
+                0.f, 0.f, rl.bits, p->beta * (float)g.N, p->error_reduction);
     if (kt) kt->end(h, st);
     if (kt) kt->begin(h, st, KT_ROW);
-    launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction);
+    if (sparse)
+        launch_rows_list(st, g, h->ws_z.ptr, x, mask, rl.rows, rl.count, p->beta, p->real_positive,
+                         p->error_reduction);
+    else
+        launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction);
     if (kt) kt->end(h, st);
 }
 
@@ -394,8 +480,9 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         h->ws_misc.ensure(sizeof(float) * g.B + 256);
         hipStream_t st = h->stream;
         const int nsw = p->sw_every > 0 ? p->iters / p->sw_every : 0;
-        // enter the row-spectral state in place
+        // enter the row-spectral state in place; list the rows with support
         fft_pass_contig(h, x, g.N, g.B * g.M, -1, true, 1.0, false);
+        if (sparse_rows_enabled(h)) support_rows_launch(h, st, support);
         const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
         int done = 0;
         bool use_graph = p->use_graph && !h->timing && p->iters >= chunk && chunk > 0;
@@ -410,7 +497,8 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
             EngineGraph* eg = h->graph;
             const bool same = eg && eg->x == x && eg->amp == amp && eg->mask == support &&
-                              eg->ws == h->ws_z.ptr &&
+                              eg->ws == h->ws_z.ptr && eg->ws_rows == h->ws_rows.ptr &&
+                              eg->sparse == sparse_rows_enabled(h) &&
                               eg->amp_batch == amp_batch && eg->iters == chunk &&
                               eg->beta == p->beta && eg->rp == p->real_positive &&
                               eg->skip == p->dc_skip_zero;
@@ -426,6 +514,8 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 eg->amp = amp;
                 eg->mask = support;
                 eg->ws = h->ws_z.ptr;
+                eg->ws_rows = h->ws_rows.ptr;
+                eg->sparse = sparse_rows_enabled(h);
                 eg->amp_batch = amp_batch;
                 eg->iters = chunk;
                 eg->beta = p->beta;
@@ -452,6 +542,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 h->stream = gst;
                 kt.begin(h, gst, KT_SW);
                 shrinkwrap_launch(h, gst, x, true, support, sigma, p->sw_frac, nullptr);
+                if (sparse_rows_enabled(h)) support_rows_launch(h, gst, support);
                 kt.end(h, gst);
                 h->stream = save;
                 ++k;
@@ -568,6 +659,7 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         double best = 1e30;
         int best_v = 0;
         const bool saved_timing = h->timing;
+        h->tuning[(uint64_t)2 << 32] = 0;
         for (int v = 0; v < 4; ++v) {
             h->tuning[key] = v;
             h->timing = true;
@@ -591,13 +683,39 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
             }
             CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
         }
-        h->timing = saved_timing;
         h->tuning[key] = best_v;
+        // dense rows (three row FFTs on every row) against sparse rows (rows
+        // without support pixels finished in the column kernel): whole
+        // iteration time with the chosen column variant
+        const uint64_t skey = (uint64_t)2 << 32;
+        double it_us[2] = {0, 0};
+        for (int sp = 0; sp < 2; ++sp) {
+            h->tuning[skey] = sp;
+            cpra_hip_hio_params w = p;
+            w.iters = 1;
+            h->timing = false;
+            int rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &w, nullptr);
+            if (rc < 0) return rc;
+            for (int i = 0; i < 8; ++i) { h->kt_ms[i] = 0; h->kt_n[i] = 0; }
+            h->timing = true;
+            rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &p, nullptr);
+            if (rc < 0) return rc;
+            it_us[sp] = 1e3 * (h->kt_ms[0] + h->kt_ms[1]) / (double)(h->kt_n[0] > 0 ? h->kt_n[0] : 1);
+            CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
+        }
+        const int best_sp = it_us[1] < it_us[0] ? 1 : 0;
+        h->tuning[skey] = best_sp;
+        {
+            char buf[128];
+            std::snprintf(buf, sizeof(buf), "iter_dense_us=%.2f;iter_sparse_us=%.2f;", it_us[0], it_us[1]);
+            rep += buf;
+        }
+        h->timing = saved_timing;
         CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
         char head[128];
-        std::snprintf(head, sizeof(head), "col_%d=%d;col_map=%s;col_xcd=%d;", g.M, best_v,
-                      (best_v & 1) ? "wave" : "block", (best_v >> 1) & 1);
+        std::snprintf(head, sizeof(head), "col_%d=%d;rows_sparse=%d;col_map=%s;col_xcd=%d;", g.M,
+                      best_v, best_sp, (best_v & 1) ? "wave" : "block", (best_v >> 1) & 1);
         rep = head + rep;
         if (report && report_len) {
             std::snprintf(report, report_len, "%s", rep.c_str());
@@ -615,7 +733,10 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
         const char* p = spec;
         while (*p) {
             int m = 0, v = 0, used = 0;
-            if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
+            if (std::sscanf(p, "rows_sparse=%d;%n", &v, &used) == 1 && used > 0) {
+                h->tuning[(uint64_t)2 << 32] = v ? 1 : 0;
+                p += used;
+            } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
                 CPRA_ARG(v >= 0 && v < 4, "cpra_hip_set_tuning: variant %d out of range", v);
                 h->tuning[((uint64_t)1 << 32) | (uint64_t)m] = v;
                 p += used;

@@ -66,11 +66,18 @@ template <int M, int MAP> struct ColGeom {
 //  COL_BLUR: multiply by the centred Gaussian transfer function (scale 1/(MN));
 // then the inverse column DFT into Z.  Y and Z may alias (each block reads
 // its columns fully before writing them).
-template <int M, int MODE, int MAP>
+// SPARSE (HIO iterations): rows of a pattern that hold no support pixel get
+// the linear HIO update x <- x - beta x' for every pixel, which in the
+// row-spectral domain is Y <- Y - beta*N*Z.  Those rows are finished here
+// (Y rewritten in place from a re-read of the incoming Y; error reduction
+// writes 0) and only rows with support pixels are written to Z for the row
+// kernel (rowflag[b*M + row] != 0).
+template <int M, int MODE, int MAP, bool SPARSE = false>
 __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
     const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
     float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
-    int xcd_remap)
+    int xcd_remap, const unsigned* __restrict__ rowbits = nullptr, float beta_n = 0.f,
+    int error_reduction = 0)
 {
     using V = c2<float>;
     using G = ColGeom<M, MAP>;
@@ -89,11 +96,16 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
     const size_t pat = (size_t)M * (size_t)N;
     const c2<float>* Yp = Y + b * pat + col0;
     c2<float>* Zp = Z + b * pat + col0;
+    // element m of this lane sits at row t + rot(m)*T: a wave-uniform row base
+    // (rot(m)*T*N) plus one per-lane offset (t*N + c) shared by all m, so each
+    // access is an SGPR base + one VGPR offset
+    const unsigned toff = (unsigned)t * (unsigned)N + (unsigned)c;
+    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)N; };
     V v[E];
     float av[E];
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        v[m] = Yp[(t + PL::rot(m) * TT) * N + c];
+        v[m] = (Yp + rowbase(m))[toff];
     });
     if constexpr (MODE != COL_BLUR) {
         // the amplitudes are needed after the forward FFT; load them now so
@@ -101,7 +113,7 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
         const float* Ap = A + (size_t)(b % (unsigned)amp_batch) * pat + col0;
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            av[m] = Ap[(t + PL::rot(m) * TT) * N + c];
+            av[m] = (Ap + rowbase(m))[toff];
         });
     }
     float rnum = 0.f, rden = 0.f;
@@ -145,10 +157,41 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
         LdsCol<V, G::CW, true> acc{lds + wave * M * G::CW + cw};
         body(acc);
     }
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        Zp[(t + PL::rot(m) * TT) * N + c] = v[m];
-    });
+    if constexpr (SPARSE) {
+        // row r of this pattern holds support pixels iff bit r of rowbits;
+        // rows t + rot(m)*T of one register lie in one 32-bit word for every
+        // plan (T <= 32 and T | 32), so the word index is wave-uniform
+        static_assert(TT <= 32 && 32 % TT == 0, "row bits need T | 32");
+        const unsigned* rb = rowbits + (size_t)b * (M / 32);
+        c2<float>* Yw = const_cast<c2<float>*>(Yp);
+        static_assert(E % 4 == 0, "E must be a multiple of 4");
+        sfor<E / 4>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            bool sup[4];
+            V y[4];
+            sfor<4>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int m = 4 * g + k;
+                constexpr int r0 = PL::rot(m) * TT;
+                const int row = t + r0;
+                sup[k] = (rb[r0 >> 5] >> (row & 31)) & 1u;
+                if (!sup[k] && !error_reduction) y[k] = (Yp + rowbase(m))[toff];
+            });
+            sfor<4>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int m = 4 * g + k;
+                if (sup[k]) (Zp + rowbase(m))[toff] = v[m];
+                else if (error_reduction) (Yw + rowbase(m))[toff] = mk<V>(0.f, 0.f);
+                else (Yw + rowbase(m))[toff] = mk<V>(y[k].x - beta_n * v[m].x, y[k].y - beta_n * v[m].y);
+            });
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    } else {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            (Zp + rowbase(m))[toff] = v[m];
+        });
+    }
     if constexpr (MODE == COL_MOD_RF) {
         red[0][tid] = rnum;
         red[1][tid] = rden;
@@ -227,6 +270,88 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
             constexpr int m = decltype(mc)::value;
             Yb[lo + PL::rot(m) * TT] = xo[m];
         });
+    }
+}
+
+// Row kernel over the compacted list of rows that hold support pixels
+// (k_support_rows); grid-stride so the launch shape is fixed (graph-safe)
+// while the list length lives in device memory.
+template <int N, int MODE>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio_list(
+    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
+    const int* __restrict__ rows, const int* __restrict__ count, float beta, float invN,
+    int real_positive)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int nrows = *count;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    for (int g0 = blockIdx.x * LPB; g0 < nrows; g0 += gridDim.x * LPB) {
+        const int idx = g0 + ll;
+        const bool valid = idx < nrows;
+        const size_t base = (size_t)(valid ? rows[idx] : 0) * N + t;
+        V xp[E], xo[E];
+        uint8_t sm[E];
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const size_t o = base + PL::rot(m) * TT;
+            xp[m] = valid ? Z[o] : mk<V>(0, 0);
+            xo[m] = valid ? Y[o] : mk<V>(0, 0);
+            sm[m] = valid ? mask[o] : (uint8_t)0;
+        });
+        fft_line<PL, +1>(xp, t, tw, acc);
+        fft_line<PL, +1>(xo, t, tw, acc);
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const V p = xp[m];
+            const bool ok = sm[m] != 0 && (!real_positive || p.x >= 0.f);
+            V r;
+            if constexpr (MODE == ROW_ER) {
+                r = ok ? (real_positive ? mk<V>(p.x, 0.f) : p) : mk<V>(0.f, 0.f);
+            } else {
+                const V o = mk<V>(xo[m].x * invN - beta * p.x, xo[m].y * invN - beta * p.y);
+                r = ok ? (real_positive ? mk<V>(p.x, 0.f) : p) : o;
+            }
+            xo[m] = r;
+        });
+        fft_line<PL, -1>(xo, t, tw, acc);
+        if (valid) {
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                Y[base + PL::rot(m) * TT] = xo[m];
+            });
+        }
+    }
+}
+
+// Row flags and the compacted row list of a support mask [rows][N]: 16
+// threads per row, each testing 16 bytes; rows with any pixel set are
+// appended through an atomic counter (list order does not affect results,
+// rows are independent).
+__global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict__ mask,
+                                                      int64_t nrows, int N,
+                                                      unsigned* __restrict__ rowbits,
+                                                      int* __restrict__ rows, int* __restrict__ count)
+{
+    const int lane16 = threadIdx.x & 15;
+    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    int any = 0;
+    if (row < nrows) {
+        const uint8_t* p = mask + row * N;
+        for (int i = lane16; i < N; i += 16) any |= p[i];
+    }
+    for (int off = 8; off > 0; off >>= 1) any |= __shfl_xor(any, off, 16);
+    if (lane16 == 0 && row < nrows && any) {
+        // rows of a pattern are consecutive, so global row index r maps to
+        // word r/32 bit r%32 of the concatenated per-pattern bitmasks (M % 32 == 0)
+        atomicOr(&rowbits[row >> 5], 1u << (row & 31));
+        rows[atomicAdd(count, 1)] = (int)row;
     }
 }
 

@@ -56,20 +56,24 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_fft_strided(
     __shared__ V lds[N * C];
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
-    const int64_t line = (int64_t)blockIdx.x * C + c;  // nlines % C == 0, G % C == 0
-    V* base = data + (line / G) * GS + (line % G);
+    // the tile never straddles a line group (nlines % C == 0, G % C == 0), so
+    // the block base is uniform; element m adds the uniform (pm*T*es) to one
+    // per-lane offset (c + t*es)
+    const int64_t line0 = (int64_t)blockIdx.x * C;
+    V* blk = data + (line0 / G) * GS + (line0 % G);
+    const size_t loff = (size_t)c + (size_t)t * (size_t)es;
     V v[E];
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         constexpr int pm = SHIFT ? PL::rot(m) : m;
-        v[m] = base[(int64_t)(t + pm * PL::T) * es];
+        v[m] = (blk + (size_t)(pm * PL::T) * (size_t)es)[loff];
     });
     LdsCol<V, C> acc{lds + c};
     fft_line<PL, DIR>(v, t, g_tw<N, T>(), acc);
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         constexpr int pm = SHIFT ? PL::rot(m) : m;
-        base[(int64_t)(t + pm * PL::T) * es] = do_scale ? cscale(v[m], scale) : v[m];
+        (blk + (size_t)(pm * PL::T) * (size_t)es)[loff] = do_scale ? cscale(v[m], scale) : v[m];
     });
 }
 

@@ -34,17 +34,23 @@ def boundary_pixels(oracle, x, data, rp):
 @pytest.mark.parametrize("n,P,starts", [(64, 2, 2), (256, 3, 1), (128, 1, 4), (192, 2, 1),
                                         (384, 1, 1), (512, 1, 1)])
 @pytest.mark.parametrize("rp", [1, 0])
-def test_hio_steps_match_oracle(hipmod, oracle, gpu, synth, n, P, starts, rp):
+@pytest.mark.parametrize("sparse", [1, 0])
+def test_hio_steps_match_oracle(hipmod, oracle, gpu, synth, n, P, starts, rp, sparse):
     """Each GPU iteration against the FP64 oracle's iteration from the same
-    (GPU) state: 1e-5 relative L2 outside decision-boundary pixels."""
+    (GPU) state: 1e-5 relative L2 outside decision-boundary pixels.  sparse=1
+    finishes rows without support pixels in the column kernel."""
     data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, starts_per_pattern=starts,
                                config=70 + n)
+    # a tight support (the object disc) so that many rows have no support pixel
+    if sparse:
+        data["support"] = (data["obj"][np.arange(P * starts) % P] > 0).astype(np.uint8)
     B = P * starts
     x = hipmod.to_device(data["x0"])
     sup = hipmod.to_device(data["support"])
     amp = hipmod.to_device(data["amp"])
     prev = data["x0"]
     with hipmod.HipImpl(n, n, 1, B) as h:
+        h.set_tuning(f"rows_sparse={sparse};")
         for step in range(3):
             h.hio_run(x, amp, sup, iters=1, beta=0.9, real_positive=rp, dc_skip_zero=1,
                       sw_every=0, use_graph=0)
@@ -56,10 +62,13 @@ def test_hio_steps_match_oracle(hipmod, oracle, gpu, synth, n, P, starts, rp):
             prev = got
 
 
-def test_error_reduction_mode(hipmod, oracle, gpu, synth):
+@pytest.mark.parametrize("sparse", [1, 0])
+def test_error_reduction_mode(hipmod, oracle, gpu, synth, sparse):
     data = synth.make_2d_batch(2, 128, 15, 20, 1e6, config=81)
+    data["support"] = (data["obj"] > 0).astype(np.uint8)
     x = hipmod.to_device(data["x0"])
     with hipmod.HipImpl(128, 128, 1, 2) as h:
+        h.set_tuning(f"rows_sparse={sparse};")
         h.hio_run(x, hipmod.to_device(data["amp"]), hipmod.to_device(data["support"]), iters=1,
                   beta=0.9, real_positive=0, error_reduction=1, sw_every=0, use_graph=0)
         h.Sync()
