@@ -173,7 +173,8 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
 
 template <int N>
 static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                              const uint8_t* mask, float beta, int rp, int er)
+                              const uint8_t* mask, float beta, int rp, int er,
+                              const unsigned* rowbits)
 {
     const int64_t lines = g.B * g.M;
     constexpr int LPB = SizeCfg<N>::LPB;
@@ -184,7 +185,7 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
     const dim3 blk(SizeCfg<N>::TPB);
     const bool exact = lines % LPB == 0;
 #define CPRA_ROWS(MODE, EX) \
-    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp)
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
     if (er) {
         if (exact) CPRA_ROWS(ROW_ER, true); else CPRA_ROWS(ROW_ER, false);
     } else {
@@ -193,48 +194,17 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
 #undef CPRA_ROWS
 }
 
-template <int N>
-static void launch_rows_list_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                               const uint8_t* mask, const int* rows, const int* count, float beta,
-                               int rp, int er)
-{
-    constexpr int LPB = SizeCfg<N>::LPB;
-    int64_t blocks = (g.B * g.M + LPB - 1) / LPB;
-    if (blocks > 4096) blocks = 4096;
-    auto* z = static_cast<const c2<float>*>(Z);
-    auto* y = static_cast<c2<float>*>(Y);
-    const float invN = 1.0f / (float)N;
-    if (er)
-        hipLaunchKernelGGL((k_eng_rows_hio_list<N, ROW_ER>), dim3((unsigned)blocks), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, rows, count, beta, invN, rp);
-    else
-        hipLaunchKernelGGL((k_eng_rows_hio_list<N, ROW_HIO>), dim3((unsigned)blocks), dim3(SizeCfg<N>::TPB), 0, st, z, y, mask, rows, count, beta, invN, rp);
-}
-
-static void launch_rows_list(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                             const uint8_t* mask, const int* rows, const int* count, float beta,
-                             int rp, int er)
-{
-    switch (g.N) {
-    case 64: launch_rows_list_n<64>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
-    case 128: launch_rows_list_n<128>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
-    case 192: launch_rows_list_n<192>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
-    case 256: launch_rows_list_n<256>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
-    case 384: launch_rows_list_n<384>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
-    case 512: launch_rows_list_n<512>(st, g, Z, Y, mask, rows, count, beta, rp, er); break;
-    }
-    check_launch("k_eng_rows_hio_list");
-}
-
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                            const uint8_t* mask, float beta, int rp, int er)
+                            const uint8_t* mask, float beta, int rp, int er,
+                            const unsigned* rowbits = nullptr)
 {
     switch (g.N) {
-    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er); break;
-    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er); break;
-    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er); break;
-    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er); break;
-    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er); break;
-    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er); break;
+    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
+    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
+    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
+    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
+    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
+    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
     }
     check_launch("k_eng_rows_hio");
 }
@@ -361,34 +331,23 @@ static bool sparse_rows_enabled(const cpra_hip_ctx* h)
     return it == h->tuning.end() || it->second != 0;
 }
 
-struct RowList {
-    unsigned* bits;  // per pattern M/32 words: bit r = row r holds support pixels
-    int* rows;       // compacted global row indices b*M + r
-    int* count;
-};
-
-static RowList row_list(cpra_hip_ctx* h)
+// Per pattern M/32 words: bit r set when row r holds a support pixel.
+static unsigned* row_bits(cpra_hip_ctx* h)
 {
     const Geo g = geo_of(h);
-    const size_t nrows = (size_t)g.B * g.M;
-    const size_t words = (nrows + 31) / 32;
-    h->ws_rows.ensure(4 * words + 4 * nrows + 256);
-    RowList r;
-    r.bits = static_cast<unsigned*>(h->ws_rows.ptr);
-    r.rows = reinterpret_cast<int*>(r.bits + words);
-    r.count = r.rows + nrows;
-    return r;
+    const size_t words = ((size_t)g.B * g.M + 31) / 32;
+    h->ws_rows.ensure(4 * words + 256);
+    return static_cast<unsigned*>(h->ws_rows.ptr);
 }
 
 static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* mask)
 {
     const Geo g = geo_of(h);
-    const RowList r = row_list(h);
+    unsigned* bits = row_bits(h);
     const int64_t nrows = g.B * g.M;
     const int64_t words = (nrows + 31) / 32;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(words, 256, 4096)), dim3(256), 0, st, r.bits, 0u, words);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(r.count), 0u, (int64_t)1);
-    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)((nrows + 15) / 16)), dim3(256), 0, st, mask, nrows, g.N, r.bits, r.rows, r.count);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(words, 256, 4096)), dim3(256), 0, st, bits, 0u, words);
+    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)((nrows + 15) / 16)), dim3(256), 0, st, mask, nrows, g.N, bits);
     check_launch("k_support_rows");
 }
 
@@ -399,17 +358,13 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
     const Geo g = geo_of(h);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
     const bool sparse = sparse_rows_enabled(h);
-    const RowList rl = sparse ? row_list(h) : RowList{nullptr, nullptr, nullptr};
+    const unsigned* bits = sparse ? row_bits(h) : nullptr;
     if (kt) kt->begin(h, st, KT_COL);
     launch_cols(h, st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr,
-                0.f, 0.f, rl.bits, p->beta * (float)g.N, p->error_reduction);
+                0.f, 0.f, bits, p->beta * (float)g.N, p->error_reduction);
     if (kt) kt->end(h, st);
     if (kt) kt->begin(h, st, KT_ROW);
-    if (sparse)
-        launch_rows_list(st, g, h->ws_z.ptr, x, mask, rl.rows, rl.count, p->beta, p->real_positive,
-                         p->error_reduction);
-    else
-        launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction);
+    launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction, bits);
     if (kt) kt->end(h, st);
 }
 

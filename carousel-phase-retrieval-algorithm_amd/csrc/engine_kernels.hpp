@@ -217,10 +217,14 @@ enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
 // positivity), Y <- DFT_r(x_new).  A line's T threads are in one wave, so the
 // LDS exchanges use wave-scope ordering.  EXACT: the grid covers the lines
 // exactly (no bounds checks).
+// rowbits != NULL (sparse rows): lines without a support pixel were finished
+// by the column kernel; they are skipped, and a wave whose lines are all
+// finished returns at once (the row exchange never leaves the wave).
 template <int N, int MODE, bool EXACT>
 __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
-    int64_t nlines, float beta, float invN, int real_positive)
+    int64_t nlines, float beta, float invN, int real_positive,
+    const unsigned* __restrict__ rowbits = nullptr)
 {
     using V = c2<float>;
     using PL = typename SizeCfg<N>::P;
@@ -230,7 +234,11 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
     const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    const bool valid = EXACT || line < nlines;
+    bool valid = EXACT || line < nlines;
+    if (rowbits) {
+        valid = valid && ((rowbits[line >> 5] >> (line & 31)) & 1u);
+        if (__ballot(valid) == 0ull) return;
+    }
     // block-uniform base, per-lane 32-bit offsets
     const size_t bbase = (size_t)blockIdx.x * LPB * N;
     const int lo = ll * N + t;
@@ -273,86 +281,25 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     }
 }
 
-// Row kernel over the compacted list of rows that hold support pixels
-// (k_support_rows); grid-stride so the launch shape is fixed (graph-safe)
-// while the list length lives in device memory.
-template <int N, int MODE>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio_list(
-    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
-    const int* __restrict__ rows, const int* __restrict__ count, float beta, float invN,
-    int real_positive)
-{
-    using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
-    constexpr int SPAN = LdsRow<V, N>::span();
-    __shared__ V lds[LPB * SPAN];
-    const int tid = threadIdx.x;
-    const int t = tid % TT, ll = tid / TT;
-    const int nrows = *count;
-    LdsRow<V, N, true> acc{lds + ll * SPAN};
-    const V* tw = g_tw<N>();
-    for (int g0 = blockIdx.x * LPB; g0 < nrows; g0 += gridDim.x * LPB) {
-        const int idx = g0 + ll;
-        const bool valid = idx < nrows;
-        const size_t base = (size_t)(valid ? rows[idx] : 0) * N + t;
-        V xp[E], xo[E];
-        uint8_t sm[E];
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            const size_t o = base + PL::rot(m) * TT;
-            xp[m] = valid ? Z[o] : mk<V>(0, 0);
-            xo[m] = valid ? Y[o] : mk<V>(0, 0);
-            sm[m] = valid ? mask[o] : (uint8_t)0;
-        });
-        fft_line<PL, +1>(xp, t, tw, acc);
-        fft_line<PL, +1>(xo, t, tw, acc);
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            const V p = xp[m];
-            const bool ok = sm[m] != 0 && (!real_positive || p.x >= 0.f);
-            V r;
-            if constexpr (MODE == ROW_ER) {
-                r = ok ? (real_positive ? mk<V>(p.x, 0.f) : p) : mk<V>(0.f, 0.f);
-            } else {
-                const V o = mk<V>(xo[m].x * invN - beta * p.x, xo[m].y * invN - beta * p.y);
-                r = ok ? (real_positive ? mk<V>(p.x, 0.f) : p) : o;
-            }
-            xo[m] = r;
-        });
-        fft_line<PL, -1>(xo, t, tw, acc);
-        if (valid) {
-            sfor<E>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                Y[base + PL::rot(m) * TT] = xo[m];
-            });
-        }
-    }
-}
-
-// Row flags and the compacted row list of a support mask [rows][N]: 16
-// threads per row, each testing 16 bytes; rows with any pixel set are
-// appended through an atomic counter (list order does not affect results,
-// rows are independent).
+// Row bits of a support mask [rows][N] (N % 16 == 0): bit r of the
+// concatenated per-pattern words is set when row r holds a support pixel.
+// 16 threads per row, each testing 16-byte chunks.
 __global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict__ mask,
                                                       int64_t nrows, int N,
-                                                      unsigned* __restrict__ rowbits,
-                                                      int* __restrict__ rows, int* __restrict__ count)
+                                                      unsigned* __restrict__ rowbits)
 {
     const int lane16 = threadIdx.x & 15;
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    int any = 0;
+    unsigned any = 0;
     if (row < nrows) {
-        const uint8_t* p = mask + row * N;
-        for (int i = lane16; i < N; i += 16) any |= p[i];
+        const uint4* p = reinterpret_cast<const uint4*>(mask + row * N);
+        for (int i = lane16; i < N / 16; i += 16) {
+            const uint4 q = p[i];
+            any |= q.x | q.y | q.z | q.w;
+        }
     }
     for (int off = 8; off > 0; off >>= 1) any |= __shfl_xor(any, off, 16);
-    if (lane16 == 0 && row < nrows && any) {
-        // rows of a pattern are consecutive, so global row index r maps to
-        // word r/32 bit r%32 of the concatenated per-pattern bitmasks (M % 32 == 0)
-        atomicOr(&rowbits[row >> 5], 1u << (row & 31));
-        rows[atomicAdd(count, 1)] = (int)row;
-    }
+    if (lane16 == 0 && row < nrows && any) atomicOr(&rowbits[row >> 5], 1u << (row & 31));
 }
 
 // Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
