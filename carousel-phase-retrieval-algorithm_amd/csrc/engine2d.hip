@@ -44,6 +44,7 @@ struct EngineGraph {
     const void* ws = nullptr;
     const void* ws_rows = nullptr;
     bool sparse = true;
+    int64_t chunk = 0;
     uint64_t amp_batch = 0;
     int iters = 0;
     float beta = 0.f;
@@ -112,7 +113,7 @@ template <int M, int MODE, bool SP>
 static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, const void* Y,
                            void* Z, const float* A, int64_t amp_batch, float scale, int skip,
                            float* rf_part, float gr, float gc, const unsigned* rowflag,
-                           float beta_n, int er)
+                           float beta_n, int er, int p0)
 {
     const unsigned grid = (unsigned)(g.B * g.N / SizeCfg<M>::C);
     const int xcd = (cv.xcd && grid % 8 == 0) ? 1 : 0;
@@ -120,25 +121,25 @@ static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, c
     auto* y = static_cast<const c2<float>*>(Y);
     auto* z = static_cast<c2<float>*>(Z);
     if (cv.map == MAP_WAVE)
-        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_WAVE, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er);
+        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_WAVE, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er, p0);
     else
-        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_BLOCK, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er);
+        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_BLOCK, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er, p0);
 }
 
 template <int M>
 static void launch_cols_m(hipStream_t st, const Geo& g, const ColVariant& cv, int mode,
                           const void* Y, void* Z, const float* A, int64_t amp_batch, float scale,
                           int skip, float* rf_part, float gr, float gc, const unsigned* rowflag,
-                          float beta_n, int er)
+                          float beta_n, int er, int p0)
 {
     if (mode == COL_MOD && rowflag)
-        launch_cols_mm<M, COL_MOD, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er);
+        launch_cols_mm<M, COL_MOD, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0);
     else if (mode == COL_MOD)
-        launch_cols_mm<M, COL_MOD, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0);
+        launch_cols_mm<M, COL_MOD, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
     else if (mode == COL_MOD_RF)
-        launch_cols_mm<M, COL_MOD_RF, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0);
+        launch_cols_mm<M, COL_MOD_RF, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
     else
-        launch_cols_mm<M, COL_BLUR, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0);
+        launch_cols_mm<M, COL_BLUR, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
 }
 
 static ColVariant col_variant(const cpra_hip_ctx* h, int M)
@@ -155,10 +156,11 @@ static ColVariant col_variant(const cpra_hip_ctx* h, int M)
 static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode, const void* Y,
                         void* Z, const float* A, int64_t amp_batch, float scale, int skip,
                         float* rf_part, float gr = 0.f, float gc = 0.f,
-                        const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0)
+                        const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0,
+                        int p0 = 0)
 {
     const ColVariant cv = col_variant(h, g.M);
-#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er)
+#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0)
     switch (g.M) {
     case 64: CPRA_COLS(64); break;
     case 128: CPRA_COLS(128); break;
@@ -184,13 +186,16 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
     const float invN = 1.0f / (float)N;
     const dim3 blk(SizeCfg<N>::TPB);
     const bool exact = lines % LPB == 0;
-#define CPRA_ROWS(MODE, EX) \
-    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
+#define CPRA_ROWS(MODE, EX, SP) \
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX, SP>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
+#define CPRA_ROWS2(MODE, SP) \
+    do { if (exact) CPRA_ROWS(MODE, true, SP); else CPRA_ROWS(MODE, false, SP); } while (0)
     if (er) {
-        if (exact) CPRA_ROWS(ROW_ER, true); else CPRA_ROWS(ROW_ER, false);
+        if (rowbits) CPRA_ROWS2(ROW_ER, true); else CPRA_ROWS2(ROW_ER, false);
     } else {
-        if (exact) CPRA_ROWS(ROW_HIO, true); else CPRA_ROWS(ROW_HIO, false);
+        if (rowbits) CPRA_ROWS2(ROW_HIO, true); else CPRA_ROWS2(ROW_HIO, false);
     }
+#undef CPRA_ROWS2
 #undef CPRA_ROWS
 }
 
@@ -328,7 +333,7 @@ enum KernelKind { KT_COL = 0, KT_ROW = 1, KT_SW = 2 };
 static bool sparse_rows_enabled(const cpra_hip_ctx* h)
 {
     auto it = h->tuning.find((uint64_t)2 << 32);
-    return it == h->tuning.end() || it->second != 0;
+    return it != h->tuning.end() && it->second != 0;
 }
 
 // Per pattern M/32 words: bit r set when row r holds a support pixel.
@@ -351,6 +356,17 @@ static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* 
     check_launch("k_support_rows");
 }
 
+// Patterns per launch of the iteration kernels (0 = the whole batch).  With
+// chunks of a few dozen patterns the intermediate Z and the second read of Y
+// stay in the 256 MB Infinity Cache, so HBM sees only the compulsory traffic
+// (read Y and A, write Y, read the support).
+static int64_t iteration_chunk(const cpra_hip_ctx* h)
+{
+    auto it = h->tuning.find((uint64_t)3 << 32);
+    const int64_t c = it == h->tuning.end() ? 0 : it->second;
+    return (c <= 0 || c >= (int64_t)h->batch) ? (int64_t)h->batch : c;
+}
+
 static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
                           int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p,
                           KernelTimer* kt = nullptr)
@@ -359,13 +375,22 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
     const float scale = (float)(1.0 / ((double)g.M * g.N));
     const bool sparse = sparse_rows_enabled(h);
     const unsigned* bits = sparse ? row_bits(h) : nullptr;
-    if (kt) kt->begin(h, st, KT_COL);
-    launch_cols(h, st, g, COL_MOD, x, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero, nullptr,
-                0.f, 0.f, bits, p->beta * (float)g.N, p->error_reduction);
-    if (kt) kt->end(h, st);
-    if (kt) kt->begin(h, st, KT_ROW);
-    launch_rows_hio(st, g, h->ws_z.ptr, x, mask, p->beta, p->real_positive, p->error_reduction, bits);
-    if (kt) kt->end(h, st);
+    const int64_t chunk = iteration_chunk(h);
+    const size_t px = (size_t)g.M * g.N;
+    for (int64_t p0 = 0; p0 < g.B; p0 += chunk) {
+        const Geo gc{g.M, g.N, (g.B - p0) < chunk ? (g.B - p0) : chunk};
+        void* xc = static_cast<c2<float>*>(x) + p0 * px;
+        const uint8_t* mc = mask + p0 * px;
+        const unsigned* bc = bits ? bits + p0 * g.M / 32 : nullptr;
+        if (kt) kt->begin(h, st, KT_COL);
+        launch_cols(h, st, gc, COL_MOD, xc, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero,
+                    nullptr, 0.f, 0.f, bc, p->beta * (float)g.N, p->error_reduction, (int)p0);
+        if (kt) kt->end(h, st);
+        if (kt) kt->begin(h, st, KT_ROW);
+        launch_rows_hio(st, gc, h->ws_z.ptr, xc, mc, p->beta, p->real_positive, p->error_reduction,
+                        bc);
+        if (kt) kt->end(h, st);
+    }
 }
 
 static hipStream_t aux_stream(cpra_hip_ctx* h)
@@ -454,6 +479,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             const bool same = eg && eg->x == x && eg->amp == amp && eg->mask == support &&
                               eg->ws == h->ws_z.ptr && eg->ws_rows == h->ws_rows.ptr &&
                               eg->sparse == sparse_rows_enabled(h) &&
+                              eg->chunk == iteration_chunk(h) &&
                               eg->amp_batch == amp_batch && eg->iters == chunk &&
                               eg->beta == p->beta && eg->rp == p->real_positive &&
                               eg->skip == p->dc_skip_zero;
@@ -471,6 +497,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 eg->ws = h->ws_z.ptr;
                 eg->ws_rows = h->ws_rows.ptr;
                 eg->sparse = sparse_rows_enabled(h);
+                eg->chunk = iteration_chunk(h);
                 eg->amp_batch = amp_batch;
                 eg->iters = chunk;
                 eg->beta = p->beta;
@@ -615,6 +642,7 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         int best_v = 0;
         const bool saved_timing = h->timing;
         h->tuning[(uint64_t)2 << 32] = 0;
+        h->tuning[(uint64_t)3 << 32] = 0;
         for (int v = 0; v < 4; ++v) {
             h->tuning[key] = v;
             h->timing = true;
@@ -639,24 +667,38 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
             CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
         }
         h->tuning[key] = best_v;
-        // dense rows (three row FFTs on every row) against sparse rows (rows
-        // without support pixels finished in the column kernel): whole
-        // iteration time with the chosen column variant
+        // whole iterations as the engine runs them (graph replays), timed with
+        // events around the run: dense against sparse rows, then the patterns
+        // per launch (whole batch against Infinity-Cache-sized chunks)
+        h->timing = false;
+        auto run_us = [&](int n_iter) -> double {
+            cpra_hip_hio_params w = p;
+            w.iters = n_iter;
+            w.sw_every = 0;
+            w.use_graph = 1;
+            int rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &w, nullptr);  // capture
+            if (rc < 0) fail(rc, "%s", h->last_error.c_str());
+            hipEvent_t e0, e1;
+            CPRA_CHECK_HIP(hipEventCreate(&e0));
+            CPRA_CHECK_HIP(hipEventCreate(&e1));
+            CPRA_CHECK_HIP(hipEventRecord(e0, h->stream));
+            rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &w, nullptr);
+            if (rc < 0) fail(rc, "%s", h->last_error.c_str());
+            CPRA_CHECK_HIP(hipEventRecord(e1, h->stream));
+            CPRA_CHECK_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CPRA_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
+            return 1e3 * ms / n_iter;
+        };
         const uint64_t skey = (uint64_t)2 << 32;
+        const uint64_t ckey = (uint64_t)3 << 32;
         double it_us[2] = {0, 0};
         for (int sp = 0; sp < 2; ++sp) {
             h->tuning[skey] = sp;
-            cpra_hip_hio_params w = p;
-            w.iters = 1;
-            h->timing = false;
-            int rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &w, nullptr);
-            if (rc < 0) return rc;
-            for (int i = 0; i < 8; ++i) { h->kt_ms[i] = 0; h->kt_n[i] = 0; }
-            h->timing = true;
-            rc = cpra_hip_hio_run(h, x, amp, amp_batch, support, &p, nullptr);
-            if (rc < 0) return rc;
-            it_us[sp] = 1e3 * (h->kt_ms[0] + h->kt_ms[1]) / (double)(h->kt_n[0] > 0 ? h->kt_n[0] : 1);
-            CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
+            it_us[sp] = run_us(iters);
         }
         const int best_sp = it_us[1] < it_us[0] ? 1 : 0;
         h->tuning[skey] = best_sp;
@@ -665,12 +707,29 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
             std::snprintf(buf, sizeof(buf), "iter_dense_us=%.2f;iter_sparse_us=%.2f;", it_us[0], it_us[1]);
             rep += buf;
         }
+        int best_chunk = 0;
+        double best_it = it_us[best_sp];
+        const int cands[] = {256, 128, 96, 64, 48, 32};
+        for (int cc : cands) {
+            if ((uint64_t)cc >= h->batch) continue;
+            h->tuning[ckey] = cc;
+            const double us = run_us(iters);
+            char buf[64];
+            std::snprintf(buf, sizeof(buf), "iter_chunk%d_us=%.2f;", cc, us);
+            rep += buf;
+            if (us < best_it) {
+                best_it = us;
+                best_chunk = cc;
+            }
+        }
+        h->tuning[ckey] = best_chunk;
         h->timing = saved_timing;
         CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
         char head[128];
-        std::snprintf(head, sizeof(head), "col_%d=%d;rows_sparse=%d;col_map=%s;col_xcd=%d;", g.M,
-                      best_v, best_sp, (best_v & 1) ? "wave" : "block", (best_v >> 1) & 1);
+        std::snprintf(head, sizeof(head), "col_%d=%d;rows_sparse=%d;chunk=%d;col_map=%s;col_xcd=%d;",
+                      g.M, best_v, best_sp, best_chunk, (best_v & 1) ? "wave" : "block",
+                      (best_v >> 1) & 1);
         rep = head + rep;
         if (report && report_len) {
             std::snprintf(report, report_len, "%s", rep.c_str());
@@ -688,7 +747,11 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
         const char* p = spec;
         while (*p) {
             int m = 0, v = 0, used = 0;
-            if (std::sscanf(p, "rows_sparse=%d;%n", &v, &used) == 1 && used > 0) {
+            if (std::sscanf(p, "chunk=%d;%n", &v, &used) == 1 && used > 0) {
+                CPRA_ARG(v >= 0, "cpra_hip_set_tuning: chunk < 0");
+                h->tuning[(uint64_t)3 << 32] = v;
+                p += used;
+            } else if (std::sscanf(p, "rows_sparse=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)2 << 32] = v ? 1 : 0;
                 p += used;
             } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
@@ -701,6 +764,18 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
                 p = semi + 1;
             }
         }
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
+        const ColVariant cv = col_variant(h, (int)h->m);
+        std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;", (int)h->m,
+                      cv.map | (cv.xcd << 1), sparse_rows_enabled(h) ? 1 : 0,
+                      (long long)iteration_chunk(h));
         return CPRA_HIP_OK;
     });
 }

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
     const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
     float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
     int xcd_remap, const unsigned* __restrict__ rowbits = nullptr, float beta_n = 0.f,
-    int error_reduction = 0)
+    int error_reduction = 0, int pattern0 = 0)
 {
     using V = c2<float>;
     using G = ColGeom<M, MAP>;
@@ -110,7 +110,9 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
     if constexpr (MODE != COL_BLUR) {
         // the amplitudes are needed after the forward FFT; load them now so
         // their latency overlaps the transform
-        const float* Ap = A + (size_t)(b % (unsigned)amp_batch) * pat + col0;
+        // pattern0: index of this launch's first pattern in the whole batch
+        // (chunked launches), for the amplitude broadcast b % amp_batch
+        const float* Ap = A + (size_t)(((unsigned)pattern0 + b) % (unsigned)amp_batch) * pat + col0;
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             av[m] = (Ap + rowbase(m))[toff];
@@ -220,11 +222,11 @@ enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
 // rowbits != NULL (sparse rows): lines without a support pixel were finished
 // by the column kernel; they are skipped, and a wave whose lines are all
 // finished returns at once (the row exchange never leaves the wave).
-template <int N, int MODE, bool EXACT>
+template <int N, int MODE, bool EXACT, bool SPARSE>
 __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
     int64_t nlines, float beta, float invN, int real_positive,
-    const unsigned* __restrict__ rowbits = nullptr)
+    const unsigned* __restrict__ rowbits)
 {
     using V = c2<float>;
     using PL = typename SizeCfg<N>::P;
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     const int t = tid % TT, ll = tid / TT;
     const int64_t line = (int64_t)blockIdx.x * LPB + ll;
     bool valid = EXACT || line < nlines;
-    if (rowbits) {
+    if constexpr (SPARSE) {
         valid = valid && ((rowbits[line >> 5] >> (line & 31)) & 1u);
         if (__ballot(valid) == 0ull) return;
     }

@@ -103,6 +103,7 @@ _SIGS = {
     "cpra_hip_set_kernel_timing": (C.c_int, [_H, C.c_int]),
     "cpra_hip_autotune_2d": (C.c_int, [_H, _P, _P, _U64, _P, C.c_int, C.c_char_p, _U64]),
     "cpra_hip_set_tuning": (C.c_int, [_H, C.c_char_p]),
+    "cpra_hip_get_tuning": (C.c_int, [_H, C.c_char_p, _U64]),
     "cpra_hip_get_kernel_times": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                             C.c_int]),
 }
@@ -386,6 +387,11 @@ class HipImpl:
 
     def set_tuning(self, spec: str) -> bool:
         return self._check(lib().cpra_hip_set_tuning(self._h, spec.encode()))
+
+    def get_tuning(self) -> dict:
+        buf = C.create_string_buffer(256)
+        self._check(lib().cpra_hip_get_tuning(self._h, buf, 256))
+        return dict(kv.split("=") for kv in buf.value.decode().strip(";").split(";"))
 
     def set_kernel_timing(self, enable: bool) -> bool:
         return self._check(lib().cpra_hip_set_kernel_timing(self._h, int(enable)))

@@ -63,15 +63,19 @@ int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t*
 /* Autotuner (north-star addition): times each column-kernel variant
  * (thread mapping x XCD-aware tile order) for `iters` eager HIO iterations on
  * the caller's data, then dense against sparse row passes (rows without a
- * support pixel finished in the column kernel) on whole iterations, keeps
+ * support pixel finished in the column kernel) and the patterns per launch
+ * (whole batch or Infinity-Cache-sized chunks) on whole iterations, keeps
  * the fastest for this handle, restores x and the support, and writes
- * "col_<m>=<v>;rows_sparse=<0|1>;...;col_v<k>_us=..;iter_*_us=..;" into
+ * "col_<m>=<v>;rows_sparse=<0|1>;chunk=<c>;...;col_v<k>_us=..;iter_*_us=..;" into
  * report (may be NULL). */
 int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                          uint8_t* support, int iters, char* report, uint64_t report_len);
-/* Re-applies the "col_<m>=<v>;" and "rows_sparse=<0|1>;" choices of an
- * autotune report. */
+/* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1>;" and "chunk=<patterns>;"
+ * choices of an autotune report (chunk: patterns per launch of the iteration
+ * kernels, 0 = whole batch). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
+/* Current choices as "col_<m>=<v>;rows_sparse=<0|1>;chunk=<patterns>;". */
+int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 
 /* Per-kernel timing of the engines: when enabled, engine runs launch
  * eagerly (no graph) with a hipEvent pair around every fused kernel on the
