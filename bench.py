@@ -143,6 +143,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--timing-iters", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one MergeAddData (dst = 1*src + 0*dst) over 2^26 complex64 "
+                         "first: a kernel of known traffic (16 B read, 8 B written per "
+                         "element) to scale PMC FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -171,8 +176,18 @@ def main():
     sched.pop("iters")
     h = hip.HipImpl(n, n, 1, P)
 
-    # autotune the column-kernel variant on this workload (restores x, support)
-    tune = h.autotune(x, amp, sup, iters=10)
+    if args.calibrate_copy:
+        ncal = 1 << 26
+        a_ = hip.zeros(ncal, np.complex64)
+        b_ = hip.zeros(ncal, np.complex64)
+        h.MergeAddData(a_, b_, 1.0, 0.0, ncal)
+        h.Sync()
+        a_.free()
+        b_.free()
+        log("bench: copy calibration kernel done (2^26 complex64)")
+
+    # autotune the kernel variants on this workload (restores x, support)
+    tune = "" if args.no_autotune else h.autotune(x, amp, sup, iters=10)
     log(f"bench: autotune {tune}")
 
     # warmup (also captures the 20-iteration graph)
