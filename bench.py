@@ -46,6 +46,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known traffic)."""
+    path = os.path.join(ROOT, "profiles", "hio_512x256sq_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(parts[2])
+    return None
+
+
 def cpu_baseline_oracle(data, n_patterns, min_seconds=10.0):
     """Times the FP64 oracle's HIO (same schedule) on a sample of patterns."""
     import numpy as np
@@ -229,6 +242,11 @@ def main():
             "kernels": kern,
             "shrinkwrap_update_ms": sw_ms / max(sw_n, 1),
             "tuning": tuning}
+    # counters were taken on whole-batch launches of the 512 x 256^2 workload
+    if P == 512 and n == 256 and int(tuning.get("chunk", "512")) in (0, P):
+        tb = measured_traffic(roof["kernel"])
+        roof["traffic"] = tb
+        roof["traffic_source"] = "profiles/hio_512x256sq_pmc.md (rocprofv3 FETCH_SIZE+WRITE_SIZE)"
     iter_algo_frac = px * HIO_ALGO_BYTES_PER_PX * its / 1e9 / HBM_PEAK_GBS
     h.close()
 
