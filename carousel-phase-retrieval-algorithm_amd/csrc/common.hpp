@@ -82,7 +82,7 @@ struct cpra_hip_ctx {
     // twiddle tables keyed by (n << 1) | is_double
     std::map<uint64_t, void*> twiddles;
     // engine workspaces
-    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows;
+    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm;
     // tuning choice per (kind, n) set by the autotuner (0 = default)
     std::map<uint64_t, int> tuning;
     cpra::EngineGraph* graph = nullptr;

@@ -393,6 +393,78 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
     }
 }
 
+template <int N>
+static void dm_entry_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p,
+                       const c2<float>* psi, const c2<float>* t1, const c2<float>* t2,
+                       const uint8_t* mask, c2<float>* Y1, c2<float>* Y2)
+{
+    const int64_t lines = g.B * g.M;
+    constexpr int LPB = SizeCfg<N>::LPB;
+    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(SizeCfg<N>::TPB);
+#define CPRA_DME(TB, SP) hipLaunchKernelGGL((k_dm_entry<N, TB, SP>), grid, blk, 0, st, psi, t1, t2, mask, Y1, Y2, lines, p->beta)
+    if (p->textbook) {
+        if (p->space_mode == 0) CPRA_DME(true, SPACE_MKL);
+        else if (p->space_mode == 1) CPRA_DME(true, SPACE_POS);
+        else CPRA_DME(true, SPACE_CPLX);
+    } else {
+        if (p->space_mode == 0) CPRA_DME(false, SPACE_MKL);
+        else if (p->space_mode == 1) CPRA_DME(false, SPACE_POS);
+        else CPRA_DME(false, SPACE_CPLX);
+    }
+#undef CPRA_DME
+}
+
+template <int N>
+static void dm_rows_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p, bool last,
+                      const c2<float>* Z1, const c2<float>* Z2, c2<float>* psi,
+                      const uint8_t* mask, c2<float>* Y1, c2<float>* Y2, c2<float>* t1,
+                      c2<float>* t2)
+{
+    const int64_t lines = g.B * g.M;
+    constexpr int LPB = SizeCfg<N>::LPB;
+    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(SizeCfg<N>::TPB);
+#define CPRA_DMR(TB, SP, LA) hipLaunchKernelGGL((k_dm_rows<N, TB, SP, LA>), grid, blk, 0, st, Z1, Z2, psi, mask, Y1, Y2, t1, t2, lines, p->beta)
+#define CPRA_DMR2(TB, SP) do { if (last) CPRA_DMR(TB, SP, true); else CPRA_DMR(TB, SP, false); } while (0)
+    if (p->textbook) {
+        if (p->space_mode == 0) CPRA_DMR2(true, SPACE_MKL);
+        else if (p->space_mode == 1) CPRA_DMR2(true, SPACE_POS);
+        else CPRA_DMR2(true, SPACE_CPLX);
+    } else {
+        if (p->space_mode == 0) CPRA_DMR2(false, SPACE_MKL);
+        else if (p->space_mode == 1) CPRA_DMR2(false, SPACE_POS);
+        else CPRA_DMR2(false, SPACE_CPLX);
+    }
+#undef CPRA_DMR2
+#undef CPRA_DMR
+}
+
+#define CPRA_SIZE_SWITCH(NV, CALL)                       \
+    switch (NV) {                                        \
+    case 64: { constexpr int NN = 64; CALL; } break;     \
+    case 128: { constexpr int NN = 128; CALL; } break;   \
+    case 192: { constexpr int NN = 192; CALL; } break;   \
+    case 256: { constexpr int NN = 256; CALL; } break;   \
+    case 384: { constexpr int NN = 384; CALL; } break;   \
+    case 512: { constexpr int NN = 512; CALL; } break;   \
+    }
+
+static void dm_entry_launch(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p,
+                            const c2<float>* psi, const c2<float>* t1, const c2<float>* t2,
+                            const uint8_t* mask, c2<float>* Y1, c2<float>* Y2)
+{
+    CPRA_SIZE_SWITCH(g.N, (dm_entry_n<NN>(st, g, p, psi, t1, t2, mask, Y1, Y2)));
+    check_launch("k_dm_entry");
+}
+
+static void dm_rows_launch(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p, bool last,
+                           const c2<float>* Z1, const c2<float>* Z2, c2<float>* psi,
+                           const uint8_t* mask, c2<float>* Y1, c2<float>* Y2, c2<float>* t1,
+                           c2<float>* t2)
+{
+    CPRA_SIZE_SWITCH(g.N, (dm_rows_n<NN>(st, g, p, last, Z1, Z2, psi, mask, Y1, Y2, t1, t2)));
+    check_launch("k_dm_rows");
+}
+
 static hipStream_t aux_stream(cpra_hip_ctx* h)
 {
     if (!h->aux_stream) CPRA_CHECK_HIP(hipStreamCreateWithFlags(&h->aux_stream, hipStreamNonBlocking));
@@ -545,6 +617,75 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             CPRA_CHECK_HIP(hipStreamSynchronize(st));
         }
         kt.collect(h);
+        return CPRA_HIP_OK;
+    });
+}
+
+void cpra_hip_dm_default_params(cpra_hip_dm_params* p)
+{
+    if (!p) return;
+    p->iters = 100;
+    p->beta = 0.9f;
+    p->textbook = 0;
+    p->space_mode = CPRA_HIP_DM_SPACE_MKL;
+    p->dc_skip_zero = 1;
+    p->use_graph = 1;
+}
+
+int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
+                    uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && psi && amp && support && p, "cpra_hip_dm_run: NULL argument");
+        CPRA_ARG(p->textbook || (t1 && t2), "cpra_hip_dm_run: the reference variant needs t1 and t2");
+        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0, "cpra_hip_dm_run: amp_batch must divide batch");
+        CPRA_ARG(p->iters >= 0, "cpra_hip_dm_run: iters < 0");
+        CPRA_ARG(p->beta != 0.f, "cpra_hip_dm_run: beta == 0");
+        CPRA_ARG(p->space_mode >= 0 && p->space_mode <= 2, "cpra_hip_dm_run: bad space_mode");
+        engine_check_geometry(h, "cpra_hip_dm_run");
+        if (p->iters == 0) return CPRA_HIP_OK;
+        const Geo g = geo_of(h);
+        const size_t total = (size_t)g.M * g.N * (size_t)g.B;
+        h->ws_z.ensure(sizeof(c2<float>) * 2 * total);    // [Z1; Z2]
+        h->ws_dm.ensure(sizeof(c2<float>) * 2 * total);   // [Y1; Y2]
+        c2<float>* Y = static_cast<c2<float>*>(h->ws_dm.ptr);
+        c2<float>* Z = static_cast<c2<float>*>(h->ws_z.ptr);
+        hipStream_t st = h->stream;
+        dm_entry_launch(st, g, p, static_cast<const c2<float>*>(psi), static_cast<const c2<float>*>(t1),
+                        static_cast<const c2<float>*>(t2), support, Y, Y + total);
+        const Geo g2{g.M, g.N, 2 * g.B};
+        const float scale = (float)(1.0 / ((double)g.M * g.N));
+        auto one = [&](hipStream_t s, bool last) {
+            launch_cols(h, s, g2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr);
+            dm_rows_launch(s, g, p, last, Z, Z + total, static_cast<c2<float>*>(psi), support, Y,
+                           Y + total, static_cast<c2<float>*>(t1), static_cast<c2<float>*>(t2));
+        };
+        const int body = p->iters - 1;  // the last iteration writes t1, t2 in pattern space
+        if (p->use_graph && body >= 2) {
+            hipStream_t gst = aux_stream(h);
+            hipEvent_t ev0, ev1;
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            hipGraph_t graph;
+            hipGraphExec_t exec;
+            CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < body; ++i) one(gst, false);
+            CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
+            CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            CPRA_CHECK_HIP(hipStreamSynchronize(gst));
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+        } else {
+            for (int i = 0; i < body; ++i) one(st, false);
+        }
+        one(st, true);
         return CPRA_HIP_OK;
     });
 }

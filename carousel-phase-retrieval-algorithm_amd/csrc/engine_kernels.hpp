@@ -439,3 +439,152 @@ __global__ void k_u8_from_S(const float* __restrict__ S, uint8_t* __restrict__ m
 }
 
 }  // namespace cpra
+
+namespace cpra {
+
+// ---------------------------------------------------------------------------
+// difference map (reference variant, src/CPRA_MKL.cpp:94-110, and Elser's
+// textbook map).  State: psi in pattern space, Y1 = DFT_r(t1) and
+// Y2 = DFT_r(psi/beta + (1 - 1/beta) P_S t2) in the row-spectral domain (the
+// textbook map uses t1 = t2 = psi).  The column kernel turns [Y1; Y2] (2B
+// patterns, one launch) into [Z1; Z2] with Z = IDFT_c(P_M(DFT_c Y))/(MN), so
+// IDFT_r(Z1) = P_M t1 and IDFT_r(Z2) = P_M(psi/beta + (1-1/beta) P_S t2).
+// ---------------------------------------------------------------------------
+enum SpaceMode { SPACE_MKL = 0, SPACE_POS = 1, SPACE_CPLX = 2 };
+
+// P_S with a u8 support: SPACE_MKL is the reference's SpaceConstraint on the
+// equivalent mask (im <- 0; re <- 0 outside, MKL_Impl.inl:79-81), SPACE_POS
+// also clamps re < 0 to 0, SPACE_CPLX keeps the complex value inside.
+template <int SPACE> __device__ __forceinline__ c2<float> ps_apply(c2<float> z, bool in)
+{
+    if constexpr (SPACE == SPACE_CPLX) return in ? z : mk<c2<float>>(0.f, 0.f);
+    else if constexpr (SPACE == SPACE_POS) return mk<c2<float>>(in && z.x > 0.f ? z.x : 0.f, 0.f);
+    else return mk<c2<float>>(in ? z.x : 0.f, 0.f);
+}
+
+// Entry: Y1 = DFT_r(t1 | psi), Y2 = DFT_r(psi/beta + (1-1/beta) P_S(t2 | psi)).
+// Y1 and Y2 are the two halves of one 2B-pattern array (Y2 = Y1 + B*M*N).
+template <int N, bool TEXTBOOK, int SPACE>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_entry(
+    const c2<float>* __restrict__ psi, const c2<float>* __restrict__ t1,
+    const c2<float>* __restrict__ t2, const uint8_t* __restrict__ mask,
+    c2<float>* __restrict__ Y1, c2<float>* __restrict__ Y2, int64_t nlines, float beta)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const size_t base = (size_t)(valid ? line : 0) * N + t;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    const float ib = 1.f / beta, cb = 1.f - 1.f / beta;
+    V a[E], u[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const size_t o = base + PL::rot(m) * TT;
+        const V p = valid ? psi[o] : mk<V>(0, 0);
+        const V s1 = TEXTBOOK ? p : (valid ? t1[o] : mk<V>(0, 0));
+        const V s2 = TEXTBOOK ? p : (valid ? t2[o] : mk<V>(0, 0));
+        const V q = ps_apply<SPACE>(s2, valid && mask[o] != 0);
+        a[m] = s1;
+        u[m] = mk<V>(p.x * ib + cb * q.x, p.y * ib + cb * q.y);
+    });
+    fft_line<PL, -1>(a, t, tw, acc);
+    fft_line<PL, -1>(u, t, tw, acc);
+    if (valid) {
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const size_t o = base + PL::rot(m) * TT;
+            Y1[o] = a[m];
+            Y2[o] = u[m];
+        });
+    }
+}
+
+// One DM iteration's row pass (after the column kernel on [Y1; Y2]):
+//   a   = IDFT_r(Z1) = P_M t1
+//   t1' = P_S(-psi/beta + (1 + 1/beta) a)                 CPRA_MKL.cpp:95-99
+//   t2' = IDFT_r(Z2) = P_M(psi/beta + (1 - 1/beta) P_S t2)      :102-106
+//   psi' = (psi + beta t1') - beta t2'                          :109-110
+// then Y1 = DFT_r(t1' | psi'), Y2 = DFT_r(psi'/beta + (1-1/beta) P_S(t2' | psi'))
+// for the next iteration; LAST writes psi', t1', t2' in pattern space instead.
+template <int N, bool TEXTBOOK, int SPACE, bool LAST>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
+    const c2<float>* __restrict__ Z1, const c2<float>* __restrict__ Z2,
+    c2<float>* __restrict__ psi, const uint8_t* __restrict__ mask, c2<float>* __restrict__ Y1,
+    c2<float>* __restrict__ Y2, c2<float>* __restrict__ t1out, c2<float>* __restrict__ t2out,
+    int64_t nlines, float beta)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const bool valid = line < nlines;
+    const size_t base = (size_t)(valid ? line : 0) * N + t;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    const float ib = 1.f / beta, pb = 1.f + 1.f / beta, cb = 1.f - 1.f / beta;
+    V a[E], b[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const size_t o = base + PL::rot(m) * TT;
+        a[m] = valid ? Z1[o] : mk<V>(0, 0);
+        b[m] = valid ? Z2[o] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(a, t, tw, acc);
+    fft_line<PL, +1>(b, t, tw, acc);
+    // a <- t1' (or the next Y1 source), b <- t2' (or next Y2 source)
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const size_t o = base + PL::rot(m) * TT;
+        const bool in = valid && mask[o] != 0;
+        const V p = valid ? psi[o] : mk<V>(0, 0);
+        const V w = mk<V>(-ib * p.x + pb * a[m].x, -ib * p.y + pb * a[m].y);
+        const V t1n = ps_apply<SPACE>(w, in);
+        const V t2n = b[m];
+        V pn = mk<V>(p.x + beta * t1n.x, p.y + beta * t1n.y);
+        pn = mk<V>(pn.x - beta * t2n.x, pn.y - beta * t2n.y);
+        if (valid) psi[o] = pn;
+        if constexpr (LAST) {
+            a[m] = t1n;
+            b[m] = t2n;
+        } else {
+            const V s1 = TEXTBOOK ? pn : t1n;
+            const V q = ps_apply<SPACE>(TEXTBOOK ? pn : t2n, in);
+            a[m] = s1;
+            b[m] = mk<V>(pn.x * ib + cb * q.x, pn.y * ib + cb * q.y);
+        }
+    });
+    if constexpr (LAST) {
+        if (valid) {
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                const size_t o = base + PL::rot(m) * TT;
+                if (t1out) t1out[o] = a[m];
+                if (t2out) t2out[o] = b[m];
+            });
+        }
+    } else {
+        fft_line<PL, -1>(a, t, tw, acc);
+        fft_line<PL, -1>(b, t, tw, acc);
+        if (valid) {
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                const size_t o = base + PL::rot(m) * TT;
+                Y1[o] = a[m];
+                Y2[o] = b[m];
+            });
+        }
+    }
+}
+
+}  // namespace cpra

@@ -46,6 +46,17 @@ class CpraHipError(RuntimeError):
         self.code = code
 
 
+class DmParams(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int32),
+        ("beta", C.c_float),
+        ("textbook", C.c_int32),
+        ("space_mode", C.c_int32),
+        ("dc_skip_zero", C.c_int32),
+        ("use_graph", C.c_int32),
+    ]
+
+
 class HioParams(C.Structure):
     _fields_ = [
         ("iters", C.c_int32),
@@ -95,6 +106,8 @@ _SIGS = {
     "cpra_hip_hio_default_params": (None, [C.POINTER(HioParams)]),
     "cpra_hip_hio_run": (C.c_int, [_H, _P, _P, _U64, _P, C.POINTER(HioParams), _P]),
     "cpra_hip_shrinkwrap": (C.c_int, [_H, _P, _P, C.c_float, C.c_float, _P]),
+    "cpra_hip_dm_default_params": (None, [C.POINTER(DmParams)]),
+    "cpra_hip_dm_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
     "cpra_hip_rfactor": (C.c_int, [_H, _P, _P, _U64, _P]),
     "cpra_hip_support_from_constraint": (C.c_int, [_H, _P, _P, _U64]),
     "cpra_hip_real2d_to_3d": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
@@ -364,6 +377,17 @@ class HipImpl:
         self._check(lib().cpra_hip_hio_run(self._h, _addr(x), _addr(amp), ab, _addr(support),
                                            C.byref(p), rf.ctypes.data if rf is not None else None))
         return rf
+
+    def dm_run(self, psi, amp, support, t1=None, t2=None, amp_batch=None, **kw) -> bool:
+        p = DmParams()
+        lib().cpra_hip_dm_default_params(C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n)
+        return self._check(lib().cpra_hip_dm_run(self._h, _addr(psi), _addr(t1), _addr(t2),
+                                                 _addr(amp), ab, _addr(support), C.byref(p)))
 
     def shrinkwrap(self, x, support, sigma: float, frac: float, blur_out=None) -> bool:
         return self._check(lib().cpra_hip_shrinkwrap(self._h, _addr(x), _addr(support), sigma,

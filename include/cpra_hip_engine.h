@@ -44,6 +44,34 @@ void cpra_hip_hio_default_params(cpra_hip_hio_params* p);
 int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                      uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host);
 
+/* Difference map.  The reference variant (textbook = 0) is the iteration
+ * body of src/CPRA_MKL.cpp:94-110 with t1, t2 carried across iterations:
+ *   t1 <- P_S(-psi/beta + (1+1/beta) P_M t1)
+ *   t2 <- P_M(psi/beta + (1-1/beta) P_S t2)
+ *   psi <- psi + beta t1 - beta t2
+ * textbook = 1 is Elser's map (t1 = t2 = psi at every iteration).  P_M is
+ * Forward2D, DataConstraint (dc_skip_zero as in MklImpl), Backward2D; P_S uses
+ * the u8 support: SPACE_MKL = the reference SpaceConstraint on the mask
+ * equivalent of its float constraint (cpra_hip_support_from_constraint),
+ * SPACE_POSITIVE additionally clamps Re < 0, SPACE_COMPLEX keeps complex
+ * values inside the support.  psi (and t1, t2 for the reference variant) are
+ * [batch][m][n] complex64 in pattern space, updated in place. */
+#define CPRA_HIP_DM_SPACE_MKL 0
+#define CPRA_HIP_DM_SPACE_POSITIVE 1
+#define CPRA_HIP_DM_SPACE_COMPLEX 2
+typedef struct {
+    int32_t iters;
+    float beta;
+    int32_t textbook;
+    int32_t space_mode;
+    int32_t dc_skip_zero;
+    int32_t use_graph;
+} cpra_hip_dm_params;
+
+void cpra_hip_dm_default_params(cpra_hip_dm_params* p);
+int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
+                    uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p);
+
 /* Shrinkwrap of x [batch][m][n]: b = Re Backward2D(Forward2D(|x|) * G_sigma),
  * G_sigma(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-m/2)/m)^2 + ((kc-n/2)/n)^2]);
  * support = (b >= frac * max_pattern(b)).  blur_out (device, [batch][m][n]

@@ -1,0 +1,103 @@
+"""GPU parity of the fused difference-map engine against the FP64 oracle:
+the reference variant (src/CPRA_MKL.cpp:94-110, t1 and t2 carried across
+iterations, MKL SpaceConstraint semantics) and Elser's textbook map."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def synth(pkg):
+    from cpra_amd import synth
+
+    return synth
+
+
+def starts(data, B, seed):
+    rng = np.random.default_rng(seed)
+    n = data["amp"].shape[-1]
+    x = np.stack([np.fft.fftshift(np.fft.ifft2(np.fft.fftshift(
+        data["amp"][b % data["amp"].shape[0]] * np.exp(2j * np.pi * rng.random((n, n))))))
+        for b in range(B)])
+    return x.astype(np.complex64)
+
+
+@pytest.mark.parametrize("n,P,B", [(64, 1, 2), (256, 1, 2), (128, 2, 4), (192, 1, 1)])
+@pytest.mark.parametrize("iters", [1, 4])
+def test_dm_reference_variant_matches_oracle(hipmod, oracle, gpu, synth, n, P, B, iters):
+    data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, config=150 + n)
+    S = (data["support"][:P]).astype(np.float32)       # reference float constraint
+    psi0 = starts(data, B, 1)
+    t10, t20 = starts(data, B, 2), starts(data, B, 3)
+    psi, t1, t2 = (hipmod.to_device(a) for a in (psi0, t10, t20))
+    mask = hipmod.zeros((P, n, n), np.uint8)
+    with hipmod.HipImpl(n, n, 1, B) as h:
+        assert h.support_from_constraint(hipmod.to_device(S), mask, P * n * n)
+        # the support broadcast over the starts like the reference's i % (num/batch)
+        full = hipmod.to_device(np.tile(mask.numpy(), (B // P, 1, 1)))
+        assert h.dm_run(psi, hipmod.to_device(data["amp"]), full, t1=t1, t2=t2,
+                        iters=iters, beta=0.9, textbook=0, space_mode=0, use_graph=1)
+        h.Sync()
+    rp, r1, r2 = oracle.dm_ref(psi0, t10, t20, data["amp"], S, 0.9, iters)
+    tol = 2e-5 * iters
+    assert relerr(psi.numpy(), rp) < tol
+    assert relerr(t1.numpy(), r1) < tol
+    assert relerr(t2.numpy(), r2) < tol
+
+
+@pytest.mark.parametrize("space_mode", [0, 1, 2])
+def test_dm_textbook_steps_match_oracle(hipmod, oracle, gpu, synth, space_mode):
+    n, P = 128, 3
+    data = synth.make_2d_batch(P, n, 15, 20, 1e6, config=160)
+    sup = (data["obj"] > 0).astype(np.uint8)
+    prev = data["x0"]
+    psi = hipmod.to_device(prev)
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        for step in range(3):
+            assert h.dm_run(psi, hipmod.to_device(data["amp"]), hipmod.to_device(sup),
+                            iters=1, beta=0.8, textbook=1, space_mode=space_mode, use_graph=0)
+            h.Sync()
+            got = psi.numpy()
+            if space_mode == 0:
+                ref = oracle.dm_textbook(prev, data["amp"], 0.8, 1, S=sup.astype(float),
+                                         space_mode=0)
+            else:
+                ref = oracle.dm_textbook(prev, data["amp"], 0.8, 1, mask=sup,
+                                         space_mode=space_mode)
+            assert relerr(got, ref) < 1e-5, step
+            prev = got
+
+
+def test_dm_fixed_points(hipmod, gpu):
+    n = 64
+    yy, xx = np.mgrid[0:n, 0:n]
+    sup = ((yy - n / 2) ** 2 + (xx - n / 2) ** 2 <= 36).astype(np.uint8)
+    obj = (np.random.default_rng(0).uniform(0.2, 1.0, (n, n)) * sup).astype(np.complex64)
+    A = np.abs(np.fft.fftshift(np.fft.fft2(np.fft.fftshift(obj)))).astype(np.float32)
+    for textbook in (0, 1):
+        psi = hipmod.to_device(obj[None])
+        t1 = hipmod.to_device(obj[None])
+        t2 = hipmod.to_device(obj[None])
+        with hipmod.HipImpl(n, n, 1, 1) as h:
+            assert h.dm_run(psi, hipmod.to_device(A[None]), hipmod.to_device(sup[None]), t1=t1,
+                            t2=t2, iters=6, beta=0.9, textbook=textbook, space_mode=1)
+            h.Sync()
+        assert relerr(psi.numpy()[0], obj) < 1e-5
+        if not textbook:
+            assert relerr(t1.numpy()[0], obj) < 1e-5 and relerr(t2.numpy()[0], obj) < 1e-5
+
+
+def test_dm_argument_checks(hipmod, gpu):
+    with hipmod.HipImpl(64, 64, 1, 2) as h:
+        z = hipmod.zeros((2, 64, 64), np.complex64)
+        with pytest.raises(hipmod.CpraHipError):
+            h.dm_run(z, hipmod.zeros((2, 64, 64), np.float32),
+                     hipmod.zeros((2, 64, 64), np.uint8), iters=1, textbook=0)  # no t1, t2
+        with pytest.raises(hipmod.CpraHipError):
+            h.dm_run(z, hipmod.zeros((2, 64, 64), np.float32),
+                     hipmod.zeros((2, 64, 64), np.uint8), iters=1, textbook=1, beta=0.0)
