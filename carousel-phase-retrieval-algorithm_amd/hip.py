@@ -119,6 +119,12 @@ _SIGS = {
     "cpra_hip_get_tuning": (C.c_int, [_H, C.c_char_p, _U64]),
     "cpra_hip_get_kernel_times": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                             C.c_int]),
+    "cpra_hip_amp_loss": (C.c_int, [_H, _P, _P, _U64, _P]),
+    "cpra_hip_amp_grad": (C.c_int, [_H, _P, _P, _U64, _P, _P]),
+    "cpra_hip_amp_loss_3d": (C.c_int, [_H, _P, _P, _U64, _P]),
+    "cpra_hip_amp_grad_3d": (C.c_int, [_H, _P, _P, _U64, _P, _P]),
+    "cpra_hip_amp_gd_run": (C.c_int, [_H, _P, _P, _U64, C.c_int, C.c_float, _P]),
+    "cpra_hip_amp_gd_run_3d": (C.c_int, [_H, _P, _P, _U64, C.c_int, C.c_float, _P]),
 }
 
 
@@ -430,6 +436,42 @@ class HipImpl:
     def support_from_constraint(self, S, mask, num: int) -> bool:
         return self._check(lib().cpra_hip_support_from_constraint(self._h, _addr(S), _addr(mask),
                                                                   num))
+
+    # ---- amplitude loss L = sum (|F x| - A)^2 and its adjoint -------------
+    def _amp_batch(self, amp, amp_batch, vol: bool):
+        if amp_batch is not None:
+            return amp_batch
+        per = self.m * self.n * (self.l if vol else 1)
+        return amp.numel() // per
+
+    def amp_loss(self, x, amp, amp_batch=None, volume: bool = False):
+        """Per-pattern loss (batch doubles) of x under amplitudes amp."""
+        import numpy as np
+        out = np.zeros(self.batch, dtype=np.float64)
+        fn = lib().cpra_hip_amp_loss_3d if volume else lib().cpra_hip_amp_loss
+        self._check(fn(self._h, _addr(x), _addr(amp), self._amp_batch(amp, amp_batch, volume),
+                       out.ctypes.data))
+        return out
+
+    def amp_grad(self, x, amp, grad, amp_batch=None, volume: bool = False):
+        """grad <- dL/d(re x) + i dL/d(im x); returns the per-pattern loss."""
+        import numpy as np
+        out = np.zeros(self.batch, dtype=np.float64)
+        fn = lib().cpra_hip_amp_grad_3d if volume else lib().cpra_hip_amp_grad
+        self._check(fn(self._h, _addr(x), _addr(amp), self._amp_batch(amp, amp_batch, volume),
+                       _addr(grad), out.ctypes.data))
+        return out
+
+    def amp_gd_run(self, x, amp, iters: int, mu: float, amp_batch=None, volume: bool = False,
+                   want_loss: bool = True):
+        """iters fused gradient-descent steps on x in place; returns the loss
+        of the final iterate (or None)."""
+        import numpy as np
+        out = np.zeros(self.batch, dtype=np.float64) if want_loss else None
+        fn = lib().cpra_hip_amp_gd_run_3d if volume else lib().cpra_hip_amp_gd_run
+        self._check(fn(self._h, _addr(x), _addr(amp), self._amp_batch(amp, amp_batch, volume),
+                       int(iters), float(mu), out.ctypes.data if want_loss else None))
+        return out
 
 
 def device_count() -> int:

@@ -113,6 +113,35 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable);
 int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n);
 
+/* ---- amplitude loss and its adjoint (north-star addition; the reference has
+ * no loss or gradient) ------------------------------------------------------
+ * L_b = sum_k (|F x_b|_k - A_k)^2 with F the centred, unnormalised DFT of the
+ * handle's 2D (m x n) or 3D (l x m x n) geometry, A [amp_batch][...] real,
+ * broadcast over the batch by b % amp_batch.  The gradient follows the
+ * convention dL/d(re x) + i dL/d(im x):
+ *   grad = 2 F^H (F x - A (F x)/|F x|),  residual 0 where F x = 0,
+ * computed from x at every call (nothing of F x is kept between calls, also
+ * for 3D volumes).  loss_host: batch doubles (may be NULL for the gradient).
+ * F32 and F64 handles; m, n (and l) even. grad must not alias x. */
+int cpra_hip_amp_loss(cpra_hip_handle h, const void* x, const void* amp, uint64_t amp_batch,
+                      double* loss_host);
+int cpra_hip_amp_grad(cpra_hip_handle h, const void* x, const void* amp, uint64_t amp_batch,
+                      void* grad, double* loss_host);
+int cpra_hip_amp_loss_3d(cpra_hip_handle h, const void* x, const void* amp, uint64_t amp_batch,
+                         double* loss_host);
+int cpra_hip_amp_grad_3d(cpra_hip_handle h, const void* x, const void* amp, uint64_t amp_batch,
+                         void* grad, double* loss_host);
+
+/* `iters` steps of gradient descent x <- x - mu * grad on the amplitude loss,
+ * in place (F32 handles).  One fused pass per step along the outer axis (m in
+ * 2D, l in 3D, which must be 64/128/192/256/384/512) over the state kept as
+ * the DFT over the inner axes.  loss_host (may be NULL) receives the loss of
+ * the final iterate, batch doubles. */
+int cpra_hip_amp_gd_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                        int iters, float mu, double* loss_host);
+int cpra_hip_amp_gd_run_3d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                           int iters, float mu, double* loss_host);
+
 #ifdef __cplusplus
 }
 #endif
