@@ -403,6 +403,7 @@ int cpra_hip_destroy(cpra_hip_handle h)
     h->ws_backup.release();
     h->ws_rows.release();
     h->ws_dm.release();
+    h->ws_trig.release();
     if (h->aux_stream) (void)hipStreamDestroy(h->aux_stream);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;

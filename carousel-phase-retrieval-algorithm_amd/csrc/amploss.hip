@@ -319,7 +319,9 @@ static void gd_run(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batc
     const float step = (float)(2.0 * (double)mu * (double)W);  // 2 mu * (size of the inner axes)
     const unsigned grid = (unsigned)tiles;
     const int remap = (grid % 8u == 0u) ? 1 : 0;
+    KernelTimer kt;
     auto launch = [&](int mode, float stp) {
+        kt.begin(h, st, KT_GD);
         switch (len) {
         case 64: gd_launch<64>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
         case 128: gd_launch<128>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
@@ -329,6 +331,7 @@ static void gd_run(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batc
         case 512: gd_launch<512>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
         }
         check_launch("k_gd_cols");
+        kt.end(h, st);
     };
     for (int i = 0; i < iters; ++i) launch(GD_STEP, step);
     if (loss_host) {
@@ -348,6 +351,7 @@ static void gd_run(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batc
         CPRA_CHECK_HIP(hipMemcpyAsync(loss_host, tot, sizeof(double) * B, hipMemcpyDeviceToHost, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
     }
+    kt.collect(h);
 }
 
 }  // namespace cpra

@@ -1,0 +1,286 @@
+// carousel.hip -- 3D carousel reconstruction: batched 2D HIO on P projections
+// made consistent through one 3D object every outer iteration.
+//
+// Projection p (angle th_p about the z axis) is the sum of the object along the
+// beam; by the Fourier-slice theorem its centred 2D DFT is the central slice of
+// the object's centred 3D DFT that merge3d.hip samples (rows u along z, columns
+// v along (cos th, sin th)).  One outer iteration:
+//   1. `inner` HIO iterations on every projection (measured amplitudes, 2D
+//      support, real-positive inside);
+//   2. F_p = Forward2D(x_p);
+//   3. V = R^T(F) / R^T(1)                       (gather-trilinear merge);
+//   4. v = Backward3D(V); v <- (Re v)^+ * S3 (Im v <- 0); V = Forward3D(v);
+//   5. F_p = R V                                  (re-projection);
+//   6. x_p = Backward2D(F_p).
+// All projections advance together (the Jacobi form of the reference's
+// sequential sweep, src/CPRA_MKL.cpp:58-124).
+//
+// MI355X mapping: the projections stay in the row-spectral domain
+// Y = DFT_rows(x) for the whole run, so steps 2 and 6 are one column pass each
+// (the row transforms cancel against the HIO engine's).  Step 4 is five passes,
+// not seven: the z-line kernel runs IDFT_z, the constraint and DFT_z on each
+// line in registers, between the inverse and forward (y, x) passes.  The
+// outer body (12 launches for inner = 1) is captured once in a hipGraph and
+// replayed; its working set (the projections, the transposed slices and the
+// volume: 56 MB at 180 x 128^2 and 128^3) stays in the Infinity Cache.
+#include <cmath>
+
+#include "common.hpp"
+#include "fft_core.hpp"
+#include "plans.hpp"
+
+namespace cpra {
+
+enum ZlMode { ZL_ROUNDTRIP = 0, ZL_INVERSE = 1 };
+
+// z-lines of the volume [L][W] (W = M*N points per plane), after the inverse
+// x and y passes: IDFT_z (scale), the real-positive support projection, and
+// (ZL_ROUNDTRIP) DFT_z back.  Block = C consecutive lines.
+template <int L, int MODE>
+__global__ __launch_bounds__(SizeCfg<L>::TPB) void k_zline_constraint(c2<float>* V, const uint8_t* __restrict__ S,
+                                                                     int64_t W, float scale)
+{
+    using Vt = c2<float>;
+    using PL = typename SizeCfg<L>::P;
+    constexpr int E = PL::E, TT = PL::T, C = SizeCfg<L>::C;
+    __shared__ Vt lds[L * C];
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const int64_t col0 = (int64_t)blockIdx.x * C;
+    Vt* Vp = V + col0;
+    const uint8_t* Sp = S + col0;
+    const unsigned toff = (unsigned)t * (unsigned)W + (unsigned)c;  // < T*W <= 2^24
+    // the volume is in natural order along z; rot() places the centring shift
+    auto zbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)W; };
+    Vt v[E];
+    uint8_t sm[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = (Vp + zbase(m))[toff];
+        sm[m] = (Sp + zbase(m))[toff];
+    });
+    LdsCol<Vt, C, false> acc{lds + c};
+    fft_line<PL, +1>(v, t, g_tw<L>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float re = v[m].x * scale;
+        v[m] = mk<Vt>(sm[m] && re > 0.f ? re : 0.f, 0.f);
+    });
+    if constexpr (MODE == ZL_ROUNDTRIP) fft_line<PL, -1>(v, t, g_tw<L>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        (Vp + zbase(m))[toff] = v[m];
+    });
+}
+
+static int zline_tile(int n)
+{
+    switch (n) {
+    case 64: return SizeCfg<64>::C;
+    case 128: return SizeCfg<128>::C;
+    case 192: return SizeCfg<192>::C;
+    case 256: return SizeCfg<256>::C;
+    case 384: return SizeCfg<384>::C;
+    case 512: return SizeCfg<512>::C;
+    }
+    return 0;
+}
+
+template <int L>
+static void zline_n(hipStream_t st, int mode, c2<float>* V, const uint8_t* S, int64_t W, float scale)
+{
+    const unsigned grid = (unsigned)(W / SizeCfg<L>::C);
+    if (mode == ZL_ROUNDTRIP)
+        hipLaunchKernelGGL((k_zline_constraint<L, ZL_ROUNDTRIP>), dim3(grid), dim3(SizeCfg<L>::TPB), 0, st, V, S, W, scale);
+    else
+        hipLaunchKernelGGL((k_zline_constraint<L, ZL_INVERSE>), dim3(grid), dim3(SizeCfg<L>::TPB), 0, st, V, S, W, scale);
+}
+
+static void zline_launch(hipStream_t st, int n, int mode, c2<float>* V, const uint8_t* S)
+{
+    const int64_t W = (int64_t)n * n;
+    const float scale = (float)(1.0 / ((double)n * n * n));
+    switch (n) {
+    case 64: zline_n<64>(st, mode, V, S, W, scale); break;
+    case 128: zline_n<128>(st, mode, V, S, W, scale); break;
+    case 192: zline_n<192>(st, mode, V, S, W, scale); break;
+    case 256: zline_n<256>(st, mode, V, S, W, scale); break;
+    case 384: zline_n<384>(st, mode, V, S, W, scale); break;
+    case 512: zline_n<512>(st, mode, V, S, W, scale); break;
+    }
+    check_launch("k_zline_constraint");
+}
+
+struct Carousel {
+    cpra_hip_ctx* h;
+    int n, P;
+    c2<float>* Y;  // projections, row-spectral state [P][n][n]
+    const float* amp;
+    int64_t amp_batch;
+    const uint8_t* sup2d;
+    const float* angles;
+    const uint8_t* sup3d;
+    c2<float>* vol;  // [n][n][n]
+    cpra_hip_hio_params hp;
+    int inner;
+
+    // inverse (x, y) passes, fused z-line constraint, forward (y, x) passes
+    void volume_constraint(int mode)
+    {
+        const int64_t W = (int64_t)n * n;
+        fft_pass_contig(h, vol, n, (int64_t)n * n, +1, true, 1.0, false);            // x
+        fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, +1, true, 1.0, false);  // y
+        zline_launch(h->stream, n, mode, vol, sup3d);                                // z
+        if (mode == ZL_ROUNDTRIP) {
+            fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, -1, true, 1.0, false);
+            fft_pass_contig(h, vol, n, (int64_t)n * n, -1, true, 1.0, false);
+        }
+    }
+
+    // one outer iteration on h->stream
+    void outer(KernelTimer* kt)
+    {
+        const int64_t W = (int64_t)n * n;
+        for (int i = 0; i < inner; ++i) engine_hio_iteration(h, h->stream, Y, amp, amp_batch, sup2d, &hp, kt);
+        // Y -> F = Forward2D(x): the column DFT of the row-spectral state
+        if (kt) kt->begin(h, h->stream, KT_PFFT);
+        fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, -1, true, 1.0, false);
+        if (kt) kt->end(h, h->stream);
+        if (kt) kt->begin(h, h->stream, KT_MERGE);
+        merge_c64(h, Y, vol, angles, n, P);
+        if (kt) kt->end(h, h->stream);
+        if (kt) kt->begin(h, h->stream, KT_VOL);
+        volume_constraint(ZL_ROUNDTRIP);
+        if (kt) kt->end(h, h->stream);
+        if (kt) kt->begin(h, h->stream, KT_EXTRACT);
+        extract_c64(h, vol, Y, angles, n, P);
+        if (kt) kt->end(h, h->stream);
+        // F -> Y = DFT_rows(Backward2D(F)): the column IDFT scaled 1/n
+        if (kt) kt->begin(h, h->stream, KT_PFFT);
+        fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, +1, true, 1.0 / (double)n, false);
+        if (kt) kt->end(h, h->stream);
+    }
+};
+
+template <typename F> static int cguard(cpra_hip_ctx* h, F&& f)
+{
+    try {
+        return f();
+    } catch (const Status& s) {
+        if (h) h->last_error = s.msg;
+        return s.code;
+    } catch (const std::exception& e) {
+        if (h) h->last_error = e.what();
+        return CPRA_HIP_ERR_HIP;
+    }
+}
+
+}  // namespace cpra
+
+using namespace cpra;
+
+extern "C" {
+
+void cpra_hip_carousel_default_params(cpra_hip_carousel_params* p)
+{
+    if (!p) return;
+    p->outer_iters = 50;
+    p->inner_iters = 1;
+    p->beta = 0.9f;
+    p->dc_skip_zero = 1;
+    p->use_graph = 1;
+}
+
+int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                            const uint8_t* support2d, const float* angles, const uint8_t* support3d,
+                            void* vol, const cpra_hip_carousel_params* p, float* rfactor_host)
+{
+    return cguard(h, [&]() -> int {
+        CPRA_ARG(h && x && amp && support2d && angles && support3d && vol && p,
+                 "cpra_hip_carousel3d_run: NULL argument");
+        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0,
+                 "cpra_hip_carousel3d_run: amp_batch must divide the projection count");
+        CPRA_ARG(p->outer_iters >= 0 && p->inner_iters >= 0, "cpra_hip_carousel3d_run: negative iteration count");
+        CPRA_ARG(x != vol, "cpra_hip_carousel3d_run: vol must not alias x");
+        if (h->dtype != CPRA_HIP_F32) fail(CPRA_HIP_ERR_UNSUPPORTED, "cpra_hip_carousel3d_run: FP32 handles only");
+        if (h->m != h->n)
+            fail(CPRA_HIP_ERR_UNSUPPORTED, "cpra_hip_carousel3d_run: projections must be square (m == n)");
+        const int n = (int)h->n;
+        if (!zline_tile(n))
+            fail(CPRA_HIP_ERR_UNSUPPORTED, "cpra_hip_carousel3d_run: n=%d not in {64,128,192,256,384,512}", n);
+        Carousel cr;
+        cr.h = h;
+        cr.n = n;
+        cr.P = (int)h->batch;
+        cr.Y = static_cast<c2<float>*>(x);
+        cr.amp = amp;
+        cr.amp_batch = (int64_t)amp_batch;
+        cr.sup2d = support2d;
+        cr.angles = angles;
+        cr.sup3d = support3d;
+        cr.vol = static_cast<c2<float>*>(vol);
+        cpra_hip_hio_default_params(&cr.hp);
+        cr.hp.beta = p->beta;
+        cr.hp.real_positive = 1;
+        cr.hp.dc_skip_zero = p->dc_skip_zero;
+        cr.hp.error_reduction = 0;
+        cr.hp.sw_every = 0;
+        cr.inner = p->inner_iters;
+        hipStream_t st = h->stream;
+        engine_hio_prepare(h, st, support2d);
+        // enter the row-spectral state of the projections
+        fft_pass_contig(h, x, n, (int64_t)cr.P * n, -1, true, 1.0, false);
+        KernelTimer kt;
+        KernelTimer* ktp = h->timing ? &kt : nullptr;
+        int done = 0;
+        if (p->outer_iters > 0) {
+            cr.outer(ktp);  // eager: sizes every workspace before any capture
+            done = 1;
+        }
+        const int rest = p->outer_iters - done;
+        if (p->use_graph && !h->timing && !debug_launches() && rest >= 2) {
+            hipStream_t gst = nullptr;
+            CPRA_CHECK_HIP(hipStreamCreateWithFlags(&gst, hipStreamNonBlocking));
+            hipEvent_t ev0, ev1;
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            h->stream = gst;
+            try {
+                CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
+                cr.outer(nullptr);
+                CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
+            } catch (...) {
+                h->stream = st;
+                throw;
+            }
+            h->stream = st;
+            CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            for (int i = 0; i < rest; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            CPRA_CHECK_HIP(hipStreamSynchronize(gst));
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+            (void)hipStreamDestroy(gst);
+        } else {
+            for (int i = 0; i < rest; ++i) cr.outer(ktp);
+        }
+        // leave: x = IDFT_rows(Y) / n; vol = the constrained real-space object
+        fft_pass_contig(h, x, n, (int64_t)cr.P * n, +1, true, 1.0 / (double)n, false);
+        if (p->outer_iters > 0) cr.volume_constraint(ZL_INVERSE);
+        kt.collect(h);
+        if (rfactor_host) {
+            const int rc = cpra_hip_rfactor(h, x, amp, amp_batch, rfactor_host);
+            if (rc < 0) return rc;
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+}  // extern "C"

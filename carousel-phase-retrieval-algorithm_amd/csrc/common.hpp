@@ -9,6 +9,7 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/cpra_hip.h"
 #include "../../include/cpra_hip_engine.h"
@@ -82,7 +83,7 @@ struct cpra_hip_ctx {
     // twiddle tables keyed by (n << 1) | is_double
     std::map<uint64_t, void*> twiddles;
     // engine workspaces
-    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm;
+    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm, ws_trig;
     // tuning choice per (kind, n) set by the autotuner (0 = default)
     std::map<uint64_t, int> tuning;
     cpra::EngineGraph* graph = nullptr;
@@ -131,6 +132,57 @@ void fft_pass_contig(cpra_hip_ctx* h, void* data, int n, int64_t lines, int dir,
 void fft_pass_strided(cpra_hip_ctx* h, void* data, int n, int64_t lines, int64_t es, int64_t G,
                       int64_t GS, int dir, bool shift, double scale, bool dbl);
 bool fft_fast_size(int n);
+// Event pair around one engine kernel when the handle's timing is on; the
+// elapsed time is read back after the run (kernel_timer_collect).
+struct KernelTimer {
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    void begin(cpra_hip_ctx* h, hipStream_t st, int kind)
+    {
+        if (!h->timing) return;
+        Rec r{kind, nullptr, nullptr};
+        CPRA_CHECK_HIP(hipEventCreate(&r.a));
+        CPRA_CHECK_HIP(hipEventCreate(&r.b));
+        CPRA_CHECK_HIP(hipEventRecord(r.a, st));
+        recs.push_back(r);
+    }
+    void end(cpra_hip_ctx* h, hipStream_t st)
+    {
+        if (!h->timing) return;
+        CPRA_CHECK_HIP(hipEventRecord(recs.back().b, st));
+    }
+    void collect(cpra_hip_ctx* h)
+    {
+        for (auto& r : recs) {
+            float ms = 0.f;
+            CPRA_CHECK_HIP(hipEventSynchronize(r.b));
+            CPRA_CHECK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            h->kt_ms[r.kind] += ms;
+            h->kt_n[r.kind] += 1;
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        recs.clear();
+    }
+};
+
+// kinds of cpra_hip_get_kernel_times
+enum KernelKind {
+    KT_COL = 0, KT_ROW = 1, KT_SW = 2, KT_GD = 3, KT_MERGE = 4, KT_VOL = 5, KT_EXTRACT = 6, KT_PFFT = 7
+};
+
+// engine2d.hip: one HIO iteration on the row-spectral state (see there)
+void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support);
+void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
+                          int64_t amp_batch, const uint8_t* support, const cpra_hip_hio_params* p,
+                          KernelTimer* kt);
+// merge3d.hip: complex64 gather merge R^T(s)/R^T(1) and re-projection R
+void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P);
+void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* angles, int n, int P);
+
 // frees the captured engine graphs of a handle (engine2d.hip)
 void cpra_engine_release(cpra_hip_ctx* h);
 

@@ -290,44 +290,6 @@ static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, co
     check_launch("k_rf_fin");
 }
 
-// Event pair around one engine kernel when the handle's timing is on; the
-// elapsed time is read back after the run (kernel_timer_collect).
-struct KernelTimer {
-    struct Rec {
-        int kind;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    void begin(cpra_hip_ctx* h, hipStream_t st, int kind)
-    {
-        if (!h->timing) return;
-        Rec r{kind, nullptr, nullptr};
-        CPRA_CHECK_HIP(hipEventCreate(&r.a));
-        CPRA_CHECK_HIP(hipEventCreate(&r.b));
-        CPRA_CHECK_HIP(hipEventRecord(r.a, st));
-        recs.push_back(r);
-    }
-    void end(cpra_hip_ctx* h, hipStream_t st)
-    {
-        if (!h->timing) return;
-        CPRA_CHECK_HIP(hipEventRecord(recs.back().b, st));
-    }
-    void collect(cpra_hip_ctx* h)
-    {
-        for (auto& r : recs) {
-            float ms = 0.f;
-            CPRA_CHECK_HIP(hipEventSynchronize(r.b));
-            CPRA_CHECK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-            h->kt_ms[r.kind] += ms;
-            h->kt_n[r.kind] += 1;
-            (void)hipEventDestroy(r.a);
-            (void)hipEventDestroy(r.b);
-        }
-        recs.clear();
-    }
-};
-
-enum KernelKind { KT_COL = 0, KT_ROW = 1, KT_SW = 2 };
 
 // Row flags + compacted list of the rows that hold support pixels.
 static bool sparse_rows_enabled(const cpra_hip_ctx* h)
@@ -391,6 +353,25 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
                         bc);
         if (kt) kt->end(h, st);
     }
+}
+
+// One HIO iteration on the row-spectral state Y of the handle's batch, for
+// callers that compose their own iteration body (the carousel driver).
+// engine_hio_prepare sizes the workspaces and lists the support rows; it must
+// run (eagerly) before any capture of engine_hio_iteration.
+void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support)
+{
+    engine_check_geometry(h, "carousel");
+    const Geo g = geo_of(h);
+    h->ws_z.ensure(sizeof(c2<float>) * (size_t)g.M * g.N * g.B);
+    if (sparse_rows_enabled(h)) support_rows_launch(h, st, support);
+}
+
+void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
+                          int64_t amp_batch, const uint8_t* support, const cpra_hip_hio_params* p,
+                          KernelTimer* kt)
+{
+    hio_iteration(h, st, Y, amp, amp_batch, support, p, kt);
 }
 
 template <int N>

@@ -70,9 +70,24 @@ template <typename T> struct ElemOps<c2<T>> {
 
 constexpr int kMergeMaxZPerLane = 8;  // n <= 512
 
+// (cos th_p, sin th_p) in FP64 for the slice geometry.  The footprint
+// offsets s*cos - dx and the weights derived from them are formed in FP64 and
+// rounded to T only at the end: in FP32, |s| cos th carries an absolute error
+// of ~4e-6 at s = 64, which is a large relative error on the small weights at
+// the edge of a slice's footprint, and R^T(s)/R^T(1) divides by their sum.
+template <typename T>
+__global__ void k_trig_table(const T* __restrict__ angles, int P, double* __restrict__ trig)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const double th = (double)angles[p];
+    trig[2 * p] = cos(th);
+    trig[2 * p + 1] = sin(th);
+}
+
 // E: element type (T or c2<T>), T: real type of the weights/angles.
 template <typename E, typename T>
-__global__ __launch_bounds__(256) void k_merge_gather(const E* __restrict__ sT, const T* __restrict__ angles,
+__global__ __launch_bounds__(256) void k_merge_gather(const E* __restrict__ sT, const double* __restrict__ trig,
                                                      int P, int n, int XT, E* __restrict__ vol)
 {
     using Ops = ElemOps<E>;
@@ -85,27 +100,26 @@ __global__ __launch_bounds__(256) void k_merge_gather(const E* __restrict__ sT, 
     T* tw = reinterpret_cast<T*>(tile + (int64_t)n * XT);                       // [XT]
     const int y = blockIdx.y;
     const int x0 = blockIdx.x * XT;
-    const T c = (T)(n / 2);
+    const int c = n / 2;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for (int xl = wave; xl < XT; xl += 4) {
         const int x = x0 + xl;
-        const T dx = (T)x - c, dy = (T)y - c;
+        const double dx = (double)(x - c), dy = (double)(y - c);
         E acc[kMergeMaxZPerLane];
 #pragma unroll
         for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
         T wsum = 0;
         for (int p0 = 0; p0 < P; p0 += 64) {
             const int p = p0 + lane;
-            T cs = 0, sn = 0;
+            double cs = 0, sn = 0;
             bool live = p < P;
             if (live) {
-                const T th = angles[p];
-                cs = cos(th);
-                sn = sin(th);
-                const T dperp = -dx * sn + dy * cs;
-                live = fabs(dperp) < (T)1.4142135623730951;
+                cs = trig[2 * p];
+                sn = trig[2 * p + 1];
+                const double dperp = -dx * sn + dy * cs;
+                live = fabs(dperp) < 1.4142135623730951;
             }
-            const T s0 = dx * cs + dy * sn;
+            const double s0 = dx * cs + dy * sn;
             const int sb = (int)floor(s0) - 1;
             int cnt_base = 0;
 #pragma unroll
@@ -114,9 +128,9 @@ __global__ __launch_bounds__(256) void k_merge_gather(const E* __restrict__ sT, 
                 const int v = s + n / 2;
                 T w = 0;
                 if (live && v >= 0 && v < n) {
-                    const T ex = (T)s * cs - dx, ey = (T)s * sn - dy;
-                    const T wx = (T)1 - fabs(ex), wy = (T)1 - fabs(ey);
-                    if (wx > (T)0 && wy > (T)0) w = wx * wy;
+                    const double ex = (double)s * cs - dx, ey = (double)s * sn - dy;
+                    const double wx = 1.0 - fabs(ex), wy = 1.0 - fabs(ey);
+                    if (wx > 0.0 && wy > 0.0) w = (T)(wx * wy);
                 }
                 const bool has = w > (T)0;
                 const uint64_t m = __ballot(has);
@@ -163,21 +177,22 @@ __global__ __launch_bounds__(256) void k_merge_gather(const E* __restrict__ sT, 
 
 // R: slices[p][u][v] = bilinear sample of the plane z = u at the slice point.
 template <typename E, typename T>
-__global__ __launch_bounds__(256) void k_slice_extract(const E* __restrict__ vol, const T* __restrict__ angles,
+__global__ __launch_bounds__(256) void k_slice_extract(const E* __restrict__ vol, const double* __restrict__ trig,
                                                       int P, int n, E* __restrict__ out)
 {
     using Ops = ElemOps<E>;
     const int64_t total = (int64_t)P * n * n;
-    const T c = (T)(n / 2);
+    const double c = (double)(n / 2);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int v = (int)(i % n);
         const int u = (int)((i / n) % n);
         const int p = (int)(i / ((int64_t)n * n));
-        const T th = angles[p];
-        const T px = c + ((T)v - c) * cos(th), py = c + ((T)v - c) * sin(th);
-        const T fx = floor(px), fy = floor(py);
-        const T ax = px - fx, ay = py - fy;
+        // the sample point in FP64 (see k_trig_table), the weights in T
+        const double pxd = c + ((double)v - c) * trig[2 * p], pyd = c + ((double)v - c) * trig[2 * p + 1];
+        const double fxd = floor(pxd), fyd = floor(pyd);
+        const T fx = (T)fxd, fy = (T)fyd;
+        const T ax = (T)(pxd - fxd), ay = (T)(pyd - fyd);
         E acc = Ops::zero();
 #pragma unroll
         for (int dy = 0; dy < 2; ++dy)
@@ -197,8 +212,13 @@ static void merge_launch(cpra_hip_ctx* h, const void* slices, void* vol, const v
                          int P)
 {
     const size_t per = (size_t)P * n * n;
-    h->ws_misc.ensure(per * sizeof(E));
-    E* sT = static_cast<E*>(h->ws_misc.ptr);
+    const size_t trig_bytes = ((sizeof(double) * 2 * (size_t)P + 255) / 256) * 256;
+    h->ws_misc.ensure(trig_bytes + per * sizeof(E));
+    double* trig = static_cast<double*>(h->ws_misc.ptr);
+    E* sT = reinterpret_cast<E*>(static_cast<char*>(h->ws_misc.ptr) + trig_bytes);
+    hipLaunchKernelGGL((k_trig_table<T>), dim3((P + 63) / 64), dim3(64), 0, h->stream,
+                       static_cast<const T*>(angles), P, trig);
+    check_launch("k_trig_table");
     hipLaunchKernelGGL((k_transpose_slices<E>), dim3((n + 31) / 32, (n + 31) / 32, P), dim3(256), 0,
                        h->stream, static_cast<const E*>(slices), sT, n);
     check_launch("k_transpose_slices");
@@ -210,7 +230,7 @@ static void merge_launch(cpra_hip_ctx* h, const void* slices, void* vol, const v
     if (n % XT) fail(CPRA_HIP_ERR_UNSUPPORTED, "merge: n=%d not a multiple of the tile %d", n, XT);
     const size_t lds = lds_of(XT);
     hipLaunchKernelGGL((k_merge_gather<E, T>), dim3(n / XT, n), dim3(256), lds, h->stream, sT,
-                       static_cast<const T*>(angles), P, n, XT, static_cast<E*>(vol));
+                       trig, P, n, XT, static_cast<E*>(vol));
     check_launch("k_merge_gather");
 }
 
@@ -219,9 +239,13 @@ static void extract_launch(cpra_hip_ctx* h, const void* vol, void* slices, const
                            int P)
 {
     const int64_t total = (int64_t)P * n * n;
+    h->ws_trig.ensure(sizeof(double) * 2 * (size_t)P);
+    double* trig = static_cast<double*>(h->ws_trig.ptr);
+    hipLaunchKernelGGL((k_trig_table<T>), dim3((P + 63) / 64), dim3(64), 0, h->stream,
+                       static_cast<const T*>(angles), P, trig);
+    check_launch("k_trig_table");
     hipLaunchKernelGGL((k_slice_extract<E, T>), dim3(grid_1d(total, 256, 1 << 16)), dim3(256), 0,
-                       h->stream, static_cast<const E*>(vol), static_cast<const T*>(angles), P, n,
-                       static_cast<E*>(slices));
+                       h->stream, static_cast<const E*>(vol), trig, P, n, static_cast<E*>(slices));
     check_launch("k_slice_extract");
 }
 
@@ -232,6 +256,19 @@ static void check_merge_geometry(uint64_t m, uint64_t n, uint64_t p, uint64_t l,
         fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: the carousel geometry needs a cube (m == n == l)", who);
     if (n < 2 || n > 64 * kMergeMaxZPerLane || (n & 1))
         fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: n must be even and <= %d", who, 64 * kMergeMaxZPerLane);
+}
+
+// complex64 merge / extraction on the handle's stream (the carousel driver)
+void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P)
+{
+    check_merge_geometry(n, n, P, n, "carousel merge");
+    merge_launch<c2<float>, float>(h, slices, vol, angles, n, P);
+}
+
+void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* angles, int n, int P)
+{
+    check_merge_geometry(n, n, P, n, "carousel extraction");
+    extract_launch<c2<float>, float>(h, vol, slices, angles, n, P);
 }
 
 template <typename F> static int mguard(cpra_hip_ctx* h, F&& f)

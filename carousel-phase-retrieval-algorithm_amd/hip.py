@@ -57,6 +57,16 @@ class DmParams(C.Structure):
     ]
 
 
+class CarouselParams(C.Structure):
+    _fields_ = [
+        ("outer_iters", C.c_int32),
+        ("inner_iters", C.c_int32),
+        ("beta", C.c_float),
+        ("dc_skip_zero", C.c_int32),
+        ("use_graph", C.c_int32),
+    ]
+
+
 class HioParams(C.Structure):
     _fields_ = [
         ("iters", C.c_int32),
@@ -125,6 +135,9 @@ _SIGS = {
     "cpra_hip_amp_grad_3d": (C.c_int, [_H, _P, _P, _U64, _P, _P]),
     "cpra_hip_amp_gd_run": (C.c_int, [_H, _P, _P, _U64, C.c_int, C.c_float, _P]),
     "cpra_hip_amp_gd_run_3d": (C.c_int, [_H, _P, _P, _U64, C.c_int, C.c_float, _P]),
+    "cpra_hip_carousel_default_params": (None, [C.POINTER(CarouselParams)]),
+    "cpra_hip_carousel3d_run": (C.c_int, [_H, _P, _P, _U64, _P, _P, _P, _P,
+                                          C.POINTER(CarouselParams), _P]),
 }
 
 
@@ -395,6 +408,26 @@ class HipImpl:
         return self._check(lib().cpra_hip_dm_run(self._h, _addr(psi), _addr(t1), _addr(t2),
                                                  _addr(amp), ab, _addr(support), C.byref(p)))
 
+    def carousel3d_run(self, x, amp, support2d, angles, support3d, vol, amp_batch=None,
+                       want_rfactor: bool = False, **kw):
+        """3D carousel reconstruction (cpra_hip_carousel3d_run): x [P][n][n]
+        projections in/out, vol [n][n][n] receives the object.  Keyword
+        arguments set cpra_hip_carousel_params fields.  Returns the final
+        R-factors (P floats) when want_rfactor."""
+        import numpy as np
+        p = CarouselParams()
+        lib().cpra_hip_carousel_default_params(C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n)
+        rf = np.zeros(self.batch, dtype=np.float32) if want_rfactor else None
+        self._check(lib().cpra_hip_carousel3d_run(
+            self._h, _addr(x), _addr(amp), ab, _addr(support2d), _addr(angles), _addr(support3d),
+            _addr(vol), C.byref(p), rf.ctypes.data if want_rfactor else None))
+        return rf
+
     def shrinkwrap(self, x, support, sigma: float, frac: float, blur_out=None) -> bool:
         return self._check(lib().cpra_hip_shrinkwrap(self._h, _addr(x), _addr(support), sigma,
                                                      frac, _addr(blur_out)))
@@ -427,11 +460,12 @@ class HipImpl:
         return self._check(lib().cpra_hip_set_kernel_timing(self._h, int(enable)))
 
     def kernel_times(self):
-        """{kind: (total_ms, launches)} for kind in column/row/shrinkwrap."""
-        ms = (C.c_double * 3)()
-        cnt = (C.c_int64 * 3)()
-        self._check(lib().cpra_hip_get_kernel_times(self._h, ms, cnt, 3))
-        return {k: (ms[i], cnt[i]) for i, k in enumerate(("column", "row", "shrinkwrap"))}
+        """{kind: (total_ms, launches)} per fused-kernel kind."""
+        names = ("column", "row", "shrinkwrap", "gd", "merge", "volume", "extract", "proj_fft")
+        ms = (C.c_double * len(names))()
+        cnt = (C.c_int64 * len(names))()
+        self._check(lib().cpra_hip_get_kernel_times(self._h, ms, cnt, len(names)))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
 
     def support_from_constraint(self, S, mask, num: int) -> bool:
         return self._check(lib().cpra_hip_support_from_constraint(self._h, _addr(S), _addr(mask),

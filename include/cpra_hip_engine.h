@@ -108,8 +108,11 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 /* Per-kernel timing of the engines: when enabled, engine runs launch
  * eagerly (no graph) with a hipEvent pair around every fused kernel on the
  * stream it runs on; cpra_hip_get_kernel_times returns, per kernel kind
- * (0 = column pass, 1 = row pass, 2 = one shrinkwrap update), the summed
- * milliseconds and the launch count since the last enable. */
+ * (0 = column pass, 1 = row pass, 2 = one shrinkwrap update, 3 = fused
+ * gradient-descent pass, 4 = 2D->3D merge, 5 = 3D volume constraint (its
+ * five FFT passes), 6 = slice extraction, 7 = column pass of the carousel's
+ * projections), the summed milliseconds and the launch count since the last
+ * enable. */
 int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable);
 int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n);
 
@@ -141,6 +144,36 @@ int cpra_hip_amp_gd_run(cpra_hip_handle h, void* x, const float* amp, uint64_t a
                         int iters, float mu, double* loss_host);
 int cpra_hip_amp_gd_run_3d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                            int iters, float mu, double* loss_host);
+
+/* ---- 3D carousel (north-star addition) --------------------------------------
+ * Projections of one real object at angles th_p about the z axis, their
+ * diffraction amplitudes measured.  One outer iteration: inner_iters HIO
+ * iterations on every projection (real-positive inside support2d); F_p =
+ * Forward2D(x_p); V = gather-trilinear merge of the F_p as central slices of
+ * the object's centred 3D DFT (cpra_hip_complex2d_to_3d geometry: slice rows
+ * along z, columns along (cos th, sin th)); v = Backward3D(V), v <- (Re v)^+
+ * inside support3d and 0 elsewhere; V = Forward3D(v); F_p = slices of V
+ * (cpra_hip_slice_extract); x_p = Backward2D(F_p).  All projections advance
+ * together (the Jacobi form of the reference's sequential sweep,
+ * src/CPRA_MKL.cpp:58-124).
+ * h: F32 handle with m == n in {64,128,192,256,384,512} and batch = P.
+ * x [P][n][n] complex64 (in/out), amp [amp_batch][n][n], support2d u8
+ * [P][n][n], angles [P] float radians, support3d u8 [n][n][n] (device
+ * pointers).  vol [n][n][n] complex64 receives the constrained real-space
+ * object of the last iteration.  rfactor_host (NULL or P floats): R-factor of
+ * the final projections. */
+typedef struct {
+    int32_t outer_iters;   /* outer iterations                                  */
+    int32_t inner_iters;   /* HIO iterations on the projections per outer one   */
+    float beta;            /* HIO feedback                                      */
+    int32_t dc_skip_zero;  /* 1: A == 0 leaves the Fourier pixel unconstrained  */
+    int32_t use_graph;     /* capture one outer iteration in a hipGraph, replay */
+} cpra_hip_carousel_params;
+
+void cpra_hip_carousel_default_params(cpra_hip_carousel_params* p);
+int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
+                            const uint8_t* support2d, const float* angles, const uint8_t* support3d,
+                            void* vol, const cpra_hip_carousel_params* p, float* rfactor_host);
 
 #ifdef __cplusplus
 }

@@ -374,3 +374,26 @@ def amp_grad_3d(x, A):
     g = np.zeros_like(x)
     lib().orc_amp_grad_3d(_ptr(x), _ptr(A), x.shape[-1], _ptr(g))
     return g
+
+
+def carousel3d(x, A, support2d, angles, support3d, outer, inner=1, beta=0.9, dc_skip_zero=True):
+    """FP64 restatement of one 3D carousel run (the north-star carousel: the
+    reference's Jacobi-form sweep, src/CPRA_MKL.cpp:58-124, with the 3D
+    consistency step): per outer iteration, `inner` HIO steps on every
+    projection (orc_hio_step, real-positive), F = Forward2D(x), V = slice merge
+    (orc_slice_merge, complex), v = Backward3D(V), v <- max(Re v, 0) inside
+    support3d and 0 elsewhere, V = Forward3D(v), F = slice extraction,
+    x = Backward2D(F).  Returns (x, v) with v the constrained real-space
+    object of the last iteration.  Composed from functions each pinned on
+    its own."""
+    x = _c128(x).copy()
+    s3 = np.asarray(support3d) != 0
+    v = None
+    for _ in range(outer):
+        for _ in range(inner):
+            x = hio_step(x, A, support2d, beta, True, dc_skip_zero)
+        V = slice_merge(forward2d(x), angles, True)
+        w = backward3d(V)
+        v = np.where(s3 & (w.real > 0), w.real, 0.0).astype(np.complex128)
+        x = backward2d(slice_extract(forward3d(v), angles, True))
+    return x, v

@@ -32,11 +32,14 @@ def test_merge_matches_oracle(hipmod, oracle, gpu, n, P, cplx):
         h.Sync()
     ref = oracle.slice_merge(s, ang.astype(np.float32).astype(np.float64), complex_data=cplx)
     got = vol.numpy()
-    # voxels whose total weight is tiny are sensitive to the FP32 footprint
+    # the footprint geometry is FP64 on both sides, so also the voxels at the
+    # edge of the covered set (tiny total weight) agree
     _, w = oracle.slice_scatter(np.ones((P, n, n)), ang.astype(np.float32).astype(np.float64),
                                 complex_data=False)
-    sel = w > 1e-3
-    assert relerr(got[sel], ref[sel]) < 1e-5
+    assert relerr(got, ref) < 1e-5
+    edge = (w > 0) & (w < 1e-2)
+    if edge.any():
+        assert np.abs(got[edge] - ref[edge]).max() <= 1e-5 * np.abs(ref).max()
     assert np.all(got[w == 0] == 0)
 
 
