@@ -146,6 +146,216 @@ print(json.dumps({
 """
 
 
+def kernel_entry(ms, cnt, iters, bytes_per_iter, launches_per_iter=1):
+    """One per-kernel roofline record from event-timed eager launches."""
+    per_iter = ms / max(iters, 1)
+    ach = bytes_per_iter / (per_iter * 1e-3) / 1e9 if per_iter > 0 else 0.0
+    return {"avg_ms": ms / max(cnt, 1), "launches": cnt, "ms_per_iteration": per_iter,
+            "bytes_per_launch": int(bytes_per_iter // max(launches_per_iter, 1)),
+            "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
+def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=5, timing_iters=5, fp64=True):
+    """3D carousel: 180 projections of 128^2 (12 random ellipsoids in 128^3,
+    Poisson noise at 1e7 photons), 50 outer iterations of [1 HIO iteration on
+    every projection, merge, 3D constraint, re-projection]; graph-captured
+    outer body.  Returns a result record (value = outer iterations / s)."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    d = synth.make_carousel(n, P, config=2)
+    log(f"bench[carousel]: data ready in {time.perf_counter() - t0:.1f} s")
+    dev = {k: hip.to_device(d[k]) for k in ("x0", "amp", "support2d", "angles", "support3d")}
+    vol = hip.zeros((n, n, n), np.complex64)
+    h = hip.HipImpl(n, n, 1, P)
+    run = lambda k, g=1: h.carousel3d_run(dev["x0"], dev["amp"], dev["support2d"], dev["angles"],
+                                          dev["support3d"], vol, outer_iters=k, inner_iters=1,
+                                          beta=0.9, use_graph=g)
+    run(warmup)
+    h.Sync()
+    t0 = time.perf_counter()
+    run(outer)
+    h.Sync()
+    el = time.perf_counter() - t0
+    its = outer / el
+    log(f"bench[carousel]: {outer} outer iterations in {el * 1e3:.2f} ms -> {its:.1f} it/s")
+    h.set_kernel_timing(True)
+    run(timing_iters, 0)
+    h.Sync()
+    kt = h.kernel_times()
+    h.set_kernel_timing(False)
+    px2, vx = P * n * n, n ** 3
+    spec = {  # kind -> algorithmic bytes per outer iteration, launches per iteration
+        "column": (px2 * COL_BYTES_PER_PX, 1),
+        "row": (px2 * ROW_BYTES_PER_PX, 1),
+        "proj_fft": (2 * px2 * 16, 2),                 # column DFT + column IDFT
+        "merge": (px2 * 8 * 3 + vx * 8, 3),            # transpose r+w, gather r, volume w
+        "volume": (5 * vx * 16 + vx, 5),               # five FFT passes + the support mask
+        "extract": (px2 * 8 + vx * 8, 2),
+    }
+    kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}
+    dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
+    algo = px2 * HIO_ALGO_BYTES_PER_PX + (px2 * 8 + vx * 8) * 2 + vx * 17
+    rec = {
+        "metric": "carousel outer iters/s (180 x 128^2 projections, 128^3 volume)",
+        "value": round(its, 2), "unit": "iters/s", "steps": outer, "warmup": warmup,
+        "ms_per_step": round(1e3 * el / outer, 4),
+        "config": {"workload": f"3D carousel, {P} projections of {n}x{n} complex64, {n}^3 volume, "
+                               "1 HIO iteration (beta 0.9) + merge + 3D constraint + "
+                               "re-projection per outer iteration", "graph": True},
+        "algorithmic_bytes_per_iteration": algo,
+        "hbm_frac_algorithmic": round(algo * its / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern[dom]["frac"],
+                     "kernels": kern},
+    }
+    if fp64:
+        o = ge.load_oracle()
+        x1 = hip.to_device(d["x0"])
+        with hip.HipImpl(n, n, 1, P) as h1:
+            h1.carousel3d_run(x1, dev["amp"], dev["support2d"], dev["angles"], dev["support3d"], vol,
+                              outer_iters=1, use_graph=0)
+            h1.Sync()
+        xr, vr = o.carousel3d(d["x0"], d["amp"].astype(np.float64), d["support2d"],
+                              d["angles"].astype(np.float64), d["support3d"], 1)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        rec["fp64_check"] = {"outer_iters": 1, "projections_rel_l2": rel(x1.numpy(), xr),
+                             "volume_rel_l2": rel(vol.numpy(), vr)}
+    h.close()
+    return rec
+
+
+def bench_gd2d(hip, synth, ge, P=256, n=256, steps=500, warmup=20, timing_iters=50, fp64=True):
+    """Amplitude-loss gradient descent: 256 patterns of 256^2 (objects as the
+    batched HIO workload), 500 steps of x <- x - mu grad with mu = 0.5/max(I),
+    one fused pass per step."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    d = synth.make_2d_batch(P, n, 30.0 * n / 256, 50.0 * n / 256, 1e6, starts_per_pattern=1, config=4)
+    log(f"bench[gd2d]: data ready in {time.perf_counter() - t0:.1f} s")
+    A32 = d["amp"]
+    mu = 0.5 / float((A32.astype(np.float64) ** 2).max())
+    # start: a random real guess inside the autocorrelation support (a start
+    # with the measured magnitudes, like the HIO one, is already a minimiser)
+    rng = np.random.default_rng(4000)
+    x0 = (d["support"] * rng.random(d["support"].shape)).astype(np.complex64)
+    x = hip.to_device(x0)
+    amp = hip.to_device(A32)
+    h = hip.HipImpl(n, n, 1, P)
+    L0 = h.amp_loss(x, amp)
+    h.amp_gd_run(x, amp, warmup, mu, want_loss=False)
+    h.Sync()
+    t0 = time.perf_counter()
+    h.amp_gd_run(x, amp, steps, mu, want_loss=False)
+    h.Sync()
+    el = time.perf_counter() - t0
+    its = steps / el
+    log(f"bench[gd2d]: {steps} steps in {el * 1e3:.1f} ms -> {its:.1f} steps/s")
+    h.set_kernel_timing(True)
+    h.amp_gd_run(x, amp, timing_iters, mu, want_loss=False)
+    h.Sync()
+    ms, cnt = h.kernel_times()["gd"]
+    h.set_kernel_timing(False)
+    px = P * n * n
+    k = kernel_entry(ms, cnt, timing_iters, px * 20)
+    rec = {
+        "metric": "amplitude-loss GD steps/s (256 x 256^2)",
+        "value": round(its, 2), "unit": "steps/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * el / steps, 4),
+        "config": {"workload": f"gradient descent on sum(|F x| - A)^2, {P} x {n}x{n} complex64, "
+                               f"mu = 0.5/max(I)", "patterns": P, "size": n},
+        "hbm_frac_algorithmic_20Bpx": round(px * 20 * its / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": f"k_gd_cols<{n}>", "achieved": k["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k["frac"], "kernels": {"gd": k}},
+    }
+    if fp64:
+        o = ge.load_oracle()
+        kk, ns = 2, 100
+        xs = hip.to_device(x0[:kk])
+        with hip.HipImpl(n, n, 1, kk) as hs:
+            L = hs.amp_gd_run(xs, hip.to_device(A32[:kk]), ns, mu)
+        xr = x0[:kk].astype(np.complex128)
+        Ar = A32[:kk].astype(np.float64)
+        for _ in range(ns):
+            xr = xr - np.float32(mu) * o.amp_grad(xr, Ar)
+        Lr = o.amp_loss(xr, Ar)
+        rec["fp64_check"] = {"patterns": kk, "steps": ns,
+                             "loss_start": [float(v) for v in L0[:kk]],
+                             "loss_gpu": [float(v) for v in L], "loss_fp64": [float(v) for v in Lr],
+                             "x_rel_l2": float(np.linalg.norm(xs.numpy() - xr) / np.linalg.norm(xr)),
+                             "loss_rel_err": float(np.max(np.abs(L - Lr) / Lr))}
+    h.close()
+    return rec
+
+
+def bench_gd3d(hip, synth, ge, n=256, steps=500, warmup=10, timing_iters=20, fp64=True):
+    """Amplitude-loss gradient descent on one 256^3 volume (12 random
+    ellipsoids), 500 steps, mu = 0.5/max(I); the gradient is recomputed from x
+    at every step (no stored transforms)."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    obj, A32 = synth.ellipsoid_amplitude_volume(n, config=4)
+    # start: a random real guess inside the ball of radius 64 that holds the
+    # object
+    rng = np.random.default_rng(4004)
+    k = np.arange(n) - n / 2
+    ball = (k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2) <= 64.0 ** 2
+    x0 = (ball * rng.random((n, n, n)) * float(obj[obj > 0].mean())).astype(np.complex64)
+    del ball
+    log(f"bench[gd3d]: data ready in {time.perf_counter() - t0:.1f} s")
+    mu = 0.5 / float((A32.astype(np.float64) ** 2).max())
+    x = hip.to_device(x0[None])
+    amp = hip.to_device(A32[None])
+    h = hip.HipImpl(n, n, n, 1)
+    L0 = h.amp_loss(x, amp, volume=True)
+    h.amp_gd_run(x, amp, warmup, mu, volume=True, want_loss=False)
+    h.Sync()
+    t0 = time.perf_counter()
+    h.amp_gd_run(x, amp, steps, mu, volume=True, want_loss=False)
+    h.Sync()
+    el = time.perf_counter() - t0
+    its = steps / el
+    log(f"bench[gd3d]: {steps} steps in {el * 1e3:.1f} ms -> {its:.1f} steps/s")
+    h.set_kernel_timing(True)
+    h.amp_gd_run(x, amp, timing_iters, mu, volume=True, want_loss=False)
+    h.Sync()
+    ms, cnt = h.kernel_times()["gd"]
+    h.set_kernel_timing(False)
+    vx = n ** 3
+    k = kernel_entry(ms, cnt, timing_iters, vx * 20)
+    rec = {
+        "metric": "amplitude-loss GD steps/s (256^3 volume)",
+        "value": round(its, 2), "unit": "steps/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * el / steps, 4),
+        "config": {"workload": f"gradient descent on sum(|F x| - A)^2, one {n}^3 complex64 volume, "
+                               "mu = 0.5/max(I), gradient recomputed every step", "size": n},
+        "hbm_frac_algorithmic_20Bpx": round(vx * 20 * its / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": f"k_gd_cols<{n}> (z-lines)", "achieved": k["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k["frac"], "kernels": {"gd": k}},
+    }
+    if fp64:
+        o = ge.load_oracle()
+        ns = 2
+        xs = hip.to_device(x0[None])
+        L = h.amp_gd_run(xs, amp, ns, mu, volume=True)
+        xr = x0.astype(np.complex128)
+        Ar = A32.astype(np.float64)
+        for _ in range(ns):
+            xr = xr - np.float32(mu) * o.amp_grad_3d(xr, Ar)
+        Lr = o.amp_loss_3d(xr, Ar)
+        rec["fp64_check"] = {"steps": ns, "loss_start": float(L0[0]), "loss_gpu": float(L[0]),
+                             "loss_fp64": float(Lr),
+                             "x_rel_l2": float(np.linalg.norm(xs.numpy()[0] - xr) / np.linalg.norm(xr)),
+                             "loss_rel_err": float(abs(L[0] - Lr) / Lr)}
+    h.close()
+    return rec
+
+
+WORKLOADS = ("hio2d", "carousel3d", "gd2d", "gd3d")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +363,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--patterns", type=int, default=512)
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--workload", choices=WORKLOADS, default="hio2d",
+                    help="hio2d (default): the headline line, which also carries the carousel "
+                         "and gradient-descent workloads under 'workloads'; the others print "
+                         "their own line only")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only (no nested carousel / gradient-descent runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--timing-iters", type=int, default=20)
@@ -176,6 +392,14 @@ def main():
     from cpra_amd import hip, synth
 
     assert hip.device_count() > 0, "bench.py needs a GPU"
+    fp64 = not args.no_fp64_check
+    if args.workload != "hio2d":
+        fn = {"carousel3d": bench_carousel, "gd2d": bench_gd2d, "gd3d": bench_gd3d}[args.workload]
+        rec = fn(hip, synth, ge, fp64=fp64)
+        rec.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "dtype": "c64", "data": "synthetic"})
+        print(json.dumps(rec), flush=True)
+        return
     n, P = args.size, args.patterns
     log(f"bench: generating {P} x {n}^2 patterns")
     t0 = time.perf_counter()
@@ -312,6 +536,12 @@ def main():
             "rfactor_fp64_self_spread": float(np.max(np.abs(rpert - rref) / rref)),
         }
         log(f"bench: fp64 check {result['fp64_check']}")
+
+    if not args.no_extra:
+        result["workloads"] = {}
+        for name, fn in (("carousel3d", bench_carousel), ("gd2d", bench_gd2d), ("gd3d", bench_gd3d)):
+            log(f"bench: workload {name}")
+            result["workloads"][name] = fn(hip, synth, ge, fp64=fp64)
 
     if not args.no_cpu:
         log("bench: CPU baseline (oracle)")

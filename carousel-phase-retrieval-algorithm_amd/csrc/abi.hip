@@ -404,6 +404,9 @@ int cpra_hip_destroy(cpra_hip_handle h)
     h->ws_rows.release();
     h->ws_dm.release();
     h->ws_trig.release();
+    h->ws_mcount.release();
+    h->ws_mlist.release();
+    h->ws_mheavy.release();
     if (h->aux_stream) (void)hipStreamDestroy(h->aux_stream);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;

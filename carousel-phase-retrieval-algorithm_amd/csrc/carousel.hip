@@ -8,7 +8,8 @@
 //   1. `inner` HIO iterations on every projection (measured amplitudes, 2D
 //      support, real-positive inside);
 //   2. F_p = Forward2D(x_p);
-//   3. V = R^T(F) / R^T(1)                       (gather-trilinear merge);
+//   3. V = R^T(F) / R^T(1)                       (gather-trilinear merge; the
+//      footprint list is built once per run, each merge is a plain gather);
 //   4. v = Backward3D(V); v <- (Re v)^+ * S3 (Im v <- 0); V = Forward3D(v);
 //   5. F_p = R V                                  (re-projection);
 //   6. x_p = Backward2D(F_p).
@@ -147,7 +148,7 @@ struct Carousel {
         fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, -1, true, 1.0, false);
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_MERGE);
-        merge_c64(h, Y, vol, angles, n, P);
+        merge_apply_c64(h, Y, vol, n, P);
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_VOL);
         volume_constraint(ZL_ROUNDTRIP);
@@ -228,6 +229,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         cr.inner = p->inner_iters;
         hipStream_t st = h->stream;
         engine_hio_prepare(h, st, support2d);
+        if (p->outer_iters > 0) merge_plan_c64(h, angles, n, cr.P);  // the footprint, once per run
         // enter the row-spectral state of the projections
         fft_pass_contig(h, x, n, (int64_t)cr.P * n, -1, true, 1.0, false);
         KernelTimer kt;

@@ -83,7 +83,10 @@ struct cpra_hip_ctx {
     // twiddle tables keyed by (n << 1) | is_double
     std::map<uint64_t, void*> twiddles;
     // engine workspaces
-    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm, ws_trig;
+    cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm, ws_trig, ws_mcount, ws_mlist, ws_mheavy;
+    // gather list of the carousel merge (merge3d.hip)
+    int mplan_n = 0, mplan_P = 0, mplan_heavy = 0;
+    int64_t mplan_entries = 0;
     // tuning choice per (kind, n) set by the autotuner (0 = default)
     std::map<uint64_t, int> tuning;
     cpra::EngineGraph* graph = nullptr;
@@ -182,6 +185,9 @@ void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float*
 // merge3d.hip: complex64 gather merge R^T(s)/R^T(1) and re-projection R
 void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P);
 void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* angles, int n, int P);
+// merge through a gather list built once per geometry (synchronises)
+void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P);
+void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P);
 
 // frees the captured engine graphs of a handle (engine2d.hip)
 void cpra_engine_release(cpra_hip_ctx* h);

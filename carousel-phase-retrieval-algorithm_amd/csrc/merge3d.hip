@@ -21,6 +21,7 @@
 // one y and stages its [n][XT] output tile in LDS so the volume rows are
 // written contiguously.
 #include <cmath>
+#include <vector>
 
 #include "common.hpp"
 
@@ -207,6 +208,66 @@ __global__ __launch_bounds__(256) void k_slice_extract(const E* __restrict__ vol
     }
 }
 
+// LDS of the plane-staged extraction: the plane (at most 128 KiB) after a
+// 4 KiB table of its angles' (cos, sin) (at most 256 angles per group); the
+// CU has 160 KiB
+constexpr size_t kExtractPlaneLds = 128 * 1024;
+constexpr size_t kExtractTrigLds = 4096;
+
+// R with the plane in LDS (n*n elements fit): block (u, angle group) loads
+// plane z = u once, coalesced, and computes the samples of its angles from
+// LDS, writing each slice row along v.  Same arithmetic as k_slice_extract.
+template <typename E, typename T>
+__global__ __launch_bounds__(1024) void k_extract_plane(const E* __restrict__ vol, const double* __restrict__ trig,
+                                                       int P, int n, int groups, E* __restrict__ out)
+{
+    using Ops = ElemOps<E>;
+    extern __shared__ unsigned char smem[];
+    double* tl = reinterpret_cast<double*>(smem);              // [2 * (p1 - p0)]
+    E* plane = reinterpret_cast<E*>(smem + kExtractTrigLds);  // [n][n]
+    const int ld = n;
+    const int u = blockIdx.x / groups, g = blockIdx.x % groups;
+    const int p0 = (int)((int64_t)g * P / groups), p1 = (int)((int64_t)(g + 1) * P / groups);
+    const E* src = vol + (int64_t)u * n * n;
+    // eight loads in flight per thread before their LDS stores
+    for (int base = 0; base < n * n; base += 8 * (int)blockDim.x) {
+        E r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = base + j * (int)blockDim.x + (int)threadIdx.x;
+            if (i < n * n) r[j] = src[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = base + j * (int)blockDim.x + (int)threadIdx.x;
+            if (i < n * n) plane[i] = r[j];
+        }
+    }
+    for (int i = threadIdx.x; i < 2 * (p1 - p0); i += blockDim.x) tl[i] = trig[2 * p0 + i];
+    __syncthreads();
+    const double c = (double)(n / 2);
+    for (int i = threadIdx.x; i < (p1 - p0) * n; i += blockDim.x) {
+        const int pl = i / n, v = i % n;
+        const int p = p0 + pl;
+        const double pxd = c + ((double)v - c) * tl[2 * pl], pyd = c + ((double)v - c) * tl[2 * pl + 1];
+        const double fxd = floor(pxd), fyd = floor(pyd);
+        const T fx = (T)fxd, fy = (T)fyd;
+        const T ax = (T)(pxd - fxd), ay = (T)(pyd - fyd);
+        E acc = Ops::zero();
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dxx = 0; dxx < 2; ++dxx) {
+                const int xi = (int)fx + dxx, yi = (int)fy + dy;
+                const T w = (dxx ? ax : (T)1 - ax) * (dy ? ay : (T)1 - ay);
+                if (xi >= 0 && xi < n && yi >= 0 && yi < n && w != (T)0)
+                    acc = Ops::fma(w, plane[yi * ld + xi], acc);
+            }
+        out[((int64_t)p * n + u) * n + v] = acc;
+    }
+}
+
+
 template <typename E, typename T>
 static void merge_launch(cpra_hip_ctx* h, const void* slices, void* vol, const void* angles, int n,
                          int P)
@@ -236,14 +297,32 @@ static void merge_launch(cpra_hip_ctx* h, const void* slices, void* vol, const v
 
 template <typename E, typename T>
 static void extract_launch(cpra_hip_ctx* h, const void* vol, void* slices, const void* angles, int n,
-                           int P)
+                           int P, bool reuse_trig = false)
 {
     const int64_t total = (int64_t)P * n * n;
-    h->ws_trig.ensure(sizeof(double) * 2 * (size_t)P);
     double* trig = static_cast<double*>(h->ws_trig.ptr);
-    hipLaunchKernelGGL((k_trig_table<T>), dim3((P + 63) / 64), dim3(64), 0, h->stream,
-                       static_cast<const T*>(angles), P, trig);
-    check_launch("k_trig_table");
+    if (!reuse_trig) {
+        h->mplan_n = 0;  // the table is rebuilt for these angles: drop the gather list
+        h->ws_trig.ensure(sizeof(double) * 2 * (size_t)P);
+        trig = static_cast<double*>(h->ws_trig.ptr);
+        hipLaunchKernelGGL((k_trig_table<T>), dim3((P + 63) / 64), dim3(64), 0, h->stream,
+                           static_cast<const T*>(angles), P, trig);
+        check_launch("k_trig_table");
+    }
+    const size_t plane_bytes = sizeof(E) * (size_t)n * n;
+    if (reuse_trig && plane_bytes <= kExtractPlaneLds) {
+        // n planes x angle groups, about one block per CU: every group reads
+        // its plane again, and the kernel is bound by that traffic (at 128^3 x
+        // 180 angles: 2 groups 19 us, 4 groups 26 us); at most 256 angles per
+        // group (the LDS table)
+        int groups = 1;
+        while ((n * groups < 256 && groups * 2 <= P) || (P + groups - 1) / groups > 256) groups *= 2;
+        hipLaunchKernelGGL((k_extract_plane<E, T>), dim3(n * groups), dim3(1024),
+                           kExtractTrigLds + plane_bytes, h->stream,
+                           static_cast<const E*>(vol), trig, P, n, groups, static_cast<E*>(slices));
+        check_launch("k_extract_plane");
+        return;
+    }
     hipLaunchKernelGGL((k_slice_extract<E, T>), dim3(grid_1d(total, 256, 1 << 16)), dim3(256), 0,
                        h->stream, static_cast<const E*>(vol), trig, P, n, static_cast<E*>(slices));
     check_launch("k_slice_extract");
@@ -258,6 +337,333 @@ static void check_merge_geometry(uint64_t m, uint64_t n, uint64_t p, uint64_t l,
         fail(CPRA_HIP_ERR_UNSUPPORTED, "%s: n must be even and <= %d", who, 64 * kMergeMaxZPerLane);
 }
 
+// ---- merge with a precomputed gather list ----------------------------------
+// The footprint of the slices on every voxel column depends on the angles
+// only, so a run that merges many times (the carousel) lists it once: per
+// column (x, y) the (p*n + v, w) entries in exactly the order k_merge_gather
+// visits them, and 1/R^T(1).  The per-merge kernel is then a gather over
+// that list (same summation order, same result bits).
+
+// walks the candidates of column (x, y) in k_merge_gather's order; `emit`
+// receives (list position, p*n + v, w) for each non-zero weight.  wl: this
+// wave's 256-entry LDS scratch; wsum (uniform) is summed over it in list
+// order per pass of 64 angles, as k_merge_gather does.
+template <typename T, typename Emit>
+__device__ __forceinline__ int footprint_walk(const double* __restrict__ trig, int P, int n, int x, int y,
+                                              int lane, T* wl, T& wsum, Emit&& emit)
+{
+    const int c = n / 2;
+    const double dx = (double)(x - c), dy = (double)(y - c);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    int total = 0;
+    for (int p0 = 0; p0 < P; p0 += 64) {
+        const int p = p0 + lane;
+        double cs = 0, sn = 0;
+        bool live = p < P;
+        if (live) {
+            cs = trig[2 * p];
+            sn = trig[2 * p + 1];
+            live = fabs(-dx * sn + dy * cs) < 1.4142135623730951;
+        }
+        const int sb = (int)floor(dx * cs + dy * sn) - 1;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int s = sb + k;
+            const int v = s + n / 2;
+            T w = 0;
+            if (live && v >= 0 && v < n) {
+                const double wx = 1.0 - fabs((double)s * cs - dx), wy = 1.0 - fabs((double)s * sn - dy);
+                if (wx > 0.0 && wy > 0.0) w = (T)(wx * wy);
+            }
+            const bool has = w > (T)0;
+            const uint64_t m = __ballot(has);
+            if (has) {
+                const int pos = cnt + __popcll(m & lt);
+                wl[pos] = w;
+                emit(total + pos, p * n + v, w);
+            }
+            cnt += __popcll(m);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int e = 0; e < cnt; ++e) wsum += wl[e];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        total += cnt;
+    }
+    return total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_merge_count(const double* __restrict__ trig, int P, int n,
+                                                     int* __restrict__ counts)
+{
+    __shared__ T wl[4][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = blockIdx.x * 4 + wave;
+    if (col >= n * n) return;  // wave-uniform: no block barrier below
+    T wsum = 0;
+    const int cnt = footprint_walk<T>(trig, P, n, col % n, col / n, lane, wl[wave], wsum,
+                                      [](int, int, T) {});
+    if (lane == 0) counts[col] = cnt;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_merge_fill(const double* __restrict__ trig, int P, int n,
+                                                    const int* __restrict__ offsets, int* __restrict__ lidx,
+                                                    T* __restrict__ lw, T* __restrict__ inv)
+{
+    __shared__ T wl[4][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = blockIdx.x * 4 + wave;
+    if (col >= n * n) return;
+    const int base = offsets[col];
+    T wsum = 0;
+    footprint_walk<T>(trig, P, n, col % n, col / n, lane, wl[wave], wsum, [&](int pos, int pv, T w) {
+        lidx[base + pos] = pv;
+        lw[base + pos] = w;
+    });
+    if (lane == 0) inv[col] = wsum > (T)0 ? (T)1 / wsum : (T)0;
+}
+
+// Columns near the rotation axis are covered by every angle (at 180 x 128^2:
+// up to 536 list entries against a mean of 5.5).  Columns with more than
+// kMergeHeavy entries get a block of their own (k_merge_heavy); the rest run
+// one wave per column with four entries' loads in flight.
+constexpr int kMergeHeavy = 32;
+
+// Block = 16 consecutive x at one y, one wave per column (a short chain of
+// dependent loads per wave); lanes over z.  Heavy columns are written as 0
+// here and overwritten by k_merge_heavy.
+template <typename E, typename T>
+__global__ __launch_bounds__(1024) void k_merge_list(const E* __restrict__ sT, const int* __restrict__ offsets,
+                                                    const int* __restrict__ lidx, const T* __restrict__ lw,
+                                                    const T* __restrict__ inv, int n, E* __restrict__ vol)
+{
+    using Ops = ElemOps<E>;
+    constexpr int XT = 16;
+    extern __shared__ unsigned char smem[];
+    E* tile = reinterpret_cast<E*>(smem);  // [n][XT]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // XCD-aware tile order (the grid is a multiple of 8): XCD x takes the
+    // contiguous tile range [x*G/8, (x+1)*G/8), a band of y, so voxel columns
+    // that read the same slice columns share that XCD's L2
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    const unsigned t_id = (b & 7u) * (G >> 3) + (b >> 3);
+    const int tiles_x = n / XT;
+    const int y = (int)t_id / tiles_x, x0 = ((int)t_id % tiles_x) * XT;
+    {
+        const int xl = wave;
+        const int col = y * n + x0 + xl;
+        int e0 = offsets[col];
+        const int e1 = offsets[col + 1];
+        E acc[kMergeMaxZPerLane];
+#pragma unroll
+        for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
+        if (e1 - e0 <= kMergeHeavy) {
+            // four entries per round: their loads are independent
+            for (; e0 + 4 <= e1; e0 += 4) {
+                const E* cp[4];
+                T w[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cp[j] = sT + (int64_t)lidx[e0 + j] * n;
+                    w[j] = lw[e0 + j];
+                }
+#pragma unroll
+                for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+                    const int z = lane + 64 * k;
+                    if (z < n) {
+                        E v[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) v[j] = cp[j][z];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[k] = Ops::fma(w[j], v[j], acc[k]);
+                    }
+                }
+            }
+            for (; e0 < e1; ++e0) {
+                const E* cp = sT + (int64_t)lidx[e0] * n;
+                const T w = lw[e0];
+#pragma unroll
+                for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+                    const int z = lane + 64 * k;
+                    if (z < n) acc[k] = Ops::fma(w, cp[z], acc[k]);
+                }
+            }
+        }
+        const T s = inv[col];
+#pragma unroll
+        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+            const int z = lane + 64 * k;
+            if (z < n) tile[z * XT + xl] = Ops::scale(acc[k], s);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n * XT; i += blockDim.x) {
+        const int z = i / XT, xl = i % XT;
+        vol[((int64_t)z * n + y) * n + x0 + xl] = tile[i];
+    }
+}
+
+// One block of 16 waves per heavy column: wave w sums the entries
+// e0 + w, e0 + w + 16, ... (four in flight), then the 16 partial columns are
+// added in LDS in a fixed order.
+constexpr int kHeavyWaves = 16;
+
+template <typename E, typename T>
+__global__ __launch_bounds__(64 * kHeavyWaves) void k_merge_heavy(
+    const E* __restrict__ sT, const int* __restrict__ offsets, const int* __restrict__ lidx,
+    const T* __restrict__ lw, const T* __restrict__ inv, const int* __restrict__ heavy, int n,
+    E* __restrict__ vol)
+{
+    using Ops = ElemOps<E>;
+    constexpr int NW = kHeavyWaves;
+    extern __shared__ unsigned char smem[];
+    E* part = reinterpret_cast<E*>(smem);  // [NW][n]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = heavy[blockIdx.x];
+    const int e0 = offsets[col], e1 = offsets[col + 1];
+    E acc[kMergeMaxZPerLane];
+#pragma unroll
+    for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
+    int e = e0 + wave;
+    for (; e + 3 * NW < e1; e += 4 * NW) {
+        const E* cp[4];
+        T w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cp[j] = sT + (int64_t)lidx[e + j * NW] * n;
+            w[j] = lw[e + j * NW];
+        }
+#pragma unroll
+        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+            const int z = lane + 64 * k;
+            if (z < n) {
+                E v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = cp[j][z];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[k] = Ops::fma(w[j], v[j], acc[k]);
+            }
+        }
+    }
+    for (; e < e1; e += NW) {
+        const E* c0 = sT + (int64_t)lidx[e] * n;
+        const T w0 = lw[e];
+#pragma unroll
+        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+            const int z = lane + 64 * k;
+            if (z < n) acc[k] = Ops::fma(w0, c0[z], acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+        const int z = lane + 64 * k;
+        if (z < n) part[wave * n + z] = acc[k];
+    }
+    __syncthreads();
+    const T s = inv[col];
+    const int y = col / n, x = col % n;
+    for (int z = threadIdx.x; z < n; z += blockDim.x) {
+        E v = part[z];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) v = Ops::fma((T)1, part[w * n + z], v);
+        vol[((int64_t)z * n + y) * n + x] = Ops::scale(v, s);
+    }
+}
+
+// Builds the gather list of (angles, n, P) in the handle (synchronises once).
+void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
+{
+    check_merge_geometry(n, n, P, n, "carousel merge");
+    if (n % 16) fail(CPRA_HIP_ERR_UNSUPPORTED, "carousel merge: n=%d not a multiple of 16", n);
+    const int cols = n * n;
+    h->ws_trig.ensure(sizeof(double) * 2 * (size_t)P);
+    double* trig = static_cast<double*>(h->ws_trig.ptr);
+    hipStream_t st = h->stream;
+    hipLaunchKernelGGL((k_trig_table<float>), dim3((P + 63) / 64), dim3(64), 0, st, angles, P, trig);
+    check_launch("k_trig_table");
+    h->ws_mcount.ensure(sizeof(int) * ((size_t)cols + 1));
+    int* counts = static_cast<int*>(h->ws_mcount.ptr);
+    hipLaunchKernelGGL((k_merge_count<float>), dim3((cols + 3) / 4), dim3(256), 0, st, trig, P, n, counts);
+    check_launch("k_merge_count");
+    std::vector<int> off((size_t)cols + 1);
+    CPRA_CHECK_HIP(hipMemcpyAsync(off.data(), counts, sizeof(int) * cols, hipMemcpyDeviceToHost, st));
+    CPRA_CHECK_HIP(hipStreamSynchronize(st));
+    int64_t run = 0;
+    for (int i = 0; i < cols; ++i) {
+        const int c = off[i];
+        off[i] = (int)run;
+        run += c;
+    }
+    off[cols] = (int)run;
+    CPRA_ARG(run < (1ll << 31), "carousel merge: gather list too long");
+    std::vector<int> heavy;
+    for (int i = 0; i < cols; ++i)
+        if (off[i + 1] - off[i] > kMergeHeavy) heavy.push_back(i);
+    h->ws_mheavy.ensure(sizeof(int) * (heavy.empty() ? 1 : heavy.size()));
+    if (!heavy.empty())
+        CPRA_CHECK_HIP(hipMemcpyAsync(h->ws_mheavy.ptr, heavy.data(), sizeof(int) * heavy.size(),
+                                      hipMemcpyHostToDevice, st));
+    CPRA_CHECK_HIP(hipMemcpyAsync(counts, off.data(), sizeof(int) * ((size_t)cols + 1), hipMemcpyHostToDevice, st));
+    const size_t ent = (size_t)(run > 0 ? run : 1);
+    const size_t idx_bytes = ((sizeof(int) * ent + 255) / 256) * 256;
+    const size_t w_bytes = ((sizeof(float) * ent + 255) / 256) * 256;
+    h->ws_mlist.ensure(idx_bytes + w_bytes + sizeof(float) * (size_t)cols);
+    char* base = static_cast<char*>(h->ws_mlist.ptr);
+    int* lidx = reinterpret_cast<int*>(base);
+    float* lw = reinterpret_cast<float*>(base + idx_bytes);
+    float* inv = reinterpret_cast<float*>(base + idx_bytes + w_bytes);
+    hipLaunchKernelGGL((k_merge_fill<float>), dim3((cols + 3) / 4), dim3(256), 0, st, trig, P, n, counts,
+                       lidx, lw, inv);
+    check_launch("k_merge_fill");
+    CPRA_CHECK_HIP(hipStreamSynchronize(st));  // `off` and `heavy` are released on return
+    if (sizeof(c2<float>) * (size_t)n * n <= kExtractPlaneLds)
+        CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_extract_plane<c2<float>, float>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(kExtractTrigLds + sizeof(c2<float>) * (size_t)n * n)));
+    h->mplan_n = n;
+    h->mplan_P = P;
+    h->mplan_entries = run;
+    h->mplan_heavy = (int)heavy.size();
+}
+
+// R^T(s)/R^T(1) of complex64 slices through the list of merge_plan_c64.
+void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P)
+{
+    CPRA_ARG(h->mplan_n == n && h->mplan_P == P, "carousel merge: no gather list for this geometry");
+    using E = c2<float>;
+    const size_t per = (size_t)P * n * n;
+    h->ws_misc.ensure(per * sizeof(E));
+    E* sT = static_cast<E*>(h->ws_misc.ptr);
+    hipLaunchKernelGGL((k_transpose_slices<E>), dim3((n + 31) / 32, (n + 31) / 32, P), dim3(256), 0,
+                       h->stream, static_cast<const E*>(slices), sT, n);
+    check_launch("k_transpose_slices");
+    const size_t ent = (size_t)(h->mplan_entries > 0 ? h->mplan_entries : 1);
+    const size_t idx_bytes = ((sizeof(int) * ent + 255) / 256) * 256;
+    const size_t w_bytes = ((sizeof(float) * ent + 255) / 256) * 256;
+    char* base = static_cast<char*>(h->ws_mlist.ptr);
+    // n / 16 * n tiles: a multiple of 8 for every n that is a multiple of 16
+    hipLaunchKernelGGL((k_merge_list<E, float>), dim3((n / 16) * n), dim3(1024), sizeof(E) * (size_t)n * 16,
+                       h->stream, sT, static_cast<const int*>(h->ws_mcount.ptr),
+                       reinterpret_cast<const int*>(base), reinterpret_cast<const float*>(base + idx_bytes),
+                       reinterpret_cast<const float*>(base + idx_bytes + w_bytes), n, static_cast<E*>(vol));
+    check_launch("k_merge_list");
+    if (h->mplan_heavy > 0) {
+        hipLaunchKernelGGL((k_merge_heavy<E, float>), dim3(h->mplan_heavy), dim3(64 * kHeavyWaves),
+                           sizeof(E) * kHeavyWaves * (size_t)n,
+                           h->stream, sT, static_cast<const int*>(h->ws_mcount.ptr),
+                           reinterpret_cast<const int*>(base), reinterpret_cast<const float*>(base + idx_bytes),
+                           reinterpret_cast<const float*>(base + idx_bytes + w_bytes),
+                           static_cast<const int*>(h->ws_mheavy.ptr), n, static_cast<E*>(vol));
+        check_launch("k_merge_heavy");
+    }
+}
+
 // complex64 merge / extraction on the handle's stream (the carousel driver)
 void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P)
 {
@@ -268,7 +674,9 @@ void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angl
 void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* angles, int n, int P)
 {
     check_merge_geometry(n, n, P, n, "carousel extraction");
-    extract_launch<c2<float>, float>(h, vol, slices, angles, n, P);
+    // the gather list's trig table of the same angles, when it is there
+    const bool reuse = h->mplan_n == n && h->mplan_P == P && h->ws_trig.ptr;
+    extract_launch<c2<float>, float>(h, vol, slices, angles, n, P, reuse);
 }
 
 template <typename F> static int mguard(cpra_hip_ctx* h, F&& f)
