@@ -353,7 +353,90 @@ def bench_gd3d(hip, synth, ge, n=256, steps=500, warmup=10, timing_iters=20, fp6
     return rec
 
 
-WORKLOADS = ("hio2d", "carousel3d", "gd2d", "gd3d")
+def bench_mixed(hip, synth, ge, sizes=(128, 192, 256, 384), per_class=256, iters=300, warmup=5,
+                timing_iters=5, pattern_iters=3, fp64=True):
+    """Mixed-size batch: 256 patterns each at 128^2, 192^2, 256^2 and 384^2
+    (built as the single-pattern workload, autocorrelation supports), Elser's
+    difference map with beta = 0.8 and positivity for 300 iterations, one
+    column and one row launch per iteration for the whole batch (graph
+    replays).  Also timed: each size class alone, one launch pair per class,
+    and one launch triple per pattern (the naive form, eager, a few
+    iterations)."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    data = {}
+    for n in sizes:
+        d = synth.make_2d_batch(per_class, n, 30.0 * n / 256, 50.0 * n / 256, 1e6,
+                                starts_per_pattern=1, config=3)
+        data[n] = d
+    log(f"bench[mixed]: data ready in {time.perf_counter() - t0:.1f} s")
+    dev = {n: (hip.to_device(data[n]["x0"]), hip.to_device(data[n]["amp"]),
+               hip.to_device(data[n]["support"])) for n in sizes}
+    classes = [dev[n] for n in sizes]
+    h = hip.HipImpl(64, 64, 1, 1)
+    kw = dict(beta=0.8, space_mode=1, textbook=1)
+
+    def timed(cls, mode, k, graph=1):
+        h.Sync()
+        t = time.perf_counter()
+        h.dm_mixed_run(cls, launch_mode=mode, iters=k, use_graph=graph, **kw)
+        h.Sync()
+        return time.perf_counter() - t
+
+    timed(classes, hip.MIXED_FUSED, warmup)
+    el = timed(classes, hip.MIXED_FUSED, iters)
+    its = iters / el
+    log(f"bench[mixed]: fused {iters} iterations in {el:.3f} s -> {its:.1f} it/s")
+    el_cls = timed(classes, hip.MIXED_PER_CLASS, iters)
+    el_pat = timed(classes, hip.MIXED_PER_PATTERN, pattern_iters, graph=0)
+    per_size = {}
+    for n in sizes:
+        e = timed([dev[n]], hip.MIXED_FUSED, iters)
+        per_size[f"{n}x{n}"] = {"iters_per_s": round(iters / e, 1),
+                                "pattern_iters_per_s": round(per_class * iters / e, 1)}
+    h.set_kernel_timing(True)
+    h.dm_mixed_run(classes, launch_mode=hip.MIXED_FUSED, iters=timing_iters, use_graph=0, **kw)
+    h.Sync()
+    kt = h.kernel_times()
+    h.set_kernel_timing(False)
+    px = per_class * sum(n * n for n in sizes)
+    kern = {"k_mixed_cols": kernel_entry(kt["column"][0], kt["column"][1], timing_iters, px * 40),
+            "k_mixed_dm_rows": kernel_entry(kt["row"][0], kt["row"][1], timing_iters, px * 49)}
+    dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
+    rec = {
+        "metric": "mixed-size DM iters/s (256 each of 128^2, 192^2, 256^2, 384^2)",
+        "value": round(its, 2), "unit": "iters/s", "steps": iters, "warmup": warmup,
+        "ms_per_step": round(1e3 * el / iters, 4),
+        "config": {"workload": f"textbook difference map beta 0.8 with positivity, {per_class} patterns "
+                               f"each of {', '.join(f'{n}^2' for n in sizes)} complex64, one column and "
+                               "one row launch per iteration (graph replays)", "patterns": per_class * len(sizes)},
+        "pattern_iters_per_s": round(per_class * len(sizes) * its, 1),
+        "per_size_class": per_size,
+        "one_launch_per_class_iters_per_s": round(iters / el_cls, 2),
+        "one_launch_per_pattern_iters_per_s": round(pattern_iters / el_pat, 2),
+        "speedup_vs_launch_per_pattern": round((iters / el) / (pattern_iters / el_pat), 1),
+        "hbm_frac_algorithmic_21Bpx": round(px * 21 * its / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern[dom]["frac"], "kernels": kern},
+    }
+    if fp64:
+        o = ge.load_oracle()
+        errs = {}
+        with hip.HipImpl(64, 64, 1, 1) as hs:
+            for n in sizes:
+                x0 = data[n]["x0"][:2]
+                xs = (hip.to_device(x0), hip.to_device(data[n]["amp"][:2]), hip.to_device(data[n]["support"][:2]))
+                hs.dm_mixed_run([xs], iters=1, use_graph=0, **kw)
+                hs.Sync()
+                ref = o.dm_textbook(x0, data[n]["amp"][:2], 0.8, 1, mask=data[n]["support"][:2], space_mode=1)
+                errs[f"{n}x{n}"] = float(np.linalg.norm(xs[0].numpy() - ref) / np.linalg.norm(ref))
+        rec["fp64_check"] = {"patterns_per_class": 2, "iters": 1, "rel_l2": errs}
+    h.close()
+    return rec
+
+
+WORKLOADS = ("hio2d", "carousel3d", "gd2d", "gd3d", "mixed")
 
 
 def main():
@@ -394,7 +477,8 @@ def main():
     assert hip.device_count() > 0, "bench.py needs a GPU"
     fp64 = not args.no_fp64_check
     if args.workload != "hio2d":
-        fn = {"carousel3d": bench_carousel, "gd2d": bench_gd2d, "gd3d": bench_gd3d}[args.workload]
+        fn = {"carousel3d": bench_carousel, "gd2d": bench_gd2d, "gd3d": bench_gd3d,
+              "mixed": bench_mixed}[args.workload]
         rec = fn(hip, synth, ge, fp64=fp64)
         rec.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                     "dtype": "c64", "data": "synthetic"})
@@ -539,7 +623,8 @@ def main():
 
     if not args.no_extra:
         result["workloads"] = {}
-        for name, fn in (("carousel3d", bench_carousel), ("gd2d", bench_gd2d), ("gd3d", bench_gd3d)):
+        for name, fn in (("carousel3d", bench_carousel), ("gd2d", bench_gd2d), ("gd3d", bench_gd3d),
+                         ("mixed", bench_mixed)):
             log(f"bench: workload {name}")
             result["workloads"][name] = fn(hip, synth, ge, fp64=fp64)
 

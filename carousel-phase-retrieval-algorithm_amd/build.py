@@ -32,7 +32,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["abi.hip", "fft.hip", "engine2d.hip", "merge3d.hip", "amploss.hip", "carousel.hip"]
+SOURCES = ["abi.hip", "fft.hip", "engine2d.hip", "merge3d.hip", "amploss.hip", "carousel.hip", "mixed.hip"]
 
 
 def hipcc() -> str:

@@ -92,8 +92,8 @@ struct cpra_hip_ctx {
     cpra::EngineGraph* graph = nullptr;
     // per-kernel event timing of the engines (eager launches only)
     bool timing = false;
-    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double kt_ms[16] = {};
+    int64_t kt_n[16] = {};
 };
 
 namespace cpra {
@@ -174,7 +174,8 @@ struct KernelTimer {
 
 // kinds of cpra_hip_get_kernel_times
 enum KernelKind {
-    KT_COL = 0, KT_ROW = 1, KT_SW = 2, KT_GD = 3, KT_MERGE = 4, KT_VOL = 5, KT_EXTRACT = 6, KT_PFFT = 7
+    KT_COL = 0, KT_ROW = 1, KT_SW = 2, KT_GD = 3, KT_MERGE = 4, KT_VOL = 5, KT_EXTRACT = 6, KT_PFFT = 7,
+    KT_MIXED = 8
 };
 
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)

@@ -31,6 +31,70 @@
 
 namespace cpra {
 
+// non-template kernels of the engine (one translation unit only)
+// Row bits of a support mask [rows][N] (N % 16 == 0): bit r of the
+// concatenated per-pattern words is set when row r holds a support pixel.
+// 16 threads per row, each testing 16-byte chunks.
+__global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict__ mask,
+                                                      int64_t nrows, int N,
+                                                      unsigned* __restrict__ rowbits)
+{
+    const int lane16 = threadIdx.x & 15;
+    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    unsigned any = 0;
+    if (row < nrows) {
+        const uint4* p = reinterpret_cast<const uint4*>(mask + row * N);
+        for (int i = lane16; i < N / 16; i += 16) {
+            const uint4 q = p[i];
+            any |= q.x | q.y | q.z | q.w;
+        }
+    }
+    for (int off = 8; off > 0; off >>= 1) any |= __shfl_xor(any, off, 16);
+    if (lane16 == 0 && row < nrows && any) atomicOr(&rowbits[row >> 5], 1u << (row & 31));
+}
+
+__global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// mask = b >= frac * max_pattern(b)
+__global__ void k_sw_thresh(const float* __restrict__ bl, const unsigned* __restrict__ keys,
+                            uint8_t* __restrict__ mask, int64_t per, int64_t total, float frac)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float thr = frac * key2f(keys[i / per]);
+        mask[i] = bl[i] >= thr ? 1 : 0;
+    }
+}
+
+// R-factor: sum the per-block partials of each pattern (fixed order, double).
+__global__ void k_rf_fin(const float* __restrict__ part, int blocks_per_pattern, int64_t B,
+                         float* __restrict__ rf)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < blocks_per_pattern; ++k) {
+        const int64_t tile = b * blocks_per_pattern + k;
+        num += part[2 * tile];
+        den += part[2 * tile + 1];
+    }
+    rf[b] = den > 0.0 ? (float)(num / den) : 0.f;
+}
+
+__global__ void k_u8_from_S(const float* __restrict__ S, uint8_t* __restrict__ mask, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        mask[i] = (double)S[i] < 0.9 ? 0 : 1;
+}
+
+
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -711,7 +775,7 @@ int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable)
     return eguard(h, [&]() -> int {
         CPRA_ARG(h, "cpra_hip_set_kernel_timing: NULL handle");
         h->timing = enable != 0;
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < 16; ++i) {
             h->kt_ms[i] = 0.0;
             h->kt_n[i] = 0;
         }
@@ -722,7 +786,7 @@ int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable)
 int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n)
 {
     return eguard(h, [&]() -> int {
-        CPRA_ARG(h && ms_total && counts && n >= 1 && n <= 8, "cpra_hip_get_kernel_times: bad argument");
+        CPRA_ARG(h && ms_total && counts && n >= 1 && n <= 16, "cpra_hip_get_kernel_times: bad argument");
         for (int i = 0; i < n; ++i) {
             ms_total[i] = h->kt_ms[i];
             counts[i] = h->kt_n[i];
@@ -768,7 +832,7 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         for (int v = 0; v < 4; ++v) {
             h->tuning[key] = v;
             h->timing = true;
-            for (int i = 0; i < 8; ++i) { h->kt_ms[i] = 0; h->kt_n[i] = 0; }
+            for (int i = 0; i < 16; ++i) { h->kt_ms[i] = 0; h->kt_n[i] = 0; }
             // one untimed warm-up iteration per candidate
             cpra_hip_hio_params w = p;
             w.iters = 1;

@@ -72,23 +72,24 @@ template <int M, int MAP> struct ColGeom {
 // (Y rewritten in place from a re-read of the incoming Y; error reduction
 // writes 0) and only rows with support pixels are written to Z for the row
 // kernel (rowflag[b*M + row] != 0).
-template <int M, int MODE, int MAP, bool SPARSE = false>
-__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
-    const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
-    float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
-    int xcd_remap, const unsigned* __restrict__ rowbits = nullptr, float beta_n = 0.f,
-    int error_reduction = 0, int pattern0 = 0)
+// The kernel body works on one tile (C columns of one pattern) given by the
+// caller, with the caller's LDS (M*C elements; `red` 2*TPB floats for
+// COL_MOD_RF), so both the single-size kernel below and the mixed-size
+// kernels (one launch over several pattern sizes) run it.
+template <int M, int MODE, int MAP, bool SPARSE>
+__device__ __forceinline__ void eng_cols_tile(
+    const unsigned tile, c2<float>* lds, float* red, const c2<float>* Y, c2<float>* Z,
+    const float* __restrict__ A, int64_t amp_batch, int N, float scale, int skip_zero,
+    float* __restrict__ rf_part, float gcoef_r, float gcoef_c, const unsigned* __restrict__ rowbits,
+    float beta_n, int error_reduction, int pattern0)
 {
     using V = c2<float>;
     using G = ColGeom<M, MAP>;
     using PL = typename G::PL;
     constexpr int E = PL::E, TT = PL::T, C = G::C;
-    __shared__ V lds[M * C];
-    __shared__ float red[2][SizeCfg<M>::TPB];
     const int tid = threadIdx.x;
     const int c = G::col(tid), t = G::t(tid);
     // block-uniform pattern and first column; per-lane 32-bit offsets
-    const unsigned tile = tile_index(xcd_remap);
     const unsigned tiles_per_pattern = (unsigned)N / C;
     const unsigned b = tile / tiles_per_pattern;
     const int col0 = (int)(tile - b * tiles_per_pattern) * C;
@@ -195,20 +196,41 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
         });
     }
     if constexpr (MODE == COL_MOD_RF) {
-        red[0][tid] = rnum;
-        red[1][tid] = rden;
+        constexpr int TPB = SizeCfg<M>::TPB;
+        red[tid] = rnum;
+        red[TPB + tid] = rden;
         __syncthreads();
-        for (int s = SizeCfg<M>::TPB / 2; s > 0; s >>= 1) {
+        for (int s = TPB / 2; s > 0; s >>= 1) {
             if (tid < s) {
-                red[0][tid] += red[0][tid + s];
-                red[1][tid] += red[1][tid + s];
+                red[tid] += red[tid + s];
+                red[TPB + tid] += red[TPB + tid + s];
             }
             __syncthreads();
         }
         if (tid == 0) {
-            rf_part[2 * tile] = red[0][0];
-            rf_part[2 * tile + 1] = red[1][0];
+            rf_part[2 * tile] = red[0];
+            rf_part[2 * tile + 1] = red[TPB];
         }
+    }
+}
+
+template <int M, int MODE, int MAP, bool SPARSE = false>
+__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
+    const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
+    float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
+    int xcd_remap, const unsigned* __restrict__ rowbits = nullptr, float beta_n = 0.f,
+    int error_reduction = 0, int pattern0 = 0)
+{
+    __shared__ c2<float> lds[M * SizeCfg<M>::C];
+    if constexpr (MODE == COL_MOD_RF) {
+        __shared__ float red[2 * SizeCfg<M>::TPB];
+        eng_cols_tile<M, MODE, MAP, SPARSE>(tile_index(xcd_remap), lds, red, Y, Z, A, amp_batch, N, scale,
+                                            skip_zero, rf_part, gcoef_r, gcoef_c, rowbits, beta_n,
+                                            error_reduction, pattern0);
+    } else {
+        eng_cols_tile<M, MODE, MAP, SPARSE>(tile_index(xcd_remap), lds, nullptr, Y, Z, A, amp_batch, N,
+                                            scale, skip_zero, rf_part, gcoef_r, gcoef_c, rowbits, beta_n,
+                                            error_reduction, pattern0);
     }
 }
 
@@ -281,27 +303,6 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
             Yb[lo + PL::rot(m) * TT] = xo[m];
         });
     }
-}
-
-// Row bits of a support mask [rows][N] (N % 16 == 0): bit r of the
-// concatenated per-pattern words is set when row r holds a support pixel.
-// 16 threads per row, each testing 16-byte chunks.
-__global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict__ mask,
-                                                      int64_t nrows, int N,
-                                                      unsigned* __restrict__ rowbits)
-{
-    const int lane16 = threadIdx.x & 15;
-    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-    unsigned any = 0;
-    if (row < nrows) {
-        const uint4* p = reinterpret_cast<const uint4*>(mask + row * N);
-        for (int i = lane16; i < N / 16; i += 16) {
-            const uint4 q = p[i];
-            any |= q.x | q.y | q.z | q.w;
-        }
-    }
-    for (int off = 8; off > 0; off >>= 1) any |= __shfl_xor(any, off, 16);
-    if (lane16 == 0 && row < nrows && any) atomicOr(&rowbits[row >> 5], 1u << (row & 31));
 }
 
 // Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
@@ -398,46 +399,6 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_blurmax(const c2<float>*
     }
 }
 
-__global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
-// mask = b >= frac * max_pattern(b)
-__global__ void k_sw_thresh(const float* __restrict__ bl, const unsigned* __restrict__ keys,
-                            uint8_t* __restrict__ mask, int64_t per, int64_t total, float frac)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const float thr = frac * key2f(keys[i / per]);
-        mask[i] = bl[i] >= thr ? 1 : 0;
-    }
-}
-
-// R-factor: sum the per-block partials of each pattern (fixed order, double).
-__global__ void k_rf_fin(const float* __restrict__ part, int blocks_per_pattern, int64_t B,
-                         float* __restrict__ rf)
-{
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    double num = 0.0, den = 0.0;
-    for (int k = 0; k < blocks_per_pattern; ++k) {
-        const int64_t tile = b * blocks_per_pattern + k;
-        num += part[2 * tile];
-        den += part[2 * tile + 1];
-    }
-    rf[b] = den > 0.0 ? (float)(num / den) : 0.f;
-}
-
-__global__ void k_u8_from_S(const float* __restrict__ S, uint8_t* __restrict__ mask, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        mask[i] = (double)S[i] < 0.9 ? 0 : 1;
-}
-
 }  // namespace cpra
 
 namespace cpra {
@@ -492,7 +453,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_entry(
         const V s2 = TEXTBOOK ? p : (valid ? t2[o] : mk<V>(0, 0));
         const V q = ps_apply<SPACE>(s2, valid && mask[o] != 0);
         a[m] = s1;
-        u[m] = mk<V>(p.x * ib + cb * q.x, p.y * ib + cb * q.y);
+        u[m] = mk<V>(fmaf(cb, q.x, p.x * ib), fmaf(cb, q.y, p.y * ib));
     });
     fft_line<PL, -1>(a, t, tw, acc);
     fft_line<PL, -1>(u, t, tw, acc);
@@ -513,9 +474,12 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_entry(
 //   psi' = (psi + beta t1') - beta t2'                          :109-110
 // then Y1 = DFT_r(t1' | psi'), Y2 = DFT_r(psi'/beta + (1-1/beta) P_S(t2' | psi'))
 // for the next iteration; LAST writes psi', t1', t2' in pattern space instead.
+// The body handles the LPB lines of block `vb` with the caller's LDS
+// (LPB * LdsRow<V, N>::span() elements); k_dm_rows and the mixed-size kernel
+// run it.
 template <int N, bool TEXTBOOK, int SPACE, bool LAST>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
-    const c2<float>* __restrict__ Z1, const c2<float>* __restrict__ Z2,
+__device__ __forceinline__ void dm_rows_block(
+    const unsigned vb, c2<float>* lds, const c2<float>* __restrict__ Z1, const c2<float>* __restrict__ Z2,
     c2<float>* __restrict__ psi, const uint8_t* __restrict__ mask, c2<float>* __restrict__ Y1,
     c2<float>* __restrict__ Y2, c2<float>* __restrict__ t1out, c2<float>* __restrict__ t2out,
     int64_t nlines, float beta)
@@ -524,10 +488,9 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
     using PL = typename SizeCfg<N>::P;
     constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
     constexpr int SPAN = LdsRow<V, N>::span();
-    __shared__ V lds[LPB * SPAN];
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
+    const int64_t line = (int64_t)vb * LPB + ll;
     const bool valid = line < nlines;
     const size_t base = (size_t)(valid ? line : 0) * N + t;
     LdsRow<V, N, true> acc{lds + ll * SPAN};
@@ -548,11 +511,14 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
         const size_t o = base + PL::rot(m) * TT;
         const bool in = valid && mask[o] != 0;
         const V p = valid ? psi[o] : mk<V>(0, 0);
-        const V w = mk<V>(-ib * p.x + pb * a[m].x, -ib * p.y + pb * a[m].y);
+        // explicit fmaf: the same operations whatever the kernel inlining this
+        // body leaves the compiler free to contract (the launch modes of the
+        // mixed-size engine agree bit for bit)
+        const V w = mk<V>(fmaf(pb, a[m].x, -ib * p.x), fmaf(pb, a[m].y, -ib * p.y));
         const V t1n = ps_apply<SPACE>(w, in);
         const V t2n = b[m];
-        V pn = mk<V>(p.x + beta * t1n.x, p.y + beta * t1n.y);
-        pn = mk<V>(pn.x - beta * t2n.x, pn.y - beta * t2n.y);
+        V pn = mk<V>(fmaf(beta, t1n.x, p.x), fmaf(beta, t1n.y, p.y));
+        pn = mk<V>(fmaf(-beta, t2n.x, pn.x), fmaf(-beta, t2n.y, pn.y));
         if (valid) psi[o] = pn;
         if constexpr (LAST) {
             a[m] = t1n;
@@ -561,7 +527,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
             const V s1 = TEXTBOOK ? pn : t1n;
             const V q = ps_apply<SPACE>(TEXTBOOK ? pn : t2n, in);
             a[m] = s1;
-            b[m] = mk<V>(pn.x * ib + cb * q.x, pn.y * ib + cb * q.y);
+            b[m] = mk<V>(fmaf(cb, q.x, pn.x * ib), fmaf(cb, q.y, pn.y * ib));
         }
     });
     if constexpr (LAST) {
@@ -585,6 +551,18 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
             });
         }
     }
+}
+
+template <int N, bool TEXTBOOK, int SPACE, bool LAST>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
+    const c2<float>* __restrict__ Z1, const c2<float>* __restrict__ Z2,
+    c2<float>* __restrict__ psi, const uint8_t* __restrict__ mask, c2<float>* __restrict__ Y1,
+    c2<float>* __restrict__ Y2, c2<float>* __restrict__ t1out, c2<float>* __restrict__ t2out,
+    int64_t nlines, float beta)
+{
+    __shared__ c2<float> lds[SizeCfg<N>::LPB * LdsRow<c2<float>, N>::span()];
+    dm_rows_block<N, TEXTBOOK, SPACE, LAST>(blockIdx.x, lds, Z1, Z2, psi, mask, Y1, Y2, t1out, t2out,
+                                            nlines, beta);
 }
 
 }  // namespace cpra

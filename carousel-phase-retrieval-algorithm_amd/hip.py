@@ -57,6 +57,19 @@ class DmParams(C.Structure):
     ]
 
 
+class SizeClass(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("count", C.c_uint32),
+        ("psi", C.c_void_p),
+        ("amp", C.c_void_p),
+        ("support", C.c_void_p),
+    ]
+
+
+MIXED_FUSED, MIXED_PER_CLASS, MIXED_PER_PATTERN = 0, 1, 2
+
+
 class CarouselParams(C.Structure):
     _fields_ = [
         ("outer_iters", C.c_int32),
@@ -136,6 +149,7 @@ _SIGS = {
     "cpra_hip_amp_gd_run": (C.c_int, [_H, _P, _P, _U64, C.c_int, C.c_float, _P]),
     "cpra_hip_amp_gd_run_3d": (C.c_int, [_H, _P, _P, _U64, C.c_int, C.c_float, _P]),
     "cpra_hip_carousel_default_params": (None, [C.POINTER(CarouselParams)]),
+    "cpra_hip_dm_mixed_run": (C.c_int, [_H, C.POINTER(SizeClass), C.c_int, C.POINTER(DmParams), C.c_int]),
     "cpra_hip_carousel3d_run": (C.c_int, [_H, _P, _P, _U64, _P, _P, _P, _P,
                                           C.POINTER(CarouselParams), _P]),
 }
@@ -428,6 +442,23 @@ class HipImpl:
             _addr(vol), C.byref(p), rf.ctypes.data if want_rfactor else None))
         return rf
 
+    def dm_mixed_run(self, classes, launch_mode: int = MIXED_FUSED, **kw) -> bool:
+        """Textbook difference map over size classes (cpra_hip_dm_mixed_run).
+        classes: sequence of (psi, amp, support) device arrays [count][n][n]."""
+        arr = (SizeClass * len(classes))()
+        for i, (psi, amp, sup) in enumerate(classes):
+            n = psi.shape[-1]
+            arr[i] = SizeClass(n, psi.numel() // (n * n), _addr(psi), _addr(amp), _addr(sup))
+        p = DmParams()
+        lib().cpra_hip_dm_default_params(C.byref(p))
+        p.textbook = 1
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        return self._check(lib().cpra_hip_dm_mixed_run(self._h, arr, len(classes), C.byref(p),
+                                                       launch_mode))
+
     def shrinkwrap(self, x, support, sigma: float, frac: float, blur_out=None) -> bool:
         return self._check(lib().cpra_hip_shrinkwrap(self._h, _addr(x), _addr(support), sigma,
                                                      frac, _addr(blur_out)))
@@ -461,7 +492,8 @@ class HipImpl:
 
     def kernel_times(self):
         """{kind: (total_ms, launches)} per fused-kernel kind."""
-        names = ("column", "row", "shrinkwrap", "gd", "merge", "volume", "extract", "proj_fft")
+        names = ("column", "row", "shrinkwrap", "gd", "merge", "volume", "extract", "proj_fft",
+                 "mixed_iteration")
         ms = (C.c_double * len(names))()
         cnt = (C.c_int64 * len(names))()
         self._check(lib().cpra_hip_get_kernel_times(self._h, ms, cnt, len(names)))

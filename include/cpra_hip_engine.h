@@ -111,8 +111,9 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
  * (0 = column pass, 1 = row pass, 2 = one shrinkwrap update, 3 = fused
  * gradient-descent pass, 4 = 2D->3D merge, 5 = 3D volume constraint (its
  * five FFT passes), 6 = slice extraction, 7 = column pass of the carousel's
- * projections), the summed milliseconds and the launch count since the last
- * enable. */
+ * projections, 8 = one whole mixed-batch iteration of the per-class and
+ * per-pattern modes; the fused mode records 0 and 1), the summed milliseconds
+ * and the launch count since the last enable (n <= 16 kinds). */
 int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable);
 int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n);
 
@@ -144,6 +145,34 @@ int cpra_hip_amp_gd_run(cpra_hip_handle h, void* x, const float* amp, uint64_t a
                         int iters, float mu, double* loss_host);
 int cpra_hip_amp_gd_run_3d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                            int iters, float mu, double* loss_host);
+
+/* ---- mixed-size batch (north-star addition) ---------------------------------
+ * Difference map (textbook map, p->textbook must be 1; beta, space_mode,
+ * dc_skip_zero, iters, use_graph as in cpra_hip_dm_run) over size classes of
+ * square patterns.  Class k: count patterns of n x n (n in
+ * {64,128,192,256,384,512}), psi [count][n][n] complex64 in/out, amp and the
+ * u8 support [count][n][n] (one amplitude per pattern).  launch_mode:
+ *   CPRA_HIP_MIXED_FUSED        one column and one row launch per iteration
+ *                               for the whole batch (blocks partitioned by
+ *                               class, LDS sized for the largest class);
+ *   CPRA_HIP_MIXED_PER_CLASS    one launch pair per class per iteration;
+ *   CPRA_HIP_MIXED_PER_PATTERN  one launch triple per pattern per iteration.
+ * The modes give identical results.  h: any F32 handle (its stream and
+ * workspaces are used). */
+#define CPRA_HIP_MAX_CLASSES 8
+#define CPRA_HIP_MIXED_FUSED 0
+#define CPRA_HIP_MIXED_PER_CLASS 1
+#define CPRA_HIP_MIXED_PER_PATTERN 2
+typedef struct {
+    uint32_t n;              /* pattern side                        */
+    uint32_t count;          /* patterns of this size               */
+    void* psi;               /* [count][n][n] complex64, in/out     */
+    const float* amp;        /* [count][n][n]                       */
+    const uint8_t* support;  /* [count][n][n]                       */
+} cpra_hip_size_class;
+
+int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes, int nclasses,
+                          const cpra_hip_dm_params* p, int launch_mode);
 
 /* ---- 3D carousel (north-star addition) --------------------------------------
  * Projections of one real object at angles th_p about the z axis, their
