@@ -75,6 +75,27 @@ def build_cpp_checks() -> list:
     return out
 
 
+DRIVERS = {"cpra_carousel": os.path.join(HERE, "drivers", "cpra_carousel.cpp")}
+DRIVER_BIN = os.path.join(HERE, "bin")
+
+
+def build_drivers() -> list:
+    """The reference's command-line driver restated over HipImpl<T>
+    (drivers/cpra_carousel.cpp), host C++ linked against libcpra_hip.so."""
+    os.makedirs(DRIVER_BIN, exist_ok=True)
+    out = []
+    gxx = shutil.which("g++") or "g++"
+    for name, src in DRIVERS.items():
+        exe = os.path.join(DRIVER_BIN, name)
+        cmd = [gxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+               "-o", exe, "-L", HERE, "-lcpra_hip", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-4000:]}")
+        out.append(exe)
+    return out
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     jobs = min(len(SOURCES), max(1, (os.cpu_count() or 2)))
@@ -88,6 +109,7 @@ def build(verbose: bool = True) -> str:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
     build_cpp_checks()
+    build_drivers()
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -397,3 +397,47 @@ def carousel3d(x, A, support2d, angles, support3d, outer, inner=1, beta=0.9, dc_
         v = np.where(s3 & (w.real > 0), w.real, 0.0).astype(np.complex128)
         x = backward2d(slice_extract(forward3d(v), angles, True))
     return x, v
+
+
+def carousel_driver(A, S, init, epi, iters, preit, beta):
+    """FP64 restatement of one batch of the reference's carousel
+    reconstruction (src/CPRA_MKL.cpp:10-154).  A, S: [P][M][N] data and space
+    constraints; init: psi [P][n][M][N].  Step A (:35-55): `preit` DM
+    iterations (orc_dm_ref) on projection 0 with t1 = t2 = psi_0; Step B
+    (:58-124): `epi` sequential sweeps, each projection warm-started from its
+    neighbours (:63-76), t1 = t2 = psi_p, `iters` DM iterations, and in the
+    last sweep the convergence error ||F psi_before - F psi_after|| / ||F
+    psi_after|| of the last iteration (:85-91, 112-121); final projection
+    (:126-140).  Returns (psi [P][n][M][N], err [P][n])."""
+    psi = _c128(init).copy()
+    P, n = psi.shape[:2]
+    M, N = psi.shape[-2:]
+    err = np.zeros((P, n))
+
+    def dm(p, k, t1, t2):
+        if k <= 0:
+            return t1, t2
+        psi[p], t1, t2 = dm_ref(psi[p], t1, t2, A[p:p + 1], S[p:p + 1], beta, k)
+        return t1, t2
+
+    dm(0, preit, psi[0].copy(), psi[0].copy())
+    for e in range(epi):
+        for p in range(P):
+            if p == P - 1:
+                psi[p] = 0.5 * psi[p - 1] + 0.5 * psi[p]
+            elif p == 0:
+                psi[p] = 0.5 * psi[p + 1] + 0.5 * psi[p]
+            else:
+                psi[p] = 0.5 * psi[p - 1]
+                psi[p] = 0.5 * psi[p + 1] + psi[p]
+            t1, t2 = psi[p].copy(), psi[p].copy()
+            if e == epi - 1 and iters >= 1:
+                t1, t2 = dm(p, iters - 1, t1, t2)
+                old = psi[p].copy()
+                dm(p, 1, t1, t2)
+                err[p], _ = converge_error(forward2d(old), forward2d(psi[p]), M, N, 1, n)
+            else:
+                dm(p, iters, t1, t2)
+    for p in range(P):
+        psi[p] = final_projection(psi[p], A[p:p + 1], S[p:p + 1], beta)
+    return psi, err

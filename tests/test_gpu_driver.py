@@ -1,0 +1,62 @@
+"""The reference's carousel driver restated over HipImpl (bin/cpra_carousel)
+against the FP64 oracle's restatement of src/CPRA_MKL.cpp:10-154
+(oracle.carousel_driver: orc_dm_ref, the warm-start merges, orc_converge_error,
+orc_final_projection), from the same initial guess (--init), through the
+reference's file formats (raw float32 in, <out>.<k> per batch member out)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_carousel")
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("M", [64, 80])   # fused engine; primitive composition
+def test_driver_matches_oracle(oracle, gpu, tmp_path, M):
+    N = M
+    P, batch, total = 4, 2, 3
+    epi, iters, preit, beta = 2, 3, 6, 0.9
+    rng = np.random.default_rng(11)
+    yy, xx = np.mgrid[:M, :N]
+    S = np.zeros((P, M, N), np.float32)
+    A = np.zeros((P, M, N), np.float32)
+    for p in range(P):
+        r = M // 7 + p
+        sup = ((yy - M / 2) ** 2 + (xx - N / 2) ** 2 <= r * r)
+        S[p] = sup
+        obj = sup * rng.uniform(0.2, 1.0, (M, N))
+        A[p] = np.abs(np.fft.fftshift(np.fft.fft2(np.fft.fftshift(obj))))
+    init = (rng.random((P, batch, M, N)) + 1j * rng.random((P, batch, M, N))).astype(np.complex64)
+    A.tofile(tmp_path / "data.bin")
+    S.tofile(tmp_path / "space.bin")
+    init.tofile(tmp_path / "init.bin")
+    r = subprocess.run([EXE, "-m", str(M), "-n", str(N), "-l", str(M), "-p", str(P), "-e", str(epi),
+                        "-i", str(iters), "--preit", str(preit), "--Batch", str(batch), "--Total",
+                        str(total), "--Beta", str(beta), "-d", str(tmp_path / "data.bin"),
+                        "-s", str(tmp_path / "space.bin"), "-r", str(tmp_path / "rec.bin"),
+                        "--output_error", str(tmp_path / "err.bin"), "--init", str(tmp_path / "init.bin")],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert ("primitive composition" in r.stdout) == (M == 80)
+    # batches of 2 and 1; the second uses member 0 of each projection block
+    batches = [(0, 2), (2, 1)]
+    k = 1
+    for start, n in batches:
+        psi, err = oracle.carousel_driver(A.astype(np.float64), S.astype(np.float64),
+                                          init[:, :n].astype(np.complex128), epi, iters, preit, beta)
+        for i in range(n):
+            rec = np.fromfile(tmp_path / f"rec.bin.{k}", dtype=np.float32)
+            e = np.fromfile(tmp_path / f"err.bin.{k}", dtype=np.float32)
+            assert rec.size == P * M * N and e.size == P
+            assert relerr(rec.reshape(P, M, N), psi[:, i].real) < 1e-4, k
+            np.testing.assert_allclose(e, err[:, i], rtol=2e-3, atol=1e-6)
+            k += 1
+    assert not (tmp_path / f"rec.bin.{k}").exists()
