@@ -46,16 +46,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, tuning: dict):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known traffic)."""
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known traffic), taken
+    with the same column variant and patterns per launch as `tuning`."""
     path = os.path.join(ROOT, "profiles", "hio_512x256sq_pmc.md")
     if not os.path.exists(path):
         return None
+    want = ";".join(f"{k}={tuning.get(k)}" for k in ("col_256", "chunk"))
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-            return int(parts[2])
+        if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel
+                and parts[2] == want):
+            return int(parts[3])
     return None
 
 
@@ -456,6 +459,9 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--timing-iters", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="apply this autotune spec (e.g. 'col_256=4;rows_sparse=0;chunk=256;') "
+                         "instead of autotuning, so that profiler runs repeat one choice")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one MergeAddData (dst = 1*src + 0*dst) over 2^26 complex64 "
                          "first: a kernel of known traffic (16 B read, 8 B written per "
@@ -508,7 +514,11 @@ def main():
         log("bench: copy calibration kernel done (2^26 complex64)")
 
     # autotune the kernel variants on this workload (restores x, support)
-    tune = "" if args.no_autotune else h.autotune(x, amp, sup, iters=10)
+    if args.tuning:
+        h.set_tuning(args.tuning)
+        tune = f"fixed: {args.tuning}"
+    else:
+        tune = "" if args.no_autotune else h.autotune(x, amp, sup, iters=10)
     log(f"bench: autotune {tune}")
 
     # warmup (also captures the 20-iteration graph)
@@ -550,11 +560,13 @@ def main():
             "kernels": kern,
             "shrinkwrap_update_ms": sw_ms / max(sw_n, 1),
             "tuning": tuning}
-    # counters were taken on whole-batch launches of the 512 x 256^2 workload
-    if P == 512 and n == 256 and int(tuning.get("chunk", "512")) in (0, P):
-        tb = measured_traffic(roof["kernel"])
-        roof["traffic"] = tb
-        roof["traffic_source"] = "profiles/hio_512x256sq_pmc.md (rocprofv3 FETCH_SIZE+WRITE_SIZE)"
+    # counters are per launch of the 512 x 256^2 workload at a given tuning
+    if P == 512 and n == 256:
+        tb = measured_traffic(roof["kernel"], tuning)
+        if tb is not None:
+            roof["traffic"] = tb
+            roof["traffic_source"] = ("profiles/hio_512x256sq_pmc.md (rocprofv3 FETCH_SIZE + "
+                                      "WRITE_SIZE, same column variant and chunk)")
     iter_algo_frac = px * HIO_ALGO_BYTES_PER_PX * its / 1e9 / HBM_PEAK_GBS
     h.close()
 

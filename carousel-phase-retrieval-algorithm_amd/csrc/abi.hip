@@ -395,6 +395,8 @@ int cpra_hip_destroy(cpra_hip_handle h)
     cpra_engine_release(h);
     for (auto& kv : h->twiddles) (void)hipFree(kv.second);
     h->ws_z.release();
+    h->ws_state.release();
+    h->ws_amp_t.release();
     h->ws_blur.release();
     h->ws_keys.release();
     h->ws_part.release();

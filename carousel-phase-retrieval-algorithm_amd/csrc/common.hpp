@@ -84,6 +84,8 @@ struct cpra_hip_ctx {
     std::map<uint64_t, void*> twiddles;
     // engine workspaces
     cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm, ws_trig, ws_mcount, ws_mlist, ws_mheavy;
+    // tile-major HIO state and amplitudes (engine2d.hip)
+    cpra::Buffer ws_state, ws_amp_t;
     // gather list of the carousel merge (merge3d.hip)
     int mplan_n = 0, mplan_P = 0, mplan_heavy = 0;
     int64_t mplan_entries = 0;
@@ -175,7 +177,7 @@ struct KernelTimer {
 // kinds of cpra_hip_get_kernel_times
 enum KernelKind {
     KT_COL = 0, KT_ROW = 1, KT_SW = 2, KT_GD = 3, KT_MERGE = 4, KT_VOL = 5, KT_EXTRACT = 6, KT_PFFT = 7,
-    KT_MIXED = 8
+    KT_MIXED = 8, KT_PENCIL = 9
 };
 
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)

@@ -112,7 +112,8 @@ struct EngineGraph {
     uint64_t amp_batch = 0;
     int iters = 0;
     float beta = 0.f;
-    int rp = 0, skip = 0;
+    int rp = 0, skip = 0, er = 0, col = -1;
+    bool tiled = false;
     void release()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -167,53 +168,94 @@ void engine_check_geometry(cpra_hip_ctx* h, const char* who)
 }
 
 // Column-kernel variant chosen per handle (the autotuner may change it):
-// thread mapping (MAP_BLOCK / MAP_WAVE) and XCD-aware tile order.
+// thread mapping (MAP_BLOCK / MAP_WAVE / MAP_WIDE) and XCD-aware tile order.
+// Tuning code: bit 0 = MAP_WAVE, bit 1 = XCD order, bit 2 = MAP_WIDE.
 struct ColVariant {
     int map = MAP_BLOCK;
     int xcd = 1;
 };
+static int col_variant_code(const ColVariant& cv)
+{
+    return (cv.map == MAP_WAVE ? 1 : 0) | (cv.xcd << 1) | (cv.map == MAP_WIDE ? 4 : 0);
+}
+static ColVariant col_variant_decode(int v)
+{
+    ColVariant cv;
+    cv.map = (v & 4) ? MAP_WIDE : (v & 1);
+    cv.xcd = (v >> 1) & 1;
+    return cv;
+}
+static bool col_variant_valid(int v) { return v >= 0 && v < 8 && (v & 5) != 5; }
 
-template <int M, int MODE, bool SP>
+template <int M, int MODE, int MAP, bool SP, bool TL>
+static void launch_cols_map(hipStream_t st, const Geo& g, int xcd_req, const void* Y, void* Z,
+                            const float* A, int64_t amp_batch, float scale, int skip,
+                            float* rf_part, float gr, float gc, const unsigned* rowflag,
+                            float beta_n, int er, int p0)
+{
+    using G = ColGeom<M, MAP>;
+    const unsigned grid = (unsigned)(g.B * g.N / G::C);
+    const int xcd = (xcd_req && grid % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP, SP, TL>), dim3(grid), dim3(G::TPB), 0, st,
+                       static_cast<const c2<float>*>(Y), static_cast<c2<float>*>(Z), A, amp_batch,
+                       g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er, p0);
+}
+
+template <int M, int MODE, bool SP, bool TL>
 static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, const void* Y,
                            void* Z, const float* A, int64_t amp_batch, float scale, int skip,
                            float* rf_part, float gr, float gc, const unsigned* rowflag,
                            float beta_n, int er, int p0)
 {
-    const unsigned grid = (unsigned)(g.B * g.N / SizeCfg<M>::C);
-    const int xcd = (cv.xcd && grid % 8 == 0) ? 1 : 0;
-    const dim3 blk(SizeCfg<M>::TPB);
-    auto* y = static_cast<const c2<float>*>(Y);
-    auto* z = static_cast<c2<float>*>(Z);
-    if (cv.map == MAP_WAVE)
-        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_WAVE, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er, p0);
-    else
-        hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP_BLOCK, SP>), dim3(grid), blk, 0, st, y, z, A, amp_batch, g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er, p0);
+    int map = cv.map;
+    // the R-factor partials are laid out per MAP_BLOCK tile (rfactor_launch);
+    // wide tiles need N to be a multiple of their width; a tiled state has
+    // the MAP_BLOCK tile width
+    if (map == MAP_WIDE && (TL || MODE == COL_MOD_RF || g.N % ColGeom<M, MAP_WIDE>::C != 0))
+        map = MAP_BLOCK;
+#define CPRA_COLMAP(MP) launch_cols_map<M, MODE, MP, SP, TL>(st, g, cv.xcd, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0)
+    if (map == MAP_WAVE) CPRA_COLMAP(MAP_WAVE);
+    else if constexpr (!TL) {
+        if (map == MAP_WIDE) CPRA_COLMAP(MAP_WIDE);
+        else CPRA_COLMAP(MAP_BLOCK);
+    } else {
+        CPRA_COLMAP(MAP_BLOCK);
+    }
+#undef CPRA_COLMAP
 }
 
 template <int M>
 static void launch_cols_m(hipStream_t st, const Geo& g, const ColVariant& cv, int mode,
                           const void* Y, void* Z, const float* A, int64_t amp_batch, float scale,
                           int skip, float* rf_part, float gr, float gc, const unsigned* rowflag,
-                          float beta_n, int er, int p0)
+                          float beta_n, int er, int p0, bool tiled)
 {
+    if (tiled) {
+        if (mode == COL_MOD && rowflag)
+            launch_cols_mm<M, COL_MOD, true, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0);
+        else if (mode == COL_MOD)
+            launch_cols_mm<M, COL_MOD, false, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        else if (mode == COL_MOD_RF)
+            launch_cols_mm<M, COL_MOD_RF, false, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        else
+            fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major column pass: mode %d", mode);
+        return;
+    }
     if (mode == COL_MOD && rowflag)
-        launch_cols_mm<M, COL_MOD, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0);
+        launch_cols_mm<M, COL_MOD, true, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0);
     else if (mode == COL_MOD)
-        launch_cols_mm<M, COL_MOD, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        launch_cols_mm<M, COL_MOD, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
     else if (mode == COL_MOD_RF)
-        launch_cols_mm<M, COL_MOD_RF, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        launch_cols_mm<M, COL_MOD_RF, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
     else
-        launch_cols_mm<M, COL_BLUR, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        launch_cols_mm<M, COL_BLUR, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
 }
 
 static ColVariant col_variant(const cpra_hip_ctx* h, int M)
 {
     ColVariant cv;
     auto it = h->tuning.find(((uint64_t)1 << 32) | (uint64_t)M);
-    if (it != h->tuning.end()) {
-        cv.map = it->second & 1;
-        cv.xcd = (it->second >> 1) & 1;
-    }
+    if (it != h->tuning.end()) cv = col_variant_decode((int)it->second);
     return cv;
 }
 
@@ -221,10 +263,11 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
                         void* Z, const float* A, int64_t amp_batch, float scale, int skip,
                         float* rf_part, float gr = 0.f, float gc = 0.f,
                         const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0,
-                        int p0 = 0)
+                        int p0 = 0, bool tiled = false)
 {
     const ColVariant cv = col_variant(h, g.M);
-#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0)
+    if (tiled && g.M != g.N) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major state needs square patterns");
+#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0, tiled)
     switch (g.M) {
     case 64: CPRA_COLS(64); break;
     case 128: CPRA_COLS(128); break;
@@ -240,7 +283,7 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
 template <int N>
 static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
                               const uint8_t* mask, float beta, int rp, int er,
-                              const unsigned* rowbits)
+                              const unsigned* rowbits, bool tiled)
 {
     const int64_t lines = g.B * g.M;
     constexpr int LPB = SizeCfg<N>::LPB;
@@ -250,6 +293,18 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
     const float invN = 1.0f / (float)N;
     const dim3 blk(SizeCfg<N>::TPB);
     const bool exact = lines % LPB == 0;
+    if (tiled) {
+        if (g.M != N || N % LPB != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major rows: square patterns only");
+#define CPRA_ROWST(MODE, SP) \
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, true, SP, true>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
+        if (er) {
+            if (rowbits) CPRA_ROWST(ROW_ER, true); else CPRA_ROWST(ROW_ER, false);
+        } else {
+            if (rowbits) CPRA_ROWST(ROW_HIO, true); else CPRA_ROWST(ROW_HIO, false);
+        }
+#undef CPRA_ROWST
+        return;
+    }
 #define CPRA_ROWS(MODE, EX, SP) \
     hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX, SP>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
 #define CPRA_ROWS2(MODE, SP) \
@@ -265,22 +320,66 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
 
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
                             const uint8_t* mask, float beta, int rp, int er,
-                            const unsigned* rowbits = nullptr)
+                            const unsigned* rowbits = nullptr, bool tiled = false)
 {
     switch (g.N) {
-    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
-    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
-    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
-    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
-    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
-    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits); break;
+    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
+    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
+    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
+    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
+    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
+    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
     }
     check_launch("k_eng_rows_hio");
 }
 
+// Entry (dir -1: row-major x -> tile-major state Y) and exit (dir +1, scale
+// 1/N: state -> x) row passes of the tiled HIO engine; in != out.
+template <int N>
+static void launch_retile_n(hipStream_t st, int64_t B, int dir, const void* in, void* out)
+{
+    constexpr int LPB = SizeCfg<N>::LPB;
+    const unsigned grid = (unsigned)(B * N / LPB);
+    auto* i = static_cast<const c2<float>*>(in);
+    auto* o = static_cast<c2<float>*>(out);
+    if (dir < 0)
+        hipLaunchKernelGGL((k_rows_retile<N, -1, false, true>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, i, o, 1.0f);
+    else
+        hipLaunchKernelGGL((k_rows_retile<N, +1, true, false>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, i, o, 1.0f / (float)N);
+}
+
+static void launch_retile(hipStream_t st, const Geo& g, int dir, const void* in, void* out)
+{
+    if (g.M != g.N) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major state needs square patterns");
+    switch (g.N) {
+    case 64: launch_retile_n<64>(st, g.B, dir, in, out); break;
+    case 128: launch_retile_n<128>(st, g.B, dir, in, out); break;
+    case 192: launch_retile_n<192>(st, g.B, dir, in, out); break;
+    case 256: launch_retile_n<256>(st, g.B, dir, in, out); break;
+    case 384: launch_retile_n<384>(st, g.B, dir, in, out); break;
+    case 512: launch_retile_n<512>(st, g.B, dir, in, out); break;
+    }
+    check_launch("k_rows_retile");
+}
+
+static void launch_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
+{
+    const int64_t total = patterns * n * n;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    switch (n) {
+    case 64: hipLaunchKernelGGL(k_amp_retile<64>, grid, dim3(256), 0, st, in, out, total); break;
+    case 128: hipLaunchKernelGGL(k_amp_retile<128>, grid, dim3(256), 0, st, in, out, total); break;
+    case 192: hipLaunchKernelGGL(k_amp_retile<192>, grid, dim3(256), 0, st, in, out, total); break;
+    case 256: hipLaunchKernelGGL(k_amp_retile<256>, grid, dim3(256), 0, st, in, out, total); break;
+    case 384: hipLaunchKernelGGL(k_amp_retile<384>, grid, dim3(256), 0, st, in, out, total); break;
+    case 512: hipLaunchKernelGGL(k_amp_retile<512>, grid, dim3(256), 0, st, in, out, total); break;
+    }
+    check_launch("k_amp_retile");
+}
+
 template <int N>
 static void launch_sw_rows_n(hipStream_t st, const Geo& g, int stage, bool from_state,
-                             const void* in, void* out, float* bl, unsigned* keys)
+                             const void* in, void* out, float* bl, unsigned* keys, bool tiled)
 {
     const int64_t lines = g.B * g.M;
     const unsigned grid = (unsigned)((lines + SizeCfg<N>::LPB - 1) / SizeCfg<N>::LPB);
@@ -288,7 +387,9 @@ static void launch_sw_rows_n(hipStream_t st, const Geo& g, int stage, bool from_
     auto* i = static_cast<const c2<float>*>(in);
     if (stage == 1) {
         auto* o = static_cast<c2<float>*>(out);
-        if (from_state)
+        if (from_state && tiled)
+            hipLaunchKernelGGL((k_sw_mag<N, true, true>), dim3(grid), blk, 0, st, i, o, lines, 1.0f / (float)N);
+        else if (from_state)
             hipLaunchKernelGGL((k_sw_mag<N, true>), dim3(grid), blk, 0, st, i, o, lines, 1.0f / (float)N);
         else
             hipLaunchKernelGGL((k_sw_mag<N, false>), dim3(grid), blk, 0, st, i, o, lines, 1.0f);
@@ -298,15 +399,17 @@ static void launch_sw_rows_n(hipStream_t st, const Geo& g, int stage, bool from_
 }
 
 static void launch_sw_rows(hipStream_t st, const Geo& g, int stage, bool from_state,
-                           const void* in, void* out, float* bl, unsigned* keys)
+                           const void* in, void* out, float* bl, unsigned* keys,
+                           bool tiled = false)
 {
+    if (tiled && g.M != g.N) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major state needs square patterns");
     switch (g.N) {
-    case 64: launch_sw_rows_n<64>(st, g, stage, from_state, in, out, bl, keys); break;
-    case 128: launch_sw_rows_n<128>(st, g, stage, from_state, in, out, bl, keys); break;
-    case 192: launch_sw_rows_n<192>(st, g, stage, from_state, in, out, bl, keys); break;
-    case 256: launch_sw_rows_n<256>(st, g, stage, from_state, in, out, bl, keys); break;
-    case 384: launch_sw_rows_n<384>(st, g, stage, from_state, in, out, bl, keys); break;
-    case 512: launch_sw_rows_n<512>(st, g, stage, from_state, in, out, bl, keys); break;
+    case 64: launch_sw_rows_n<64>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
+    case 128: launch_sw_rows_n<128>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
+    case 192: launch_sw_rows_n<192>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
+    case 256: launch_sw_rows_n<256>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
+    case 384: launch_sw_rows_n<384>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
+    case 512: launch_sw_rows_n<512>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
     }
     check_launch("k_sw_rows");
 }
@@ -316,7 +419,8 @@ static Geo geo_of(cpra_hip_ctx* h) { return Geo{(int)h->m, (int)h->n, (int64_t)h
 // Shrinkwrap on the state (from_state) or on a real-space x; writes the mask
 // and, if blur_out != NULL, copies the blurred image there.
 static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, bool from_state,
-                              uint8_t* mask, float sigma, float frac, float* blur_out)
+                              uint8_t* mask, float sigma, float frac, float* blur_out,
+                              bool tiled = false)
 {
     const Geo g = geo_of(h);
     const int64_t px = (int64_t)g.M * g.N;
@@ -327,7 +431,7 @@ static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, 
     void* W = h->ws_z.ptr;
     float* bl = blur_out ? blur_out : static_cast<float*>(h->ws_blur.ptr);
     unsigned* keys = static_cast<unsigned*>(h->ws_keys.ptr);
-    launch_sw_rows(st, g, 1, from_state, src, W, nullptr, nullptr);
+    launch_sw_rows(st, g, 1, from_state, src, W, nullptr, nullptr, tiled);
     // G(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-M/2)/M)^2 + ((kc-N/2)/N)^2])
     const double k = -2.0 * M_PI * M_PI * (double)sigma * (double)sigma;
     const float gr = (float)(k / ((double)g.M * g.M));
@@ -341,7 +445,7 @@ static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, 
 }
 
 static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, const float* amp,
-                           int64_t amp_batch, int skip, float* rf_dev)
+                           int64_t amp_batch, int skip, float* rf_dev, bool tiled = false)
 {
     const Geo g = geo_of(h);
     const int C = col_tile_rt(g.M);
@@ -349,7 +453,8 @@ static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, co
     h->ws_part.ensure(sizeof(float) * 2 * (size_t)bpp * g.B);
     float* part = static_cast<float*>(h->ws_part.ptr);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
-    launch_cols(h, st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part);
+    launch_cols(h, st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part, 0.f, 0.f,
+                nullptr, 0.f, 0, 0, tiled);
     hipLaunchKernelGGL(k_rf_fin, dim3(grid_1d(g.B, 64)), dim3(64), 0, st, part, bpp, g.B, rf_dev);
     check_launch("k_rf_fin");
 }
@@ -393,9 +498,18 @@ static int64_t iteration_chunk(const cpra_hip_ctx* h)
     return (c <= 0 || c >= (int64_t)h->batch) ? (int64_t)h->batch : c;
 }
 
+// Tile-major HIO state (Tiled<N>): square patterns, chosen by the tuning key
+// 4 (1 = tiled, the default; 0 = row-major in the caller's buffer).
+static bool tiled_state(const cpra_hip_ctx* h)
+{
+    if (h->m != h->n) return false;
+    auto it = h->tuning.find((uint64_t)4 << 32);
+    return it == h->tuning.end() || it->second != 0;
+}
+
 static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
                           int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p,
-                          KernelTimer* kt = nullptr)
+                          KernelTimer* kt = nullptr, bool tiled = false)
 {
     const Geo g = geo_of(h);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
@@ -410,11 +524,11 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
         const unsigned* bc = bits ? bits + p0 * g.M / 32 : nullptr;
         if (kt) kt->begin(h, st, KT_COL);
         launch_cols(h, st, gc, COL_MOD, xc, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero,
-                    nullptr, 0.f, 0.f, bc, p->beta * (float)g.N, p->error_reduction, (int)p0);
+                    nullptr, 0.f, 0.f, bc, p->beta * (float)g.N, p->error_reduction, (int)p0, tiled);
         if (kt) kt->end(h, st);
         if (kt) kt->begin(h, st, KT_ROW);
         launch_rows_hio(st, gc, h->ws_z.ptr, xc, mc, p->beta, p->real_positive, p->error_reduction,
-                        bc);
+                        bc, tiled);
         if (kt) kt->end(h, st);
     }
 }
@@ -577,9 +691,25 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         h->ws_misc.ensure(sizeof(float) * g.B + 256);
         hipStream_t st = h->stream;
         const int nsw = p->sw_every > 0 ? p->iters / p->sw_every : 0;
-        // enter the row-spectral state in place; list the rows with support
-        fft_pass_contig(h, x, g.N, g.B * g.M, -1, true, 1.0, false);
+        // enter the row-spectral state: in place (row-major), or into the
+        // tile-major workspace with a tile-major copy of the amplitudes;
+        // list the rows with support
+        const bool tl = tiled_state(h);
+        void* Y = x;
+        const float* A = amp;
+        if (tl) {
+            const size_t px = (size_t)g.M * g.N;
+            h->ws_state.ensure(sizeof(c2<float>) * total);
+            h->ws_amp_t.ensure(sizeof(float) * px * amp_batch);
+            Y = h->ws_state.ptr;
+            A = static_cast<const float*>(h->ws_amp_t.ptr);
+            launch_amp_retile(st, g.N, (int64_t)amp_batch, amp, static_cast<float*>(h->ws_amp_t.ptr));
+            launch_retile(st, g, -1, x, Y);
+        } else {
+            fft_pass_contig(h, x, g.N, g.B * g.M, -1, true, 1.0, false);
+        }
         if (sparse_rows_enabled(h)) support_rows_launch(h, st, support);
+        const int colv = col_variant_code(col_variant(h, g.M));
         const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
         int done = 0;
         bool use_graph = p->use_graph && !h->timing && p->iters >= chunk && chunk > 0;
@@ -593,23 +723,27 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             CPRA_CHECK_HIP(hipEventRecord(ev0, st));
             CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
             EngineGraph* eg = h->graph;
-            const bool same = eg && eg->x == x && eg->amp == amp && eg->mask == support &&
+            const bool same = eg && eg->x == Y && eg->amp == A && eg->mask == support &&
                               eg->ws == h->ws_z.ptr && eg->ws_rows == h->ws_rows.ptr &&
                               eg->sparse == sparse_rows_enabled(h) &&
                               eg->chunk == iteration_chunk(h) &&
                               eg->amp_batch == amp_batch && eg->iters == chunk &&
                               eg->beta == p->beta && eg->rp == p->real_positive &&
-                              eg->skip == p->dc_skip_zero;
+                              eg->skip == p->dc_skip_zero && eg->er == p->error_reduction &&
+                              eg->col == colv && eg->tiled == tl;
             if (!same) {
                 if (!eg) eg = h->graph = new EngineGraph();
                 eg->release();
                 CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
                 for (int i = 0; i < chunk; ++i)
-                    hio_iteration(h, gst, x, amp, (int64_t)amp_batch, support, p);
+                    hio_iteration(h, gst, Y, A, (int64_t)amp_batch, support, p, nullptr, tl);
                 CPRA_CHECK_HIP(hipStreamEndCapture(gst, &eg->graph));
                 CPRA_CHECK_HIP(hipGraphInstantiate(&eg->exec, eg->graph, nullptr, nullptr, 0));
-                eg->x = x;
-                eg->amp = amp;
+                eg->er = p->error_reduction;
+                eg->col = colv;
+                eg->tiled = tl;
+                eg->x = Y;
+                eg->amp = A;
                 eg->mask = support;
                 eg->ws = h->ws_z.ptr;
                 eg->ws_rows = h->ws_rows.ptr;
@@ -629,7 +763,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 CPRA_CHECK_HIP(hipGraphLaunch(h->graph->exec, gst));
             } else {
                 for (int i = 0; i < todo; ++i)
-                    hio_iteration(h, gst, x, amp, (int64_t)amp_batch, support, p, &kt);
+                    hio_iteration(h, gst, Y, A, (int64_t)amp_batch, support, p, &kt, tl);
             }
             done += todo;
             if (p->sw_every > 0 && todo == chunk && done % p->sw_every == 0) {
@@ -640,7 +774,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 hipStream_t save = h->stream;
                 h->stream = gst;
                 kt.begin(h, gst, KT_SW);
-                shrinkwrap_launch(h, gst, x, true, support, sigma, p->sw_frac, nullptr);
+                shrinkwrap_launch(h, gst, Y, true, support, sigma, p->sw_frac, nullptr, tl);
                 if (sparse_rows_enabled(h)) support_rows_launch(h, gst, support);
                 kt.end(h, gst);
                 h->stream = save;
@@ -654,9 +788,10 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             CPRA_CHECK_HIP(hipEventDestroy(ev1));
         }
         float* rf_dev = static_cast<float*>(h->ws_misc.ptr);
-        if (rfactor_host) rfactor_launch(h, st, x, amp, (int64_t)amp_batch, 1, rf_dev);
+        if (rfactor_host) rfactor_launch(h, st, Y, A, (int64_t)amp_batch, 1, rf_dev, tl);
         // leave the row-spectral state
-        fft_pass_contig(h, x, g.N, g.B * g.M, +1, true, 1.0 / (double)g.N, false);
+        if (tl) launch_retile(st, g, +1, Y, x);
+        else fft_pass_contig(h, x, g.N, g.B * g.M, +1, true, 1.0 / (double)g.N, false);
         if (rfactor_host) {
             CPRA_CHECK_HIP(hipMemcpyAsync(rfactor_host, rf_dev, sizeof(float) * g.B, hipMemcpyDeviceToHost, st));
             CPRA_CHECK_HIP(hipStreamSynchronize(st));
@@ -675,6 +810,86 @@ void cpra_hip_dm_default_params(cpra_hip_dm_params* p)
     p->space_mode = CPRA_HIP_DM_SPACE_MKL;
     p->dc_skip_zero = 1;
     p->use_graph = 1;
+}
+
+// Difference map on volumes [B][L][M][N] (src/CONV_MKL.cpp:47-63).  The
+// state is kept as the DFT over (y, x), Y = S_y S_x DFT_y DFT_x S_y S_x v, so
+// one iteration is four HBM passes over [Y1; Y2]:
+//   z lines   the column kernel on the L x (M*N) view of each volume: DFT_z,
+//             modulus projection with A [L][M][N] (scale 1/(MNL)), IDFT_z;
+//   y lines   inverse y DFT (strided pencil pass);
+//   x lines   the 2D engine's DM row kernel over the B*L*M rows (inverse x
+//             DFT, the update with the per-voxel support, forward x DFT);
+//   y lines   forward y DFT (skipped after the last iteration, whose row pass
+//             writes t1, t2 and psi in real space).
+static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float* amp,
+                     uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p)
+{
+    const int M = (int)h->m, N = (int)h->n, L = (int)h->l;
+    const int64_t B = (int64_t)h->batch;
+    if (!fft_fast_size(L) || !fft_fast_size(M) || !fft_fast_size(N))
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "cpra_hip_dm3d_run: volume %dx%dx%d not in {64,128,192,256,384,512}^3", L, M, N);
+    const size_t total = (size_t)L * M * N * (size_t)B;
+    h->ws_z.ensure(sizeof(c2<float>) * 2 * total);    // [Z1; Z2]
+    h->ws_dm.ensure(sizeof(c2<float>) * 2 * total);   // [Y1; Y2]
+    c2<float>* Y = static_cast<c2<float>*>(h->ws_dm.ptr);
+    c2<float>* Z = static_cast<c2<float>*>(h->ws_z.ptr);
+    hipStream_t st = h->stream;
+    const Geo grow{M, N, B * L};              // x rows of every z-plane
+    const Geo gz2{L, M * N, 2 * B};           // z lines of [Y1; Y2]
+    const int64_t ylines = 2 * B * (int64_t)L * N;  // y lines of [Y1; Y2]
+    const float scale = (float)(1.0 / ((double)M * N * L));
+    KernelTimer kt;
+    auto ypass = [&](hipStream_t s, c2<float>* data, int dir) {
+        hipStream_t save = h->stream;
+        h->stream = s;
+        kt.begin(h, s, KT_PENCIL);
+        fft_pass_strided(h, data, M, ylines, N, N, (int64_t)M * N, dir, true, 1.0, false);
+        kt.end(h, s);
+        h->stream = save;
+    };
+    dm_entry_launch(st, grow, p, static_cast<const c2<float>*>(psi), static_cast<const c2<float>*>(t1),
+                    static_cast<const c2<float>*>(t2), support, Y, Y + total);
+    ypass(st, Y, -1);
+    auto one = [&](hipStream_t s, bool last) {
+        kt.begin(h, s, KT_COL);
+        launch_cols(h, s, gz2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr);
+        kt.end(h, s);
+        ypass(s, Z, +1);
+        kt.begin(h, s, KT_ROW);
+        dm_rows_launch(s, grow, p, last, Z, Z + total, static_cast<c2<float>*>(psi), support, Y,
+                       Y + total, static_cast<c2<float>*>(t1), static_cast<c2<float>*>(t2));
+        kt.end(h, s);
+        if (!last) ypass(s, Y, -1);
+    };
+    const int body = p->iters - 1;
+    if (p->use_graph && !h->timing && body >= 2) {
+        hipStream_t gst = aux_stream(h);
+        hipEvent_t ev0, ev1;
+        CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+        CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+        CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+        CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        // one iteration captured, replayed `body` times
+        CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
+        one(gst, false);
+        CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
+        CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (int i = 0; i < body; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+        CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+        CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+        CPRA_CHECK_HIP(hipStreamSynchronize(gst));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    } else {
+        for (int i = 0; i < body; ++i) one(st, false);
+    }
+    one(st, true);
+    kt.collect(h);
 }
 
 int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
@@ -731,6 +946,26 @@ int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const floa
             for (int i = 0; i < body; ++i) one(st, false);
         }
         one(st, true);
+        return CPRA_HIP_OK;
+    });
+}
+
+// The handle's l is the volume depth here only: the reference's drivers build
+// their 2D handles with the volume's L too (src/CPRA_MKL.cpp:24 passes M, N,
+// L), so cpra_hip_dm_run stays 2D whatever l is.
+int cpra_hip_dm3d_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
+                      uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p)
+{
+    return eguard(h, [&]() -> int {
+        CPRA_ARG(h && psi && amp && support && p, "cpra_hip_dm3d_run: NULL argument");
+        CPRA_ARG(p->textbook || (t1 && t2), "cpra_hip_dm3d_run: the reference variant needs t1 and t2");
+        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0, "cpra_hip_dm3d_run: amp_batch must divide batch");
+        CPRA_ARG(p->iters >= 0, "cpra_hip_dm3d_run: iters < 0");
+        CPRA_ARG(p->beta != 0.f, "cpra_hip_dm3d_run: beta == 0");
+        CPRA_ARG(p->space_mode >= 0 && p->space_mode <= 2, "cpra_hip_dm3d_run: bad space_mode");
+        if (h->dtype != CPRA_HIP_F32)
+            fail(CPRA_HIP_ERR_UNSUPPORTED, "cpra_hip_dm3d_run: the fused engine runs in FP32 (complex64) only");
+        if (p->iters > 0) dm3d_run(h, psi, t1, t2, amp, amp_batch, support, p);
         return CPRA_HIP_OK;
     });
 }
@@ -823,13 +1058,21 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         p.sw_every = 0;
         p.use_graph = 0;
         const uint64_t key = ((uint64_t)1 << 32) | (uint64_t)g.M;
+        const uint64_t tkey = (uint64_t)4 << 32;
         std::string rep;
-        double best = 1e30;
-        int best_v = 0;
         const bool saved_timing = h->timing;
         h->tuning[(uint64_t)2 << 32] = 0;
         h->tuning[(uint64_t)3 << 32] = 0;
-        for (int v = 0; v < 4; ++v) {
+        // column variants per state layout (row-major; tile-major for square
+        // patterns, where the 32-column tiles do not apply)
+        const int nlayout = g.M == g.N ? 2 : 1;
+        int best_vl[2] = {0, 0};
+        double best_l[2] = {1e30, 1e30};
+        for (int lay = 0; lay < nlayout; ++lay)
+        for (int v = 0; v < 8; ++v) {
+            if (!col_variant_valid(v)) continue;
+            if ((v & 4) && (lay == 1 || g.N % (2 * col_tile_rt(g.M)) != 0)) continue;
+            h->tuning[tkey] = lay;
             h->tuning[key] = v;
             h->timing = true;
             for (int i = 0; i < 16; ++i) { h->kt_ms[i] = 0; h->kt_n[i] = 0; }
@@ -844,15 +1087,14 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
             if (rc < 0) return rc;
             const double us = 1e3 * h->kt_ms[0] / (double)(h->kt_n[0] > 0 ? h->kt_n[0] : 1);
             char buf[96];
-            std::snprintf(buf, sizeof(buf), "col_v%d_us=%.2f;", v, us);
+            std::snprintf(buf, sizeof(buf), "%scol_v%d_us=%.2f;", lay ? "t" : "", v, us);
             rep += buf;
-            if (us < best) {
-                best = us;
-                best_v = v;
+            if (us < best_l[lay]) {
+                best_l[lay] = us;
+                best_vl[lay] = v;
             }
             CPRA_CHECK_HIP(hipMemcpyAsync(x, bx, sizeof(c2<float>) * total, hipMemcpyDeviceToDevice, st));
         }
-        h->tuning[key] = best_v;
         // whole iterations as the engine runs them (graph replays), timed with
         // events around the run: dense against sparse rows, then the patterns
         // per launch (whole batch against Infinity-Cache-sized chunks)
@@ -881,6 +1123,23 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         };
         const uint64_t skey = (uint64_t)2 << 32;
         const uint64_t ckey = (uint64_t)3 << 32;
+        // layout, each with its own best column variant
+        int best_lay = 0;
+        if (nlayout == 2) {
+            double lay_us[2];
+            for (int lay = 0; lay < 2; ++lay) {
+                h->tuning[tkey] = lay;
+                h->tuning[key] = best_vl[lay];
+                lay_us[lay] = run_us(iters);
+            }
+            best_lay = lay_us[1] < lay_us[0] ? 1 : 0;
+            char buf[128];
+            std::snprintf(buf, sizeof(buf), "iter_rows_us=%.2f;iter_tiled_us=%.2f;", lay_us[0], lay_us[1]);
+            rep += buf;
+        }
+        h->tuning[tkey] = best_lay;
+        const int best_v = best_vl[best_lay];
+        h->tuning[key] = best_v;
         double it_us[2] = {0, 0};
         for (int sp = 0; sp < 2; ++sp) {
             h->tuning[skey] = sp;
@@ -913,9 +1172,9 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
         char head[128];
-        std::snprintf(head, sizeof(head), "col_%d=%d;rows_sparse=%d;chunk=%d;col_map=%s;col_xcd=%d;",
-                      g.M, best_v, best_sp, best_chunk, (best_v & 1) ? "wave" : "block",
-                      (best_v >> 1) & 1);
+        std::snprintf(head, sizeof(head), "col_%d=%d;rows_sparse=%d;chunk=%d;tiled=%d;col_map=%s;col_xcd=%d;",
+                      g.M, best_v, best_sp, best_chunk, best_lay,
+                      (best_v & 4) ? "wide" : ((best_v & 1) ? "wave" : "block"), (best_v >> 1) & 1);
         rep = head + rep;
         if (report && report_len) {
             std::snprintf(report, report_len, "%s", rep.c_str());
@@ -940,8 +1199,11 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
             } else if (std::sscanf(p, "rows_sparse=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)2 << 32] = v ? 1 : 0;
                 p += used;
+            } else if (std::sscanf(p, "tiled=%d;%n", &v, &used) == 1 && used > 0) {
+                h->tuning[(uint64_t)4 << 32] = v ? 1 : 0;
+                p += used;
             } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
-                CPRA_ARG(v >= 0 && v < 4, "cpra_hip_set_tuning: variant %d out of range", v);
+                CPRA_ARG(col_variant_valid(v), "cpra_hip_set_tuning: variant %d out of range", v);
                 h->tuning[((uint64_t)1 << 32) | (uint64_t)m] = v;
                 p += used;
             } else {
@@ -959,9 +1221,9 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
     return eguard(h, [&]() -> int {
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
-        std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;", (int)h->m,
-                      cv.map | (cv.xcd << 1), sparse_rows_enabled(h) ? 1 : 0,
-                      (long long)iteration_chunk(h));
+        std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;", (int)h->m,
+                      col_variant_code(cv), sparse_rows_enabled(h) ? 1 : 0,
+                      (long long)iteration_chunk(h), tiled_state(h) ? 1 : 0);
         return CPRA_HIP_OK;
     });
 }

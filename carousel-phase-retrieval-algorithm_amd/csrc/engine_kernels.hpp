@@ -31,31 +31,92 @@ __device__ __forceinline__ unsigned tile_index(int xcd_remap)
     return (b & 7u) * (gridDim.x >> 3) + (b >> 3);
 }
 
+// Tile-major layout of an N x N pattern (the HIO engine's state when tiled):
+// the columns are cut into tiles of C = SizeCfg<N>::C (the column kernel's
+// tile width) and element (r, c) sits at ((c / C) * N + r) * C + c % C.  A
+// column-kernel block then reads and writes one contiguous N*C block instead
+// of N row segments of C*8 bytes N*8 bytes apart, and a row-kernel wave's
+// load of one register over its 64/T rows is one contiguous run.
+template <int N> struct Tiled {
+    static constexpr int C = SizeCfg<N>::C;
+    static constexpr int T = SizeCfg<N>::P::T;  // threads per row line
+    static_assert(C % T == 0 || T % C == 0, "tile width and row threads must nest");
+    // offset in a pattern of element c = K*T + t of row r, split into a
+    // compile-time part per register K (a wave-uniform base: the access
+    // becomes SGPR base + one 32-bit VGPR offset, as the strides between
+    // registers exceed the load instructions' immediate offsets) and the
+    // lane part shared by all registers
+    template <int K> static constexpr size_t kpart()
+    {
+        if constexpr (C % T == 0) return (size_t)((K / (C / T)) * N * C + (K % (C / T)) * T);
+        else return (size_t)K * (T / C) * N * C;
+    }
+    // lane part for row r = r0 + rl: r0 (block-uniform) goes into the base
+    // as r0 * C; rl and t come from threadIdx, so the compiler knows the
+    // offset's range and can use it as a 32-bit VGPR offset
+    __device__ static unsigned lane(int t, int rl)
+    {
+        if constexpr (C % T == 0) return (unsigned)(rl * C + t);
+        else return (unsigned)((t / C) * N * C + rl * C + t % C);
+    }
+};
+
+// complex64 access through a buffer resource: SGPR base, SGPR byte offset
+// (soff, a compile-time constant per register) and one VGPR byte offset per
+// lane.  Used where the strides between a lane's registers exceed the
+// immediate offsets of global instructions, which would otherwise cost a
+// 64-bit VGPR address per register.  Raw buffer (stride 0): num_records is a
+// byte count; the kernels only form offsets inside the pattern.
+struct BufC64 {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ BufC64(const void* base, unsigned bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                              0x00020000))
+    {
+    }
+    __device__ c2<float> ld(unsigned voff, unsigned soff) const
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return __builtin_bit_cast(c2<float>, v);
+    }
+    __device__ void st(c2<float> x, unsigned voff, unsigned soff) const
+    {
+        using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, x), r, voff, soff, 0);
+    }
+};
+
 enum ColMode { COL_MOD = 0, COL_MOD_RF = 1, COL_BLUR = 2 };
-enum ColMap { MAP_BLOCK = 0, MAP_WAVE = 1 };
+enum ColMap { MAP_BLOCK = 0, MAP_WAVE = 1, MAP_WIDE = 2 };
 
 // Thread <-> (column, t) mapping of the column kernels.  A block owns C
-// consecutive columns of one pattern and has C*T = 256 threads.
+// consecutive columns of one pattern and has TPB = C*T threads.
 //  MAP_BLOCK: lane-consecutive columns across the whole block (a 16-column
 //             row segment is one 128-byte line per load); the column FFT's
 //             LDS exchange spans the block's waves -> workgroup barriers.
 //  MAP_WAVE:  each wave owns CW = 64/T whole columns, so the exchange stays
 //             inside the wave (no workgroup barrier) at the price of 32-byte
 //             row segments per wave load.
+//  MAP_WIDE:  MAP_BLOCK over twice the columns with twice the threads (512):
+//             256-byte row segments per load.  Column-strided HBM reads run
+//             closer to the contiguous rate the wider the segment; 512-byte
+//             segments (1024 threads) lose more to the lower block count.
 template <int M, int MAP> struct ColGeom {
     using PL = typename SizeCfg<M>::P;
     static constexpr int T = PL::T;
-    static constexpr int C = SizeCfg<M>::C;
+    static constexpr int C = (MAP == MAP_WIDE ? 2 : 1) * SizeCfg<M>::C;
+    static constexpr int TPB = (MAP == MAP_WIDE ? 2 : 1) * SizeCfg<M>::TPB;
     static constexpr int CW = 64 / T;
-    static_assert(MAP == MAP_BLOCK || C == 4 * CW, "MAP_WAVE needs 4 waves of CW columns");
+    static_assert(MAP != MAP_WAVE || C == 4 * CW, "MAP_WAVE needs 4 waves of CW columns");
+    static_assert(C * T == TPB, "column tile");
     __device__ static int col(int tid)
     {
-        if constexpr (MAP == MAP_BLOCK) return tid % C;
+        if constexpr (MAP != MAP_WAVE) return tid % C;
         else return (tid >> 6) * CW + (tid & 63) % CW;
     }
     __device__ static int t(int tid)
     {
-        if constexpr (MAP == MAP_BLOCK) return tid / C;
+        if constexpr (MAP != MAP_WAVE) return tid / C;
         else return (tid & 63) / CW;
     }
 };
@@ -76,7 +137,9 @@ template <int M, int MAP> struct ColGeom {
 // caller, with the caller's LDS (M*C elements; `red` 2*TPB floats for
 // COL_MOD_RF), so both the single-size kernel below and the mixed-size
 // kernels (one launch over several pattern sizes) run it.
-template <int M, int MODE, int MAP, bool SPARSE>
+// TILED: Y, Z and A are in the tile-major layout (Tiled<M>, square patterns):
+// the block's tile is one contiguous M x C block with row stride C.
+template <int M, int MODE, int MAP, bool SPARSE, bool TILED = false>
 __device__ __forceinline__ void eng_cols_tile(
     const unsigned tile, c2<float>* lds, float* red, const c2<float>* Y, c2<float>* Z,
     const float* __restrict__ A, int64_t amp_batch, int N, float scale, int skip_zero,
@@ -95,13 +158,18 @@ __device__ __forceinline__ void eng_cols_tile(
     const int col0 = (int)(tile - b * tiles_per_pattern) * C;
     const int col = col0 + c;
     const size_t pat = (size_t)M * (size_t)N;
-    const c2<float>* Yp = Y + b * pat + col0;
-    c2<float>* Zp = Z + b * pat + col0;
+    static_assert(!TILED || MAP != MAP_WIDE, "tiled state uses the layout's tile width");
+    // row stride and the block's first element: row-major (stride N, column
+    // col0 of pattern b) or tile-major (stride C, tile `tile` of the launch)
+    const unsigned RS = TILED ? (unsigned)C : (unsigned)N;
+    const size_t blk0 = TILED ? (size_t)tile * M * C : b * pat + col0;
+    const c2<float>* Yp = Y + blk0;
+    c2<float>* Zp = Z + blk0;
     // element m of this lane sits at row t + rot(m)*T: a wave-uniform row base
-    // (rot(m)*T*N) plus one per-lane offset (t*N + c) shared by all m, so each
-    // access is an SGPR base + one VGPR offset
-    const unsigned toff = (unsigned)t * (unsigned)N + (unsigned)c;
-    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)N; };
+    // (rot(m)*T*RS) plus one per-lane offset (t*RS + c) shared by all m, so
+    // each access is an SGPR base + one VGPR offset
+    const unsigned toff = (unsigned)t * RS + (unsigned)c;
+    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)RS; };
     V v[E];
     float av[E];
     sfor<E>([&](auto mc) {
@@ -113,7 +181,8 @@ __device__ __forceinline__ void eng_cols_tile(
         // their latency overlaps the transform
         // pattern0: index of this launch's first pattern in the whole batch
         // (chunked launches), for the amplitude broadcast b % amp_batch
-        const float* Ap = A + (size_t)(((unsigned)pattern0 + b) % (unsigned)amp_batch) * pat + col0;
+        const size_t abase = (size_t)(((unsigned)pattern0 + b) % (unsigned)amp_batch) * pat;
+        const float* Ap = A + abase + (TILED ? (size_t)(tile - b * tiles_per_pattern) * M * C : (size_t)col0);
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             av[m] = (Ap + rowbase(m))[toff];
@@ -152,7 +221,7 @@ __device__ __forceinline__ void eng_cols_tile(
         }
         fft_line<PL, +1>(v, t, g_tw<M>(), acc);
     };
-    if constexpr (MAP == MAP_BLOCK) {
+    if constexpr (MAP != MAP_WAVE) {
         LdsCol<V, C, false> acc{lds + c};
         body(acc);
     } else {
@@ -196,7 +265,7 @@ __device__ __forceinline__ void eng_cols_tile(
         });
     }
     if constexpr (MODE == COL_MOD_RF) {
-        constexpr int TPB = SizeCfg<M>::TPB;
+        constexpr int TPB = G::TPB;
         red[tid] = rnum;
         red[TPB + tid] = rden;
         __syncthreads();
@@ -214,21 +283,21 @@ __device__ __forceinline__ void eng_cols_tile(
     }
 }
 
-template <int M, int MODE, int MAP, bool SPARSE = false>
-__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_eng_cols(
+template <int M, int MODE, int MAP, bool SPARSE = false, bool TILED = false>
+__global__ __launch_bounds__((ColGeom<M, MAP>::TPB)) void k_eng_cols(
     const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
     float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
     int xcd_remap, const unsigned* __restrict__ rowbits = nullptr, float beta_n = 0.f,
     int error_reduction = 0, int pattern0 = 0)
 {
-    __shared__ c2<float> lds[M * SizeCfg<M>::C];
+    __shared__ c2<float> lds[M * ColGeom<M, MAP>::C];
     if constexpr (MODE == COL_MOD_RF) {
-        __shared__ float red[2 * SizeCfg<M>::TPB];
-        eng_cols_tile<M, MODE, MAP, SPARSE>(tile_index(xcd_remap), lds, red, Y, Z, A, amp_batch, N, scale,
+        __shared__ float red[2 * ColGeom<M, MAP>::TPB];
+        eng_cols_tile<M, MODE, MAP, SPARSE, TILED>(tile_index(xcd_remap), lds, red, Y, Z, A, amp_batch, N, scale,
                                             skip_zero, rf_part, gcoef_r, gcoef_c, rowbits, beta_n,
                                             error_reduction, pattern0);
     } else {
-        eng_cols_tile<M, MODE, MAP, SPARSE>(tile_index(xcd_remap), lds, nullptr, Y, Z, A, amp_batch, N,
+        eng_cols_tile<M, MODE, MAP, SPARSE, TILED>(tile_index(xcd_remap), lds, nullptr, Y, Z, A, amp_batch, N,
                                             scale, skip_zero, rf_part, gcoef_r, gcoef_c, rowbits, beta_n,
                                             error_reduction, pattern0);
     }
@@ -244,7 +313,9 @@ enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
 // rowbits != NULL (sparse rows): lines without a support pixel were finished
 // by the column kernel; they are skipped, and a wave whose lines are all
 // finished returns at once (the row exchange never leaves the wave).
-template <int N, int MODE, bool EXACT, bool SPARSE>
+// TILED: Z and Y in the tile-major layout of N x N patterns (Tiled<N>), the
+// mask row-major; a block's LPB lines then lie in one pattern (LPB | N).
+template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false>
 __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
     int64_t nlines, float beta, float invN, int real_positive,
@@ -263,21 +334,40 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
         valid = valid && ((rowbits[line >> 5] >> (line & 31)) & 1u);
         if (__ballot(valid) == 0ull) return;
     }
+    static_assert(!TILED || (EXACT && N % LPB == 0), "tiled rows: whole blocks inside a pattern");
     // block-uniform base, per-lane 32-bit offsets
     const size_t bbase = (size_t)blockIdx.x * LPB * N;
     const int lo = ll * N + t;
-    const c2<float>* Zb = Z + bbase;
-    c2<float>* Yb = Y + bbase;
+    // tiled state: pattern and first row of the block; element m at the
+    // register base kb(m) plus the lane offset sl
+    const size_t pbase = TILED ? ((size_t)blockIdx.x * LPB / N) * N * N
+                                     + (((size_t)blockIdx.x * LPB) % N) * Tiled<N>::C
+                               : bbase;
+    const unsigned sl = TILED ? Tiled<N>::lane(t, ll) : (unsigned)lo;
+    auto kb = [&](auto mc) -> size_t {
+        constexpr int m = decltype(mc)::value;
+        if constexpr (TILED) return Tiled<N>::template kpart<PL::rot(m)>();
+        else return (size_t)(PL::rot(m) * TT);
+    };
+    const c2<float>* Zb = Z + pbase;
+    c2<float>* Yb = Y + pbase;
     const uint8_t* Sb = mask + bbase;
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
     V xp[E], xo[E];
     uint8_t sm[E];
+    const BufC64 zr(Zb, (unsigned)(N * N * sizeof(V))), yr(Yb, (unsigned)(N * N * sizeof(V)));
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const int o = lo + PL::rot(m) * TT;
-        xp[m] = valid ? Zb[o] : mk<V>(0, 0);
-        xo[m] = valid ? Yb[o] : mk<V>(0, 0);
+        if constexpr (TILED) {
+            constexpr unsigned kbytes = (unsigned)(Tiled<N>::template kpart<PL::rot(m)>() * sizeof(V));
+            xp[m] = valid ? zr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
+            xo[m] = valid ? yr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
+        } else {
+            xp[m] = valid ? (Zb + kb(mc))[sl] : mk<V>(0, 0);
+            xo[m] = valid ? (Yb + kb(mc))[sl] : mk<V>(0, 0);
+        }
         sm[m] = valid ? Sb[o] : (uint8_t)0;
     });
     fft_line<PL, +1>(xp, t, tw, acc);
@@ -300,14 +390,76 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     if (valid) {
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            Yb[lo + PL::rot(m) * TT] = xo[m];
+            if constexpr (TILED) {
+                constexpr unsigned kbytes = (unsigned)(Tiled<N>::template kpart<PL::rot(m)>() * sizeof(V));
+                yr.st(xo[m], sl * (unsigned)sizeof(V), kbytes);
+            } else {
+                (Yb + kb(mc))[sl] = xo[m];
+            }
         });
     }
 }
 
+// Row FFT pass between the row-major and the tile-major layouts of N x N
+// patterns (not in place): out = S_r DFT_r^DIR S_r in, times scale.  Entry
+// (x -> tiled state, DIR = -1) and exit (tiled state -> x, DIR = +1, 1/N)
+// of the tiled HIO engine.
+template <int N, int DIR, bool IN_TILED, bool OUT_TILED>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_rows_retile(const c2<float>* __restrict__ in,
+                                                                 c2<float>* __restrict__ out,
+                                                                 float scale)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    static_assert(N % LPB == 0, "whole blocks inside a pattern");
+    __shared__ V lds[LPB * SPAN];
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const size_t line0 = (size_t)blockIdx.x * LPB;
+    const size_t pbase = (line0 / N) * N * N;
+    const size_t r0 = line0 % N;  // the block's first row
+    const int lo = ll * N + t;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    V v[E];
+    const c2<float>* ib = in + pbase + r0 * (IN_TILED ? Tiled<N>::C : N);
+    c2<float>* ob = out + pbase + r0 * (OUT_TILED ? Tiled<N>::C : N);
+    const unsigned il = IN_TILED ? Tiled<N>::lane(t, ll) : (unsigned)lo;
+    const unsigned ol = OUT_TILED ? Tiled<N>::lane(t, ll) : (unsigned)lo;
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        constexpr size_t kb = IN_TILED ? Tiled<N>::template kpart<PL::rot(m)>() : (size_t)(PL::rot(m) * TT);
+        v[m] = (ib + kb)[il];
+    });
+    fft_line<PL, DIR>(v, t, g_tw<N>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        constexpr size_t kb = OUT_TILED ? Tiled<N>::template kpart<PL::rot(m)>() : (size_t)(PL::rot(m) * TT);
+        (ob + kb)[ol] = cscale(v[m], scale);
+    });
+}
+
+// Amplitudes [B][N][N] row-major -> tile-major (Tiled<N>).
+template <int N>
+__global__ __launch_bounds__(256) void k_amp_retile(const float* __restrict__ in,
+                                                    float* __restrict__ out, int64_t total)
+{
+    constexpr int C = Tiled<N>::C;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // tiled index
+    if (i >= total) return;
+    const int64_t b = i / ((int64_t)N * N);
+    const int64_t w = i - b * N * N;
+    const int64_t tile = w / ((int64_t)N * C);
+    const int64_t rr = (w / C) % N;
+    const int64_t cc = w % C;
+    out[i] = in[b * N * N + rr * N + tile * C + cc];
+}
+
 // Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
 // read directly.
-template <int N, bool FROM_STATE>
+// TILED (with FROM_STATE): the state Y is tile-major (Tiled<N>), W row-major.
+template <int N, bool FROM_STATE, bool TILED = false>
 __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __restrict__ Y,
                                                            c2<float>* __restrict__ W,
                                                            int64_t nlines, float invN)
@@ -325,9 +477,16 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __r
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
     V v[E];
+    // tiled: a block's LPB lines lie in one pattern (LPB | N, whole patterns)
+    const size_t line0 = (size_t)blockIdx.x * LPB;
+    const c2<float>* Yp = Y + (line0 / N) * N * N + (line0 % N) * Tiled<N>::C;
+    const unsigned yl = TILED ? Tiled<N>::lane(t, ll) : 0u;
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+        if constexpr (TILED)
+            v[m] = valid ? (Yp + Tiled<N>::template kpart<PL::rot(m)>())[yl] : mk<V>(0, 0);
+        else
+            v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
     });
     if constexpr (FROM_STATE) fft_line<PL, +1>(v, t, tw, acc);
     sfor<E>([&](auto mc) {

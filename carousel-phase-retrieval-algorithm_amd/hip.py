@@ -131,6 +131,7 @@ _SIGS = {
     "cpra_hip_shrinkwrap": (C.c_int, [_H, _P, _P, C.c_float, C.c_float, _P]),
     "cpra_hip_dm_default_params": (None, [C.POINTER(DmParams)]),
     "cpra_hip_dm_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
+    "cpra_hip_dm3d_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
     "cpra_hip_rfactor": (C.c_int, [_H, _P, _P, _U64, _P]),
     "cpra_hip_support_from_constraint": (C.c_int, [_H, _P, _P, _U64]),
     "cpra_hip_real2d_to_3d": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
@@ -422,6 +423,18 @@ class HipImpl:
         return self._check(lib().cpra_hip_dm_run(self._h, _addr(psi), _addr(t1), _addr(t2),
                                                  _addr(amp), ab, _addr(support), C.byref(p)))
 
+    def dm3d_run(self, psi, amp, support, t1=None, t2=None, amp_batch=None, **kw) -> bool:
+        """Difference map on volumes [batch][l][m][n] (cpra_hip_dm3d_run)."""
+        p = DmParams()
+        lib().cpra_hip_dm_default_params(C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n * self.l)
+        return self._check(lib().cpra_hip_dm3d_run(self._h, _addr(psi), _addr(t1), _addr(t2),
+                                                   _addr(amp), ab, _addr(support), C.byref(p)))
+
     def carousel3d_run(self, x, amp, support2d, angles, support3d, vol, amp_batch=None,
                        want_rfactor: bool = False, **kw):
         """3D carousel reconstruction (cpra_hip_carousel3d_run): x [P][n][n]
@@ -493,7 +506,7 @@ class HipImpl:
     def kernel_times(self):
         """{kind: (total_ms, launches)} per fused-kernel kind."""
         names = ("column", "row", "shrinkwrap", "gd", "merge", "volume", "extract", "proj_fft",
-                 "mixed_iteration")
+                 "mixed_iteration", "pencil")
         ms = (C.c_double * len(names))()
         cnt = (C.c_int64 * len(names))()
         self._check(lib().cpra_hip_get_kernel_times(self._h, ms, cnt, len(names)))

@@ -55,7 +55,9 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
  * equivalent of its float constraint (cpra_hip_support_from_constraint),
  * SPACE_POSITIVE additionally clamps Re < 0, SPACE_COMPLEX keeps complex
  * values inside the support.  psi (and t1, t2 for the reference variant) are
- * [batch][m][n] complex64 in pattern space, updated in place. */
+ * [batch][m][n] complex64 in pattern space, updated in place.  The handle's
+ * l is ignored (the reference's drivers give their 2D handles the volume's
+ * depth, src/CPRA_MKL.cpp:24). */
 #define CPRA_HIP_DM_SPACE_MKL 0
 #define CPRA_HIP_DM_SPACE_POSITIVE 1
 #define CPRA_HIP_DM_SPACE_COMPLEX 2
@@ -71,6 +73,14 @@ typedef struct {
 void cpra_hip_dm_default_params(cpra_hip_dm_params* p);
 int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
                     uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p);
+/* The same map on volumes [batch][l][m][n] of the handle's 3D geometry (l, m,
+ * n in {64,128,192,256,384,512}), with Forward3D/Backward3D in P_M: the
+ * iteration body of the reference's conventional reconstruction,
+ * src/CONV_MKL.cpp:47-63.  amp [amp_batch][l][m][n], support u8
+ * [batch][l][m][n].  Four HBM passes per iteration: z lines with the modulus
+ * projection, y lines, x rows with the update, y lines. */
+int cpra_hip_dm3d_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
+                      uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p);
 
 /* Shrinkwrap of x [batch][m][n]: b = Re Backward2D(Forward2D(|x|) * G_sigma),
  * G_sigma(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-m/2)/m)^2 + ((kc-n/2)/n)^2]);
@@ -89,7 +99,8 @@ int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t*
                                      uint64_t num);
 
 /* Autotuner (north-star addition): times each column-kernel variant
- * (thread mapping x XCD-aware tile order) for `iters` eager HIO iterations on
+ * (thread mapping: 16-column blocks, per-wave columns or 32-column blocks of
+ * 512 threads, x XCD-aware tile order) for `iters` eager HIO iterations on
  * the caller's data, then dense against sparse row passes (rows without a
  * support pixel finished in the column kernel) and the patterns per launch
  * (whole batch or Infinity-Cache-sized chunks) on whole iterations, keeps
@@ -112,7 +123,8 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
  * gradient-descent pass, 4 = 2D->3D merge, 5 = 3D volume constraint (its
  * five FFT passes), 6 = slice extraction, 7 = column pass of the carousel's
  * projections, 8 = one whole mixed-batch iteration of the per-class and
- * per-pattern modes; the fused mode records 0 and 1), the summed milliseconds
+ * per-pattern modes; the fused mode records 0 and 1; 9 = the y-line pencil
+ * passes of the 3D difference map), the summed milliseconds
  * and the launch count since the last enable (n <= 16 kinds). */
 int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable);
 int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n);

@@ -393,21 +393,26 @@ int orc_merge_add_f64(const double* src, double* dst, double alpha, double beta,
 /* compositions                                                              */
 /* ------------------------------------------------------------------------ */
 
-/* P_M on one batch member: Forward2D, DataConstraint, Backward2D. */
-static void pm_one(double* z, const double* A, int64_t M, int64_t N, int skip_zero)
+/* P_M on one batch member: Forward2D, DataConstraint, Backward2D (L == 1),
+ * or Forward3D, DataConstraint, Backward3D on an L x M x N volume
+ * (src/CONV_MKL.cpp:48-50). */
+static void pm_one(double* z, const double* A, int64_t M, int64_t N, int64_t L, int skip_zero)
 {
-    orc_forward2d(z, M, N, 1);
-    orc_data_constraint_f64(z, A, (uint64_t)(M * N), 1, skip_zero);
-    orc_backward2d(z, M, N, 1);
+    if (L == 1) orc_forward2d(z, M, N, 1);
+    else orc_forward3d(z, M, N, L, 1);
+    orc_data_constraint_f64(z, A, (uint64_t)(M * N * L), 1, skip_zero);
+    if (L == 1) orc_backward2d(z, M, N, 1);
+    else orc_backward3d(z, M, N, L, 1);
 }
 
-/* src/CPRA_MKL.cpp:94-110 (one batch member at a time; the reference's
- * batched calls are independent per member). */
-int orc_dm_ref(double* psi, double* t1, double* t2, const double* A, const double* S,
-               int64_t M, int64_t N, int64_t B, int64_t amp_batch, double beta,
-               int64_t iters)
+/* src/CPRA_MKL.cpp:94-110 in 2D, src/CONV_MKL.cpp:47-63 in 3D (the same
+ * composition over Forward3D/Backward3D); one batch member at a time (the
+ * reference's batched calls are independent per member). */
+static int dm_ref_geom(double* psi, double* t1, double* t2, const double* A, const double* S,
+                       int64_t M, int64_t N, int64_t L, int64_t B, int64_t amp_batch,
+                       double beta, int64_t iters)
 {
-    const int64_t px = M * N;
+    const int64_t px = M * N * L;
     if (amp_batch < 1 || B % amp_batch) return 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
@@ -419,15 +424,15 @@ int orc_dm_ref(double* psi, double* t1, double* t2, const double* A, const doubl
         const double* Ab = A + (b % amp_batch) * px;
         const double* Sb = S + (b % amp_batch) * px;
         for (int64_t it = 0; it < iters; ++it) {
-            /* Part 1: CPRA_MKL.cpp:95-99 */
-            pm_one(a1, Ab, M, N, 1);
+            /* Part 1: CPRA_MKL.cpp:95-99 (CONV_MKL.cpp:48-52) */
+            pm_one(a1, Ab, M, N, L, 1);
             orc_merge_add_f64(p, a1, -1.0 / beta, 1.0 + 1.0 / beta, (uint64_t)px);
             orc_space_constraint_f64(a1, Sb, (uint64_t)px, 1);
-            /* Part 2: CPRA_MKL.cpp:102-106 */
+            /* Part 2: CPRA_MKL.cpp:102-106 (CONV_MKL.cpp:55-59) */
             orc_space_constraint_f64(a2, Sb, (uint64_t)px, 1);
             orc_merge_add_f64(p, a2, 1.0 / beta, 1.0 - 1.0 / beta, (uint64_t)px);
-            pm_one(a2, Ab, M, N, 1);
-            /* Merge: CPRA_MKL.cpp:109-110 */
+            pm_one(a2, Ab, M, N, L, 1);
+            /* Merge: CPRA_MKL.cpp:109-110 (CONV_MKL.cpp:62-63) */
             orc_merge_add_f64(a1, p, beta, 1.0, (uint64_t)px);
             orc_merge_add_f64(a2, p, -1.0 * beta, 1.0, (uint64_t)px);
         }
@@ -435,23 +440,50 @@ int orc_dm_ref(double* psi, double* t1, double* t2, const double* A, const doubl
     return 1;
 }
 
-/* src/CPRA_MKL.cpp:126-140 */
-int orc_final_projection(double* psi, const double* A, const double* S, int64_t M,
-                         int64_t N, int64_t B, int64_t amp_batch, double beta)
+int orc_dm_ref(double* psi, double* t1, double* t2, const double* A, const double* S,
+               int64_t M, int64_t N, int64_t B, int64_t amp_batch, double beta,
+               int64_t iters)
 {
-    const int64_t px = M * N;
+    return dm_ref_geom(psi, t1, t2, A, S, M, N, 1, B, amp_batch, beta, iters);
+}
+
+int orc_dm_ref3d(double* psi, double* t1, double* t2, const double* A, const double* S,
+                 int64_t M, int64_t N, int64_t L, int64_t B, int64_t amp_batch, double beta,
+                 int64_t iters)
+{
+    return dm_ref_geom(psi, t1, t2, A, S, M, N, L, B, amp_batch, beta, iters);
+}
+
+/* src/CPRA_MKL.cpp:126-140 in 2D, src/CONV_MKL.cpp:78-88 in 3D */
+static int final_projection_geom(double* psi, const double* A, const double* S, int64_t M,
+                                 int64_t N, int64_t L, int64_t B, int64_t amp_batch,
+                                 double beta)
+{
+    const int64_t px = M * N * L;
     if (amp_batch < 1 || B % amp_batch) return 0;
     double* t = (double*)malloc(sizeof(double) * 2 * (size_t)px);
     for (int64_t b = 0; b < B; ++b) {
         double* p = psi + 2 * b * px;
         memcpy(t, p, sizeof(double) * 2 * (size_t)px);
-        pm_one(t, A + (b % amp_batch) * px, M, N, 1);
+        pm_one(t, A + (b % amp_batch) * px, M, N, L, 1);
         orc_merge_add_f64(p, t, -1.0 / beta, 1.0 + 1.0 / beta, (uint64_t)px);
         orc_space_constraint_f64(t, S + (b % amp_batch) * px, (uint64_t)px, 1);
         memcpy(p, t, sizeof(double) * 2 * (size_t)px);
     }
     free(t);
     return 1;
+}
+
+int orc_final_projection(double* psi, const double* A, const double* S, int64_t M,
+                         int64_t N, int64_t B, int64_t amp_batch, double beta)
+{
+    return final_projection_geom(psi, A, S, M, N, 1, B, amp_batch, beta);
+}
+
+int orc_final_projection3d(double* psi, const double* A, const double* S, int64_t M,
+                           int64_t N, int64_t L, int64_t B, int64_t amp_batch, double beta)
+{
+    return final_projection_geom(psi, A, S, M, N, L, B, amp_batch, beta);
 }
 
 static void ps_apply(double* z, const double* S, const uint8_t* mask, int64_t px,
@@ -494,7 +526,7 @@ int orc_dm_textbook(double* psi, const double* A, const double* S, const uint8_t
         for (int64_t it = 0; it < iters; ++it) {
             /* u = P_S((1+1/beta) P_M psi - psi/beta) */
             memcpy(u, p, sizeof(double) * 2 * (size_t)px);
-            pm_one(u, Ab, M, N, 1);
+            pm_one(u, Ab, M, N, 1, 1);
             for (int64_t i = 0; i < 2 * px; ++i)
                 u[i] = (1.0 + 1.0 / beta) * u[i] - p[i] / beta;
             ps_apply(u, Sb, mb, px, space_mode);
@@ -503,7 +535,7 @@ int orc_dm_textbook(double* psi, const double* A, const double* S, const uint8_t
             ps_apply(w, Sb, mb, px, space_mode);
             for (int64_t i = 0; i < 2 * px; ++i)
                 w[i] = (1.0 - 1.0 / beta) * w[i] + p[i] / beta;
-            pm_one(w, Ab, M, N, 1);
+            pm_one(w, Ab, M, N, 1, 1);
             for (int64_t i = 0; i < 2 * px; ++i) p[i] += beta * (u[i] - w[i]);
         }
         free(u);
@@ -520,7 +552,7 @@ static void hio_one(double* x, double* xp, const double* A, const uint8_t* s, in
 {
     const int64_t px = M * N;
     memcpy(xp, x, sizeof(double) * 2 * (size_t)px);
-    pm_one(xp, A, M, N, skip_zero);
+    pm_one(xp, A, M, N, 1, skip_zero);
     for (int64_t i = 0; i < px; ++i) {
         const double pr = xp[2 * i], pi = xp[2 * i + 1];
         const int ok = s[i] && (!real_positive || pr >= 0.0);
@@ -775,7 +807,7 @@ int orc_amp_grad(const double* x, const double* A, int64_t M, int64_t N, int64_t
         const double* xb = x + 2 * b * px;
         double* g = grad + 2 * b * px;
         memcpy(g, xb, sizeof(double) * 2 * (size_t)px);
-        pm_one(g, A + (b % amp_batch) * px, M, N, 0);
+        pm_one(g, A + (b % amp_batch) * px, M, N, 1, 0);
         for (int64_t i = 0; i < 2 * px; ++i) g[i] = 2.0 * (double)px * (xb[i] - g[i]);
     }
     return 1;

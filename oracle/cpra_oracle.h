@@ -99,6 +99,14 @@ int orc_dm_ref(double* psi, double* t1, double* t2, const double* A, const doubl
  * psi <- P_S(-psi/beta + (1+1/beta) P_M psi). */
 int orc_final_projection(double* psi, const double* A, const double* S, int64_t M,
                          int64_t N, int64_t B, int64_t amp_batch, double beta);
+/* src/CONV_MKL.cpp:47-63: the same difference-map composition on L x M x N
+ * volumes (Forward3D/Backward3D), t1 and t2 carried; and the final
+ * projection of src/CONV_MKL.cpp:78-88. */
+int orc_dm_ref3d(double* psi, double* t1, double* t2, const double* A, const double* S,
+                 int64_t M, int64_t N, int64_t L, int64_t B, int64_t amp_batch, double beta,
+                 int64_t iters);
+int orc_final_projection3d(double* psi, const double* A, const double* S, int64_t M,
+                           int64_t N, int64_t L, int64_t B, int64_t amp_batch, double beta);
 
 /* Textbook difference map (Elser 2003) with gamma_S = -1/beta,
  * gamma_M = 1/beta:  psi <- psi + beta*[P_S((1+1/beta)P_M psi - psi/beta)

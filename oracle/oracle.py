@@ -81,6 +81,8 @@ def _declare(L):
         "orc_merge_add_f64": [_P, _P, C.c_double, C.c_double, _U64],
         "orc_dm_ref": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, C.c_double, _I64],
         "orc_final_projection": [_P, _P, _P, _I64, _I64, _I64, _I64, C.c_double],
+        "orc_dm_ref3d": [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, C.c_double, _I64],
+        "orc_final_projection3d": [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, C.c_double],
         "orc_dm_textbook": [_P, _P, _P, _P, _I64, _I64, _I64, _I64, C.c_double, C.c_int, _I64],
         "orc_hio": [_P, _P, _P, _I64, _I64, _I64, _I64, C.POINTER(HioParams)],
         "orc_hio_step": [_P, _P, _P, _I64, _I64, _I64, _I64, C.c_double, C.c_int, C.c_int],
@@ -239,6 +241,51 @@ def final_projection(psi, A, S, beta):
                                     beta)
     assert ok
     return psi
+
+
+def dm_ref3d(psi, t1, t2, A, S, beta, iters):
+    """orc_dm_ref3d on volumes [B][L][M][N]; A, S: [amp_batch][L][M][N]."""
+    psi, t1, t2 = _c128(psi).copy(), _c128(t1).copy(), _c128(t2).copy()
+    A, S = _f64(A), _f64(S)
+    L, M, N = psi.shape[-3:]
+    vx = L * M * N
+    ok = lib().orc_dm_ref3d(_ptr(psi), _ptr(t1), _ptr(t2), _ptr(A), _ptr(S), M, N, L,
+                            psi.size // vx, A.size // vx, beta, iters)
+    assert ok
+    return psi, t1, t2
+
+
+def final_projection3d(psi, A, S, beta):
+    psi = _c128(psi).copy()
+    A, S = _f64(A), _f64(S)
+    L, M, N = psi.shape[-3:]
+    vx = L * M * N
+    ok = lib().orc_final_projection3d(_ptr(psi), _ptr(A), _ptr(S), M, N, L, psi.size // vx,
+                                      A.size // vx, beta)
+    assert ok
+    return psi
+
+
+def conv_driver(A, S, init, iters, beta):
+    """FP64 restatement of one batch of the reference's conventional 3D
+    reconstruction (src/CONV_MKL.cpp:11-101).  A, S: [L][M][N] data and space
+    constraints (broadcast over the batch); init: psi [n][L][M][N].  `iters`
+    difference-map iterations with t1 = t2 = psi at the start (:33-34) and
+    carried after (:47-63); the convergence error ||F3 psi_before - F3
+    psi_after|| / ||F3 psi_after|| of the last iteration (:40-45, 65-74), over
+    the whole volume; the final projection (:78-88).  Returns (psi, err[n])."""
+    psi = _c128(init).copy()
+    n = psi.shape[0]
+    L, M, N = psi.shape[-3:]
+    t1, t2 = psi.copy(), psi.copy()
+    err = np.zeros(n)
+    if iters >= 1:
+        psi, t1, t2 = dm_ref3d(psi, t1, t2, A, S, beta, iters - 1)
+        old = psi.copy()
+        psi, t1, t2 = dm_ref3d(psi, t1, t2, A, S, beta, 1)
+        err, _ = converge_error(forward3d(old), forward3d(psi), M, N, L, n)
+    psi = final_projection3d(psi, A, S, beta)
+    return psi, err
 
 
 def dm_textbook(psi, A, beta, iters, S=None, mask=None, space_mode=1):

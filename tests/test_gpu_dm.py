@@ -101,3 +101,105 @@ def test_dm_argument_checks(hipmod, gpu):
         with pytest.raises(hipmod.CpraHipError):
             h.dm_run(z, hipmod.zeros((2, 64, 64), np.float32),
                      hipmod.zeros((2, 64, 64), np.uint8), iters=1, textbook=1, beta=0.0)
+
+
+def volume_problem(L, M, N, ab, seed):
+    """Random real objects inside a centred ellipsoid, their centred 3D
+    Fourier amplitudes, and the reference's float space constraint (1 inside
+    the support, with a few voxels at 0.9f and just below it)."""
+    rng = np.random.default_rng(seed)
+    zz, yy, xx = np.mgrid[0:L, 0:M, 0:N]
+    inside = (((zz - L / 2) / (0.3 * L)) ** 2 + ((yy - M / 2) / (0.3 * M)) ** 2
+              + ((xx - N / 2) / (0.3 * N)) ** 2) <= 1.0
+    A, S = [], []
+    for _ in range(ab):
+        obj = rng.uniform(0.2, 1.0, (L, M, N)) * inside
+        ax = (0, 1, 2)
+        F = np.fft.fftshift(np.fft.fftn(np.fft.fftshift(obj, axes=ax)), axes=ax)
+        A.append(np.abs(F).astype(np.float32))
+        s = inside.astype(np.float32)
+        edge = rng.random((L, M, N)) < 0.01
+        s[edge] = np.float32(0.9)
+        s[rng.random((L, M, N)) < 0.01] = np.nextafter(np.float32(0.9), np.float32(0))
+        S.append(s)
+    return np.stack(A), np.stack(S)
+
+
+@pytest.mark.parametrize("L,M,N,B,ab", [(64, 64, 64, 1, 1), (64, 64, 64, 2, 1),
+                                        (64, 128, 192, 2, 2), (128, 64, 64, 1, 1)])
+@pytest.mark.parametrize("iters", [1, 3])
+def test_dm3d_reference_variant_matches_oracle(hipmod, oracle, gpu, L, M, N, B, ab, iters):
+    """The 3D difference map of src/CONV_MKL.cpp:47-63 (t1, t2 carried) on the
+    fused z/y/x-pass engine against the FP64 oracle's composition of
+    Forward3D, DataConstraint, Backward3D, MergeAddData and SpaceConstraint."""
+    A, S = volume_problem(L, M, N, ab, seed=L + M + N + B)
+    rng = np.random.default_rng(7)
+    shape = (B, L, M, N)
+    mk = lambda: (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)  # noqa: E731
+    psi0 = mk()
+    t10, t20 = psi0.copy(), mk()
+    psi, t1, t2 = (hipmod.to_device(a) for a in (psi0, t10, t20))
+    vox = L * M * N
+    with hipmod.HipImpl(M, N, L, B) as h:
+        m1 = hipmod.zeros((ab, L, M, N), np.uint8)
+        assert h.support_from_constraint(hipmod.to_device(S), m1, ab * vox)
+        full = hipmod.to_device(np.stack([m1.numpy()[b % ab] for b in range(B)]))
+        assert h.dm3d_run(psi, hipmod.to_device(A), full, t1=t1, t2=t2, iters=iters, beta=0.9,
+                          textbook=0, space_mode=0, use_graph=1)
+        h.Sync()
+    rp, r1, r2 = oracle.dm_ref3d(psi0, t10, t20, A, S, 0.9, iters)
+    tol = 2e-5 * iters
+    assert relerr(psi.numpy(), rp) < tol
+    assert relerr(t1.numpy(), r1) < tol
+    assert relerr(t2.numpy(), r2) < tol
+
+
+def test_dm3d_graph_matches_eager_and_rejects_bad_sizes(hipmod, gpu):
+    """Graph replay of the captured iteration gives the eager result bit for
+    bit; volumes outside the fast sizes are refused before any launch."""
+    L = M = N = 64
+    A, S = volume_problem(L, M, N, 1, seed=3)
+    rng = np.random.default_rng(8)
+    psi0 = (rng.standard_normal((1, L, M, N)) + 1j * rng.standard_normal((1, L, M, N))).astype(np.complex64)
+    outs = []
+    for g in (0, 1):
+        psi, t1, t2 = (hipmod.to_device(psi0) for _ in range(3))
+        with hipmod.HipImpl(M, N, L, 1) as h:
+            mask = hipmod.zeros((1, L, M, N), np.uint8)
+            assert h.support_from_constraint(hipmod.to_device(S), mask, L * M * N)
+            assert h.dm3d_run(psi, hipmod.to_device(A), mask, t1=t1, t2=t2, iters=5, beta=0.9,
+                              textbook=0, space_mode=0, use_graph=g)
+            h.Sync()
+        outs.append((psi.numpy(), t1.numpy(), t2.numpy()))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    with hipmod.HipImpl(80, 64, 64, 1) as h:
+        z = hipmod.zeros((1, 64, 80, 64), np.complex64)
+        with pytest.raises(hipmod.CpraHipError) as e:
+            h.dm3d_run(z, hipmod.zeros((1, 64, 80, 64), np.float32),
+                       hipmod.zeros((1, 64, 80, 64), np.uint8), t1=z, t2=z, iters=1)
+        assert e.value.code == hipmod.ERR_UNSUPPORTED
+
+
+def test_dm_run_ignores_handle_depth(hipmod, gpu, synth):
+    """cpra_hip_dm_run stays 2D on a handle that carries a volume depth l (the
+    reference's drivers build their 2D handles with L): same bits as on an
+    l = 1 handle, and nothing written past the 2D batch."""
+    n, B = 64, 2
+    data = synth.make_2d_batch(B, n, 8, 12, 1e6, config=170)
+    mask = (data["obj"] > 0).astype(np.uint8)
+    outs = []
+    for l in (1, 64):
+        pad = np.full((3, n, n), 7 + 7j, np.complex64)  # guard region after the batch
+        psi = hipmod.to_device(np.concatenate([data["x0"], pad]))
+        t1 = hipmod.to_device(np.concatenate([data["x0"], pad]))
+        t2 = hipmod.to_device(np.concatenate([data["x0"], pad]))
+        with hipmod.HipImpl(n, n, l, B) as h:
+            assert h.dm_run(psi, hipmod.to_device(data["amp"]), hipmod.to_device(mask), t1=t1,
+                            t2=t2, iters=3, beta=0.9, textbook=0, space_mode=1)
+            h.Sync()
+        outs.append([a.numpy() for a in (psi, t1, t2)])
+        for a in outs[-1]:
+            assert np.array_equal(a[B:], pad)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)

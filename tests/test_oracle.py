@@ -235,6 +235,79 @@ def test_final_projection_fixed_point(oracle):
     assert np.abs(p[0] - obj).max() < 1e-10
 
 
+def cfft3(x):
+    ax = (-3, -2, -1)
+    return np.fft.fftshift(np.fft.fftn(np.fft.fftshift(x, axes=ax), axes=ax), axes=ax)
+
+
+def cifft3(x):
+    ax = (-3, -2, -1)
+    return np.fft.fftshift(np.fft.ifftn(np.fft.fftshift(x, axes=ax), axes=ax), axes=ax)
+
+
+def exact_problem_3d(L=8, M=10, N=12, seed=0, r=2.5):
+    rng = np.random.default_rng(seed)
+    zz, yy, xx = np.mgrid[0:L, 0:M, 0:N]
+    sup = ((zz - L / 2) ** 2 + (yy - M / 2) ** 2 + (xx - N / 2) ** 2 <= r * r).astype(np.uint8)
+    obj = rng.uniform(0.1, 1.0, (L, M, N)) * sup
+    return obj, np.abs(cfft3(obj)), sup
+
+
+def pm3(z, A):
+    f = cfft3(z)
+    m = np.abs(f)
+    out = np.where((m > 0) & (A != 0), A * f / np.where(m > 0, m, 1), f)
+    return cifft3(out)
+
+
+def test_dm_ref3d_fixed_point(oracle):
+    obj, A, sup = exact_problem_3d()
+    psi = obj[None].astype(complex)
+    p, t1, t2 = oracle.dm_ref3d(psi, psi, psi, A[None], sup[None].astype(float), 0.9, 3)
+    assert np.abs(p - psi).max() < 1e-10
+    assert np.abs(t1 - psi).max() < 1e-10 and np.abs(t2 - psi).max() < 1e-10
+
+
+def test_dm_ref3d_first_step_closed_form_beta1(oracle):
+    """beta = 1: t1 = P_S(2 P_M psi - psi), t2 = P_M psi, psi + t1 - t2, with
+    P_M through numpy's n-dimensional FFT (src/CONV_MKL.cpp:47-63)."""
+    obj, A, sup = exact_problem_3d(seed=1)
+    psi = rnd((8, 10, 12), 21)
+    p, t1, t2 = oracle.dm_ref3d(psi[None], psi[None], psi[None], A[None],
+                                sup[None].astype(float), 1.0, 1)
+    PS = lambda z: np.where(sup > 0, z.real, 0.0) + 0j  # noqa: E731
+    e_t1 = PS(2 * pm3(psi, A) - psi)
+    e_t2 = pm3(psi, A)
+    assert np.allclose(t1[0], e_t1, atol=1e-10)
+    assert np.allclose(t2[0], e_t2, atol=1e-10)
+    assert np.allclose(p[0], psi + e_t1 - e_t2, atol=1e-10)
+
+
+def test_dm_ref3d_carries_temporaries_and_batches(oracle):
+    """Two iterations differ from two separate one-iteration runs restarted at
+    t1 = t2 = psi (the temporaries carry over), and a batch of two volumes
+    with one shared constraint equals the members run alone."""
+    obj, A, sup = exact_problem_3d(seed=2)
+    S = sup[None].astype(float)
+    psi = rnd((2, 8, 10, 12), 22)
+    p2, _, _ = oracle.dm_ref3d(psi, psi, psi, A[None], S, 0.9, 2)
+    q, _, _ = oracle.dm_ref3d(psi, psi, psi, A[None], S, 0.9, 1)
+    q2, _, _ = oracle.dm_ref3d(q, q, q, A[None], S, 0.9, 1)
+    assert np.abs(p2 - q2).max() > 1e-6
+    for b in range(2):
+        pb, _, _ = oracle.dm_ref3d(psi[b:b + 1], psi[b:b + 1], psi[b:b + 1], A[None], S, 0.9, 2)
+        assert np.allclose(pb[0], p2[b], atol=1e-12)
+
+
+def test_conv_driver_fixed_point(oracle):
+    """At an exact solution the conventional driver (src/CONV_MKL.cpp:11-101)
+    returns it with zero convergence error."""
+    obj, A, sup = exact_problem_3d(seed=3)
+    p, err = oracle.conv_driver(A, sup.astype(float), obj[None].astype(complex), 4, 0.9)
+    assert np.abs(p[0] - obj).max() < 1e-10
+    assert err[0] < 1e-10
+
+
 def test_dm_textbook_fixed_point_real_positive(oracle):
     obj, A, sup = exact_problem(seed=4)
     p = oracle.dm_textbook(obj[None].astype(complex), A[None], 0.8, 4, mask=sup[None],
