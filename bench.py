@@ -356,6 +356,164 @@ def bench_gd3d(hip, synth, ge, n=256, steps=500, warmup=10, timing_iters=20, fp6
     return rec
 
 
+def bench_conv3d(hip, synth, ge, n=512, iters=20, warmup=3, timing_iters=3, prim_iters=3,
+                 fp64=True, check_n=128):
+    """The reference's conventional 3D reconstruction at its benchmark scale
+    (src/CONV_MKL.cpp:47-63 at M = N = L = 512, src/benchmark_mkl.cpp:7-13):
+    difference-map iterations (t1, t2 carried) on one 512^3 complex64 volume,
+    through the fused engine (cpra_hip_dm3d_run, graph-replayed) and, for
+    comparison, through the reference's own sequence of primitive calls.
+    Inputs are uniform random, as in the reference benchmark (Initialize); the
+    support is the reference SpaceConstraint mask of a uniform float
+    constraint (S >= 0.9)."""
+    import numpy as np
+
+    vox = n ** 3
+    c64, f32, u8 = np.complex64, np.float32, np.uint8
+    psi, t1, t2 = (hip.DeviceArray((1, n, n, n), c64) for _ in range(3))
+    A = hip.DeviceArray((1, n, n, n), f32)
+    S = hip.DeviceArray((1, n, n, n), f32)
+    mask = hip.DeviceArray((1, n, n, n), u8)
+    h = hip.HipImpl(n, n, n, 1)
+    for i, (arr, num) in enumerate(((psi, 2 * vox), (A, vox), (S, vox))):
+        h.Initialize(arr, num, seed=5000 + i)
+    h.support_from_constraint(S, mask, vox)
+    h.Memcpy(t1, psi, 8 * vox)
+    h.Memcpy(t2, psi, 8 * vox)
+    kw = dict(beta=0.9, textbook=0, space_mode=0, dc_skip_zero=1)
+    h.dm3d_run(psi, A, mask, t1=t1, t2=t2, iters=warmup, use_graph=1, **kw)
+    h.Sync()
+    t0 = time.perf_counter()
+    h.dm3d_run(psi, A, mask, t1=t1, t2=t2, iters=iters, use_graph=1, **kw)
+    h.Sync()
+    el = time.perf_counter() - t0
+    its = iters / el
+    log(f"bench[conv3d]: {iters} iterations at {n}^3 in {el * 1e3:.1f} ms -> {its:.1f} it/s")
+    h.set_kernel_timing(True)
+    h.dm3d_run(psi, A, mask, t1=t1, t2=t2, iters=timing_iters, use_graph=0, **kw)
+    h.Sync()
+    kt = h.kernel_times()
+    h.set_kernel_timing(False)
+    # algorithmic bytes per launch kind and iteration: z lines over [Y1; Y2]
+    # (read 8 + amplitude 4 + write 8, twice), two y-line passes over
+    # [Y1; Y2] (read + write 16, twice), x rows (read Z1, Z2, psi, support;
+    # write psi, Y1, Y2: 49)
+    spec = {"column": (vox * 40, 1), "pencil": (vox * 64, 2), "row": (vox * 49, 1)}
+    kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}
+    dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
+    # the reference's call sequence (12 primitive calls per iteration)
+    t0 = time.perf_counter()
+    for _ in range(prim_iters):
+        h.Forward3D(t1)
+        h.DataConstraint(t1, A, vox, 1)
+        h.Backward3D(t1)
+        h.MergeAddData(psi, t1, -1 / 0.9, 1 + 1 / 0.9, vox)
+        h.SpaceConstraint(t1, S, vox, 1)
+        h.SpaceConstraint(t2, S, vox, 1)
+        h.MergeAddData(psi, t2, 1 / 0.9, 1 - 1 / 0.9, vox)
+        h.Forward3D(t2)
+        h.DataConstraint(t2, A, vox, 1)
+        h.Backward3D(t2)
+        h.MergeAddData(t1, psi, 0.9, 1.0, vox)
+        h.MergeAddData(t2, psi, -0.9, 1.0, vox)
+    h.Sync()
+    prim_its = prim_iters / (time.perf_counter() - t0)
+    log(f"bench[conv3d]: primitive sequence {prim_its:.1f} it/s")
+    for a in (psi, t1, t2, A, S, mask):
+        a.free()
+    h.close()
+    rec = {
+        "metric": f"conventional 3D difference-map iters/s ({n}^3 complex64)",
+        "value": round(its, 2), "unit": "iters/s", "steps": iters, "warmup": warmup,
+        "ms_per_step": round(1e3 * el / iters, 4),
+        "config": {"workload": f"difference map (reference variant, beta 0.9) on one {n}^3 complex64 "
+                               "volume, uniform random inputs as in the reference benchmark",
+                   "size": n, "graph": True},
+        "primitive_sequence_iters_per_s": round(prim_its, 2),
+        "speedup_vs_primitive_sequence": round(its / prim_its, 2),
+        "hbm_frac_algorithmic_53Bpx": round(vox * 53 * its / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern[dom]["frac"],
+                     "kernels": kern},
+    }
+    if fp64:
+        # one iteration from the same random state at check_n^3 against the
+        # FP64 oracle (orc_dm_ref3d)
+        o = ge.load_oracle()
+        m = check_n
+        rng = np.random.default_rng(6)
+        shp = (1, m, m, m)
+        p0 = (rng.random(shp) + 1j * rng.random(shp)).astype(c64)
+        q0 = (rng.random(shp) + 1j * rng.random(shp)).astype(c64)
+        A0 = rng.random(shp).astype(f32)
+        S0 = rng.random(shp).astype(f32)
+        hs = hip.HipImpl(m, m, m, 1)
+        dp, d1, d2 = hip.to_device(p0), hip.to_device(p0), hip.to_device(q0)
+        mk = hip.zeros(shp, u8)
+        hs.support_from_constraint(hip.to_device(S0), mk, m ** 3)
+        hs.dm3d_run(dp, hip.to_device(A0), mk, t1=d1, t2=d2, iters=1, use_graph=0, **kw)
+        hs.Sync()
+        rp, r1, r2 = o.dm_ref3d(p0, p0, q0, A0, S0, 0.9, 1)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+        rec["fp64_check"] = {"size": m, "iters": 1, "psi_rel_l2": rel(dp.numpy(), rp),
+                             "t1_rel_l2": rel(d1.numpy(), r1), "t2_rel_l2": rel(d2.numpy(), r2)}
+        hs.close()
+        # CPU baseline: the oracle's 3D map at check_n^3 for about 10 s,
+        # scaled by the voxel count to n^3
+        k, t0 = 0, time.perf_counter()
+        p, q1, q2 = p0, p0, q0
+        while time.perf_counter() - t0 < 10.0:
+            p, q1, q2 = o.dm_ref3d(p, q1, q2, A0, S0, 0.9, 1)
+            k += 1
+        dt = time.perf_counter() - t0
+        rec["cpu_baseline"] = {
+            "value": k / dt * (m ** 3) / vox, "unit": "iters/s", "cores": o.num_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle 3D difference map, {k} iterations at {m}^3 in {dt:.1f} s, scaled "
+                      f"by the voxel ratio to {n}^3 (FFT cost grows as V log V, so this "
+                      "overstates the CPU rate by about a quarter)"}
+    return rec
+
+
+def bench_ref_benchmark(hip, synth, ge, epi=10, iters=10, fp64=True):
+    """The reference's own benchmark program (src/benchmark_mkl.cpp) restated
+    over HipImpl (bin/cpra_benchmark), run as its README documents it
+    ("benchmark 10 10": 10 episodes of 10 iterations at M = N = L = 512,
+    P = 288), with the fused engine and with the reference's primitive
+    sequence, plus the conventional mode with 10 iterations.  Value: the
+    program's own headline, the estimated time for 100 initial guesses."""
+    import re
+    import subprocess
+
+    exe = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_benchmark")
+
+    def run(*args):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpra_benchmark {' '.join(args)} failed: {r.stderr[-400:]}")
+        t = [float(x) for x in re.findall(r"^elapsed time: ([0-9.eE+-]+)s", r.stdout, re.M)]
+        c = re.search(r"Estimated step C elapsed time: ([0-9.eE+-]+)s", r.stdout)
+        tot = re.search(r"in total: ([0-9.eE+-]+)s", r.stdout)
+        return {"step_A_s": t[0], "step_B_s": t[1], "step_D_s": t[2],
+                "step_C_estimate_s": float(c.group(1)), "total_estimate_s": float(tot.group(1))}
+
+    fused = run(str(epi), str(iters))
+    log(f"bench[ref_benchmark]: fused {fused}")
+    prim = run("--primitives", str(epi), str(iters))
+    log(f"bench[ref_benchmark]: primitives {prim}")
+    r = subprocess.run([exe, str(iters)], capture_output=True, text=True, timeout=900)
+    conv = re.search(r"^elapsed time: ([0-9.eE+-]+)s", r.stdout, re.M)
+    return {
+        "metric": "reference benchmark: estimated time for 100 initial guesses (s)",
+        "value": fused["total_estimate_s"], "unit": "s", "higher_is_better": False,
+        "config": {"workload": f"bin/cpra_benchmark {epi} {iters}: M = N = L = 512, P = 288, "
+                               "1000 pre-iterations, one 512^3 3D iteration (src/benchmark_mkl.cpp)"},
+        "fused_engine": fused,
+        "primitive_sequence": prim,
+        "conventional_mode_s": float(conv.group(1)) if conv else None,
+    }
+
+
 def bench_mixed(hip, synth, ge, sizes=(128, 192, 256, 384), per_class=256, iters=300, warmup=5,
                 timing_iters=5, pattern_iters=3, fp64=True):
     """Mixed-size batch: 256 patterns each at 128^2, 192^2, 256^2 and 384^2
@@ -439,7 +597,7 @@ def bench_mixed(hip, synth, ge, sizes=(128, 192, 256, 384), per_class=256, iters
     return rec
 
 
-WORKLOADS = ("hio2d", "carousel3d", "gd2d", "gd3d", "mixed")
+WORKLOADS = ("hio2d", "carousel3d", "gd2d", "gd3d", "mixed", "conv3d", "ref_benchmark")
 
 
 def main():
@@ -484,10 +642,11 @@ def main():
     fp64 = not args.no_fp64_check
     if args.workload != "hio2d":
         fn = {"carousel3d": bench_carousel, "gd2d": bench_gd2d, "gd3d": bench_gd3d,
-              "mixed": bench_mixed}[args.workload]
+              "mixed": bench_mixed, "conv3d": bench_conv3d,
+              "ref_benchmark": bench_ref_benchmark}[args.workload]
         rec = fn(hip, synth, ge, fp64=fp64)
-        rec.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-                    "dtype": "c64", "data": "synthetic"})
+        rec = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "c64", "data": "synthetic", **rec}
         print(json.dumps(rec), flush=True)
         return
     n, P = args.size, args.patterns
@@ -636,7 +795,8 @@ def main():
     if not args.no_extra:
         result["workloads"] = {}
         for name, fn in (("carousel3d", bench_carousel), ("gd2d", bench_gd2d), ("gd3d", bench_gd3d),
-                         ("mixed", bench_mixed)):
+                         ("mixed", bench_mixed), ("conv3d", bench_conv3d),
+                         ("ref_benchmark", bench_ref_benchmark)):
             log(f"bench: workload {name}")
             result["workloads"][name] = fn(hip, synth, ge, fp64=fp64)
 

@@ -75,7 +75,9 @@ def build_cpp_checks() -> list:
     return out
 
 
-DRIVERS = {"cpra_carousel": os.path.join(HERE, "drivers", "cpra_carousel.cpp")}
+DRIVERS = {"cpra_carousel": os.path.join(HERE, "drivers", "cpra_carousel.cpp"),
+           "cpra_conv": os.path.join(HERE, "drivers", "cpra_conv.cpp"),
+           "cpra_benchmark": os.path.join(HERE, "drivers", "cpra_benchmark.cpp")}
 DRIVER_BIN = os.path.join(HERE, "bin")
 
 

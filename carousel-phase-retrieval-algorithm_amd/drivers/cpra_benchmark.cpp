@@ -1,0 +1,335 @@
+// cpra_benchmark.cpp -- the reference's benchmark program
+// (src/benchmark_mkl.cpp:1-282, identical in steps to src/benchmark_cuda.cu)
+// on the HIP backend, with the same two modes and the same printed lines:
+//
+//   cpra_benchmark EPI ITER   CPRA benchmark (ShrinkWrap_CPRA_MKL_Sample,
+//                             :15-188) at M = N = L = 512, P = 9M/16 = 288,
+//                             batch 1:
+//       Step A  1000 difference-map iterations on projection 0 (:40-70);
+//       Step B  EPI sequential sweeps over the P projections: warm start from
+//               the neighbours (:77-90), t1 = t2 = psi_p, ITER iterations
+//               (:72-125);
+//       Step C  the estimate (A + B) * 100 / batch for 100 initial guesses
+//               (:127-160);
+//       Step D  one 3D iteration on the 512^3 volume: Forward3D,
+//               DataConstraint with a complex constraint, Backward3D,
+//               SpaceConstraint (:147-176);
+//     and the total estimate (:178-180).
+//   cpra_benchmark ITER       conventional benchmark
+//                             (ShrinkWrap_Conventional_MKL_Sample, :190-252):
+//                             ITER iterations of the reference's call
+//                             sequence on the M*N*L arrays (its Forward2D and
+//                             Backward2D transform the first M x N plane, as
+//                             the batch-1 plan does there), and the x100
+//                             estimate.
+//
+// Inputs are uniform random (Initialize), as in the reference.  The
+// difference-map iterations of Steps A and B run through the fused engine
+// cpra_hip_dm_run; --primitives runs the reference's sequence of eleven
+// primitive calls per iteration instead (:47-62).  Additions: --size M
+// (default 512; M = N = L, P = 9M/16) for short runs, --seed S (the
+// reference draws from unseeded generators), --primitives.
+#include <chrono>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "HIP_Impl.hpp"
+#include "cpra_hip_engine.h"
+
+using namespace CPRA;
+using cf = std::complex<float>;
+
+namespace {
+
+struct Config {
+    uint64_t M = 512;
+    bool primitives = false;
+    unsigned long long seed = 0;
+};
+
+constexpr float BETA = 0.9f;
+
+void print_finished(const char* what, std::chrono::system_clock::time_point t,
+                    std::chrono::duration<double> el)
+{
+    const std::time_t tt = std::chrono::system_clock::to_time_t(t);
+    std::cout << "Finished " << what << " at " << std::ctime(&tt) << "elapsed time: " << el.count()
+              << "s\n";
+}
+
+void chk(bool ok, CpraImpl<float>* impl, const char* what)
+{
+    if (!ok) {
+        auto* hip = static_cast<HipImpl<float>*>(impl);
+        throw std::runtime_error(std::string(what) + ": " + hip->last_error());
+    }
+}
+
+// The reference's difference-map iteration (src/benchmark_mkl.cpp:47-62,
+// 99-115) on one M x N pattern: cur (psi), t1, t2, constraints A, S.
+void dm_primitives(CpraImpl<float>* c, cf* cur, cf* t1, cf* t2, float* A, float* S, uint64_t px,
+                   int iters)
+{
+    for (int i = 0; i < iters; ++i) {
+        chk(c->Forward2D(t1), c, "Forward2D");
+        chk(c->DataConstraint(t1, A, px, 1), c, "DataConstraint");
+        chk(c->Backward2D(t1), c, "Backward2D");
+        chk(c->MergeAddData(cur, t1, -1.f / BETA, 1.f + 1.f / BETA, px), c, "MergeAddData");
+        chk(c->SpaceConstraint(t1, S, px, 1), c, "SpaceConstraint");
+        chk(c->SpaceConstraint(t2, S, px, 1), c, "SpaceConstraint");
+        chk(c->MergeAddData(cur, t2, 1.f / BETA, 1.f - 1.f / BETA, px), c, "MergeAddData");
+        chk(c->Forward2D(t2), c, "Forward2D");
+        chk(c->DataConstraint(t2, A, px, 1), c, "DataConstraint");
+        chk(c->Backward2D(t2), c, "Backward2D");
+        chk(c->MergeAddData(t1, cur, BETA, 1.f, px), c, "MergeAddData");
+        chk(c->MergeAddData(t2, cur, -1.f * BETA, 1.f, px), c, "MergeAddData");
+    }
+}
+
+void cpra_sample(const Config& cfg, int epi, int iter)
+{
+    const uint64_t M = cfg.M, N = M, L = M, P = M * 9 / 16, px = M * N;
+    auto compute = NewHIPImpl<float>(M, N, L, 1);
+    auto* hip = static_cast<HipImpl<float>*>(compute.get());
+    cpra_hip_handle h = hip->handle();
+    auto alloc = NewHIPAllocator();
+    cf* rg = (cf*)alloc->allocate(sizeof(cf) * px * P);
+    cf* t1 = (cf*)alloc->allocate(sizeof(cf) * px);
+    cf* t2 = (cf*)alloc->allocate(sizeof(cf) * px);
+    float* dataC = (float*)alloc->allocate(sizeof(float) * px * P);
+    float* spaceC = (float*)alloc->allocate(sizeof(float) * px * P);
+    uint8_t* mask = (uint8_t*)alloc->allocate(px * P);
+    uint64_t seed = cfg.seed;
+    auto init = [&](float* p, uint64_t num) {
+        if (cpra_hip_initialize(h, p, num, seed++) < 0)
+            throw std::runtime_error(std::string("Initialize: ") + cpra_hip_last_error(h));
+    };
+    init(reinterpret_cast<float*>(rg), 2 * px * P);
+    init(reinterpret_cast<float*>(t1), 2 * px);
+    init(reinterpret_cast<float*>(t2), 2 * px);
+    init(dataC, px * P);
+    init(spaceC, px * P);
+    // the fused engine takes the support as the u8 mask of the float constraint
+    if (cpra_hip_support_from_constraint(h, spaceC, mask, px * P) < 0)
+        throw std::runtime_error(std::string("support mask: ") + cpra_hip_last_error(h));
+    chk(compute->Sync(), compute.get(), "Sync");
+
+    cpra_hip_dm_params dp;
+    cpra_hip_dm_default_params(&dp);
+    dp.beta = BETA;
+    dp.textbook = 0;
+    dp.space_mode = CPRA_HIP_DM_SPACE_MKL;
+    dp.dc_skip_zero = 1;
+    dp.use_graph = 1;
+    bool fused = !cfg.primitives;
+    auto dm = [&](uint64_t p, int iters) {
+        if (iters <= 0) return;
+        if (fused) {
+            dp.iters = iters;
+            dp.use_graph = iters > 16;  // short runs: eager launches beat a graph build
+            const int rc = cpra_hip_dm_run(h, rg + px * p, t1, t2, dataC + px * p, 1, mask + px * p, &dp);
+            if (rc == CPRA_HIP_ERR_UNSUPPORTED) {
+                fused = false;
+            } else {
+                if (rc < 0) throw std::runtime_error(std::string("difference map: ") + cpra_hip_last_error(h));
+                return;
+            }
+        }
+        dm_primitives(compute.get(), rg + px * p, t1, t2, dataC + px * p, spaceC + px * p, px, iters);
+    };
+
+    const auto start = std::chrono::system_clock::now();
+    // Step A (:40-70)
+    chk(compute->Memcpy(t1, rg, sizeof(cf) * px), compute.get(), "Memcpy");
+    chk(compute->Memcpy(t2, rg, sizeof(cf) * px), compute.get(), "Memcpy");
+    dm(0, 1000);
+    chk(compute->Sync(), compute.get(), "Sync");
+    const auto endA = std::chrono::system_clock::now();
+    print_finished("step A computation", endA, endA - start);
+
+    // Step B (:72-125)
+    for (int e = 0; e < epi; ++e) {
+        for (uint64_t p = 0; p < P; ++p) {
+            cf* cur = rg + px * p;
+            if (p == P - 1) {
+                chk(compute->MergeAddData(rg + px * (p - 1), cur, 0.5f, 0.5f, px), compute.get(), "MergeAddData");
+            } else if (p == 0) {
+                chk(compute->MergeAddData(rg + px * (p + 1), cur, 0.5f, 0.5f, px), compute.get(), "MergeAddData");
+            } else {
+                chk(compute->MergeAddData(rg + px * (p - 1), cur, 0.5f, 0.f, px), compute.get(), "MergeAddData");
+                chk(compute->MergeAddData(rg + px * (p + 1), cur, 0.5f, 1.f, px), compute.get(), "MergeAddData");
+            }
+            chk(compute->Memcpy(t1, cur, sizeof(cf) * px), compute.get(), "Memcpy");
+            chk(compute->Memcpy(t2, cur, sizeof(cf) * px), compute.get(), "Memcpy");
+            dm(p, iter);
+        }
+    }
+    chk(compute->Sync(), compute.get(), "Sync");
+    const auto endB = std::chrono::system_clock::now();
+    print_finished("step B computation", endB, endB - endA);
+
+    alloc->deallocate(rg);
+    alloc->deallocate(t1);
+    alloc->deallocate(t2);
+    alloc->deallocate(dataC);
+    alloc->deallocate(spaceC);
+    alloc->deallocate(mask);
+    compute.reset();
+
+    // Step D setup (:147-155)
+    auto c3 = NewHIPImpl<float>(M, N, L, 1);
+    cpra_hip_handle h3 = static_cast<HipImpl<float>*>(c3.get())->handle();
+    const uint64_t vox = M * N * L;
+    cf* rg3 = (cf*)alloc->allocate(sizeof(cf) * vox);
+    cf* cdc3 = (cf*)alloc->allocate(sizeof(cf) * vox);
+    float* sc3 = (float*)alloc->allocate(sizeof(float) * vox);
+    auto init3 = [&](float* p, uint64_t num) {
+        if (cpra_hip_initialize(h3, p, num, seed++) < 0)
+            throw std::runtime_error(std::string("Initialize: ") + cpra_hip_last_error(h3));
+    };
+    init3(reinterpret_cast<float*>(rg3), 2 * vox);
+    init3(reinterpret_cast<float*>(cdc3), 2 * vox);
+    init3(sc3, vox);
+    chk(c3->Sync(), c3.get(), "Sync");
+
+    const auto endC = std::chrono::system_clock::now();
+    std::chrono::duration<double> est = (endB - start) * 100.0 / 1.0;
+    std::cout << "Estimated step C elapsed time: " << est.count() << "s\n";
+
+    // Step D (:162-176)
+    chk(c3->Forward3D(rg3), c3.get(), "Forward3D");
+    chk(c3->DataConstraint(rg3, cdc3, vox, 1), c3.get(), "DataConstraint");
+    chk(c3->Backward3D(rg3), c3.get(), "Backward3D");
+    chk(c3->SpaceConstraint(rg3, sc3, vox, 1), c3.get(), "SpaceConstraint");
+    chk(c3->Sync(), c3.get(), "Sync");
+    const auto endD = std::chrono::system_clock::now();
+    print_finished("step D computation", endD, endD - endC);
+    const std::chrono::duration<double> total = (endD - endC) + (endB - start) * 100.0 / 1.0;
+    std::cout << "Estimated elapsed time with 100 initial guesses in total: " << total.count() << "s\n";
+    alloc->deallocate(rg3);
+    alloc->deallocate(cdc3);
+    alloc->deallocate(sc3);
+}
+
+void conventional_sample(const Config& cfg, int iter)
+{
+    const uint64_t M = cfg.M, N = M, L = M, vox = M * N * L;
+    auto compute = NewHIPImpl<float>(M, N, L, 1);
+    CpraImpl<float>* c = compute.get();
+    cpra_hip_handle h = static_cast<HipImpl<float>*>(c)->handle();
+    auto alloc = NewHIPAllocator();
+    cf* rg = (cf*)alloc->allocate(sizeof(cf) * vox);
+    cf* t1 = (cf*)alloc->allocate(sizeof(cf) * vox);
+    cf* t2 = (cf*)alloc->allocate(sizeof(cf) * vox);
+    float* dataC = (float*)alloc->allocate(sizeof(float) * vox);
+    float* spaceC = (float*)alloc->allocate(sizeof(float) * vox);
+    uint64_t seed = cfg.seed;
+    auto init = [&](float* p, uint64_t num) {
+        if (cpra_hip_initialize(h, p, num, seed++) < 0)
+            throw std::runtime_error(std::string("Initialize: ") + cpra_hip_last_error(h));
+    };
+    init(reinterpret_cast<float*>(rg), 2 * vox);
+    init(reinterpret_cast<float*>(t1), 2 * vox);
+    init(reinterpret_cast<float*>(t2), 2 * vox);
+    init(dataC, vox);
+    init(spaceC, vox);
+    chk(c->Sync(), c, "Sync");
+
+    const auto start = std::chrono::system_clock::now();
+    chk(c->Memcpy(t1, rg, sizeof(cf) * vox), c, "Memcpy");
+    chk(c->Memcpy(t2, rg, sizeof(cf) * vox), c, "Memcpy");
+    for (int i = 0; i < iter; ++i) {  // :215-234, call for call
+        chk(c->Forward2D(t1), c, "Forward2D");
+        chk(c->DataConstraint(t1, dataC, vox, 1), c, "DataConstraint");
+        chk(c->Backward2D(t1), c, "Backward2D");
+        chk(c->MergeAddData(t1, rg, 1.f + 1.f / BETA, -1.f / BETA, vox), c, "MergeAddData");
+        chk(c->SpaceConstraint(t1, spaceC, vox, 1), c, "SpaceConstraint");
+        chk(c->SpaceConstraint(t2, spaceC, vox, 1), c, "SpaceConstraint");
+        chk(c->MergeAddData(t2, rg, 1.f - 1.f / BETA, -1.f / BETA, vox), c, "MergeAddData");
+        chk(c->Forward2D(t2), c, "Forward2D");
+        chk(c->DataConstraint(t2, dataC, vox, 1), c, "DataConstraint");
+        chk(c->Backward2D(t2), c, "Backward2D");
+        chk(c->MergeAddData(t1, rg, BETA, 1.f, vox), c, "MergeAddData");
+        chk(c->MergeAddData(t2, rg, -1.f * BETA, 1.f, vox), c, "MergeAddData");
+    }
+    chk(c->Sync(), c, "Sync");
+    const auto end = std::chrono::system_clock::now();
+    const std::chrono::duration<double> el = end - start;
+    const std::time_t tt = std::chrono::system_clock::to_time_t(end);
+    std::cout << "Finished computation at " << std::ctime(&tt) << "elapsed time: " << el.count() << "s\n"
+              << "Estimated elapsed timewith 100 intial guesses in total : " << el.count() * 100 / 1
+              << std::endl;
+    alloc->deallocate(rg);
+    alloc->deallocate(t1);
+    alloc->deallocate(t2);
+    alloc->deallocate(dataC);
+    alloc->deallocate(spaceC);
+}
+
+[[noreturn]] void usage(const std::string& msg)
+{
+    std::cerr << "cpra_benchmark: " << msg << "\n"
+              << "usage: cpra_benchmark [--size M] [--seed S] [--primitives] EPI ITER   (CPRA benchmark)\n"
+              << "       cpra_benchmark [--size M] [--seed S] ITER                      (conventional benchmark)\n";
+    std::exit(2);
+}
+
+bool parse_int(const std::string& s, long long& v)
+{
+    char* end = nullptr;
+    v = std::strtoll(s.c_str(), &end, 10);
+    return !s.empty() && *end == '\0';
+}
+
+}  // namespace
+
+int main(int argc, char* argv[])
+{
+    Config cfg;
+    std::vector<std::string> pos;
+    for (int k = 1; k < argc; ++k) {
+        const std::string a = argv[k];
+        long long v = 0;
+        if (a == "--size" || a == "--seed") {
+            if (k + 1 >= argc || !parse_int(argv[k + 1], v) || v < 0) usage(a + " needs a non-negative integer");
+            if (a == "--size") cfg.M = (uint64_t)v;
+            else cfg.seed = (unsigned long long)v;
+            ++k;
+        } else if (a == "--primitives") {
+            cfg.primitives = true;
+        } else if (a == "-h" || a == "--help") {
+            usage("help");
+        } else {
+            pos.push_back(a);
+        }
+    }
+    if (cfg.M < 16 || cfg.M % 16 != 0) usage("--size must be a positive multiple of 16");
+    long long a0 = 0, a1 = 0;
+    try {
+        if (pos.size() == 2) {
+            if (!parse_int(pos[0], a0) || !parse_int(pos[1], a1) || a0 < 0 || a1 < 0)
+                usage("EPI and ITER must be non-negative integers");
+            std::cout << "Running CPRA benchmark, epi is " << a0 << " iter is " << a1 << std::endl;
+            cpra_sample(cfg, (int)a0, (int)a1);
+        } else if (pos.size() == 1) {
+            if (!parse_int(pos[0], a0) || a0 < 0) usage("ITER must be a non-negative integer");
+            std::cout << "Running Conventional benchmark, iter is " << a0 << std::endl;
+            conventional_sample(cfg, (int)a0);
+        } else {
+            std::cerr << "Wrong num of command line arg!" << std::endl;
+            return 2;
+        }
+    } catch (const std::exception& e) {
+        std::cerr << "cpra_benchmark: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

@@ -1,18 +1,19 @@
-"""The command-line driver (drivers/cpra_carousel.cpp) parses the reference's
-options (include/Command_Parser.hpp:21-36) and rejects bad ones before it
-touches a GPU."""
+"""The command-line drivers (drivers/cpra_carousel.cpp, drivers/cpra_conv.cpp)
+parse the reference's options (include/Command_Parser.hpp:21-36) and reject
+bad ones before they touch a GPU."""
 import os
 import subprocess
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_carousel")
+BIN = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin")
 
 
-@pytest.fixture(scope="module")
-def exe(pkg):
-    if not os.path.exists(EXE):
+@pytest.fixture(scope="module", params=["cpra_carousel", "cpra_conv"])
+def exe(pkg, request):
+    path = os.path.join(BIN, request.param)
+    if not os.path.exists(path):
         import importlib.util
 
         spec = importlib.util.spec_from_file_location(
@@ -20,7 +21,7 @@ def exe(pkg):
         mod = importlib.util.module_from_spec(spec)
         spec.loader.exec_module(mod)
         mod.build_drivers()
-    return EXE
+    return path
 
 
 def run(exe, *args):
@@ -36,16 +37,37 @@ def test_help_lists_reference_options(exe):
         assert opt in r.stdout, opt
 
 
-@pytest.mark.parametrize("args", [["--bogus", "1"], ["-m"], ["-m", "x"], ["-p", "1"], ["--Beta=0"],
-                                  ["--iter", "-1"]])
+@pytest.mark.parametrize("args", [["--bogus", "1"], ["-m"], ["-m", "x"], ["--Beta=0"],
+                                  ["--iter", "-1"], ["--Batch", "0"]])
 def test_bad_options_exit_2(exe, args):
     r = run(exe, *args)
     assert r.returncode == 2, (args, r.stdout, r.stderr)
 
 
+def test_carousel_needs_two_projections(exe):
+    r = run(exe, "-p", "1", "-d", "/nonexistent/a", "-s", "/nonexistent/b")
+    # the carousel warm start reads both neighbours of a projection
+    assert r.returncode == (2 if exe.endswith("cpra_carousel") else 1), (r.stdout, r.stderr)
+
+
 def test_missing_input_file_fails_cleanly(exe, tmp_path):
     # the input files are read before any GPU call
-    r = run(exe, "-m", "8", "-n", "8", "-p", "2", "-d", str(tmp_path / "none.bin"),
+    r = run(exe, "-m", "8", "-n", "8", "-l", "8", "-p", "2", "-d", str(tmp_path / "none.bin"),
             "-s", str(tmp_path / "none2.bin"))
     assert r.returncode == 1
     assert "not exist" in r.stderr
+
+
+BENCH = os.path.join(BIN, "cpra_benchmark")
+
+
+@pytest.mark.parametrize("args", [[], ["1", "2", "3"], ["--size", "7", "1", "1"], ["x"],
+                                  ["--size"], ["1", "-2"]])
+def test_benchmark_rejects_bad_arguments(pkg, args):
+    """bin/cpra_benchmark (src/benchmark_mkl.cpp:256-282): one positional
+    argument runs the conventional benchmark, two the CPRA benchmark; anything
+    else is refused before any GPU call."""
+    if not os.path.exists(BENCH):
+        pytest.skip("cpra_benchmark not built")
+    r = subprocess.run([BENCH, *args], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2, (args, r.stdout, r.stderr)

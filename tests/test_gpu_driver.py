@@ -60,3 +60,77 @@ def test_driver_matches_oracle(oracle, gpu, tmp_path, M):
             np.testing.assert_allclose(e, err[:, i], rtol=2e-3, atol=1e-6)
             k += 1
     assert not (tmp_path / f"rec.bin.{k}").exists()
+
+
+CONV = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_conv")
+
+
+@pytest.mark.parametrize("L,M,N", [(64, 64, 64), (40, 32, 48)])  # fused engine; primitives
+def test_conv_driver_matches_oracle(oracle, gpu, tmp_path, L, M, N):
+    """bin/cpra_conv (src/CONV_MKL.cpp:10-174) against oracle.conv_driver, from
+    the same initial guess, through the reference's file formats: batches of 2
+    and 1 of a total of 3, one output pair per member."""
+    batch, total, iters, beta = 2, 3, 4, 0.9
+    rng = np.random.default_rng(12)
+    zz, yy, xx = np.mgrid[:L, :M, :N]
+    sup = (((zz - L / 2) / (0.3 * L)) ** 2 + ((yy - M / 2) / (0.3 * M)) ** 2
+           + ((xx - N / 2) / (0.3 * N)) ** 2) <= 1.0
+    S = sup.astype(np.float32)
+    obj = sup * rng.uniform(0.2, 1.0, (L, M, N))
+    ax = (0, 1, 2)
+    A = np.abs(np.fft.fftshift(np.fft.fftn(np.fft.fftshift(obj, axes=ax)), axes=ax)).astype(np.float32)
+    init = (rng.random((batch, L, M, N)) + 1j * rng.random((batch, L, M, N))).astype(np.complex64)
+    A.tofile(tmp_path / "data.bin")
+    S.tofile(tmp_path / "space.bin")
+    init.tofile(tmp_path / "init.bin")
+    r = subprocess.run([CONV, "-m", str(M), "-n", str(N), "-l", str(L), "-i", str(iters),
+                        "--Batch", str(batch), "--Total", str(total), "--Beta", str(beta),
+                        "-d", str(tmp_path / "data.bin"), "-s", str(tmp_path / "space.bin"),
+                        "-r", str(tmp_path / "rec.bin"), "--output_error", str(tmp_path / "err.bin"),
+                        "--init", str(tmp_path / "init.bin")],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert ("primitive composition" in r.stdout) == (L == 40)
+    k = 1
+    for n in (2, 1):
+        psi, err = oracle.conv_driver(A.astype(np.float64), S.astype(np.float64),
+                                      init[:n].astype(np.complex128), iters, beta)
+        for i in range(n):
+            rec = np.fromfile(tmp_path / f"rec.bin.{k}", dtype=np.float32)
+            e = np.fromfile(tmp_path / f"err.bin.{k}", dtype=np.float32)
+            assert rec.size == L * M * N and e.size == 1
+            assert relerr(rec.reshape(L, M, N), psi[i].real) < 1e-4, k
+            np.testing.assert_allclose(e, [err[i]], rtol=2e-3, atol=1e-6)
+            k += 1
+    assert not (tmp_path / f"rec.bin.{k}").exists()
+
+
+BENCH = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_benchmark")
+
+
+@pytest.mark.parametrize("extra", [[], ["--primitives"]])
+def test_benchmark_protocol_runs(gpu, extra):
+    """bin/cpra_benchmark EPI ITER (src/benchmark_mkl.cpp:15-188) at a reduced
+    size prints the reference's step lines with positive times."""
+    r = subprocess.run([BENCH, "--size", "64", *extra, "2", "3"], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = r.stdout
+    assert "Running CPRA benchmark, epi is 2 iter is 3" in out
+    for key in ("Finished step A computation", "Finished step B computation",
+                "Estimated step C elapsed time:", "Finished step D computation",
+                "Estimated elapsed time with 100 initial guesses in total:"):
+        assert key in out, key
+    times = [float(ln.split("elapsed time:")[1].rstrip("s")) for ln in out.splitlines()
+             if ln.startswith("elapsed time:")]
+    assert len(times) == 3 and all(t > 0 for t in times)
+
+
+def test_conventional_benchmark_runs(gpu):
+    r = subprocess.run([BENCH, "--size", "64", "3"], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "Running Conventional benchmark, iter is 3" in r.stdout
+    assert "Estimated elapsed timewith 100 intial guesses in total :" in r.stdout
