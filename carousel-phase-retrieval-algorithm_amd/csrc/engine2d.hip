@@ -263,9 +263,9 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
                         void* Z, const float* A, int64_t amp_batch, float scale, int skip,
                         float* rf_part, float gr = 0.f, float gc = 0.f,
                         const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0,
-                        int p0 = 0, bool tiled = false)
+                        int p0 = 0, bool tiled = false, const ColVariant* force = nullptr)
 {
-    const ColVariant cv = col_variant(h, g.M);
+    const ColVariant cv = force ? *force : col_variant(h, g.M);
     if (tiled && g.M != g.N) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major state needs square patterns");
 #define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0, tiled)
     switch (g.M) {
@@ -851,9 +851,16 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
     dm_entry_launch(st, grow, p, static_cast<const c2<float>*>(psi), static_cast<const c2<float>*>(t1),
                     static_cast<const c2<float>*>(t2), support, Y, Y + total);
     ypass(st, Y, -1);
+    // z lines are M*N*8 bytes apart: 32-column tiles (256-byte segments) when
+    // the volume's planes split into them, unless the handle was tuned
+    ColVariant zcv = col_variant(h, L);
+    if (h->tuning.find(((uint64_t)1 << 32) | (uint64_t)L) == h->tuning.end() &&
+        ((int64_t)M * N) % (2 * col_tile_rt(L)) == 0)
+        zcv.map = MAP_WIDE;
     auto one = [&](hipStream_t s, bool last) {
         kt.begin(h, s, KT_COL);
-        launch_cols(h, s, gz2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr);
+        launch_cols(h, s, gz2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr,
+                    0.f, 0.f, nullptr, 0.f, 0, 0, false, &zcv);
         kt.end(h, s);
         ypass(s, Z, +1);
         kt.begin(h, s, KT_ROW);

@@ -655,59 +655,138 @@ __device__ __forceinline__ void dm_rows_block(
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
     const float ib = 1.f / beta, pb = 1.f + 1.f / beta, cb = 1.f - 1.f / beta;
-    V a[E], b[E];
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const size_t o = base + PL::rot(m) * TT;
-        a[m] = valid ? Z1[o] : mk<V>(0, 0);
-        b[m] = valid ? Z2[o] : mk<V>(0, 0);
-    });
-    fft_line<PL, +1>(a, t, tw, acc);
-    fft_line<PL, +1>(b, t, tw, acc);
-    // a <- t1' (or the next Y1 source), b <- t2' (or next Y2 source)
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const size_t o = base + PL::rot(m) * TT;
-        const bool in = valid && mask[o] != 0;
-        const V p = valid ? psi[o] : mk<V>(0, 0);
-        // explicit fmaf: the same operations whatever the kernel inlining this
-        // body leaves the compiler free to contract (the launch modes of the
-        // mixed-size engine agree bit for bit)
-        const V w = mk<V>(fmaf(pb, a[m].x, -ib * p.x), fmaf(pb, a[m].y, -ib * p.y));
-        const V t1n = ps_apply<SPACE>(w, in);
-        const V t2n = b[m];
-        V pn = mk<V>(fmaf(beta, t1n.x, p.x), fmaf(beta, t1n.y, p.y));
-        pn = mk<V>(fmaf(-beta, t2n.x, pn.x), fmaf(-beta, t2n.y, pn.y));
-        if (valid) psi[o] = pn;
+    if constexpr (TEXTBOOK || N > 256) {
+        // Textbook map: Y1 and Y2 both come from psi', so Z1 and Z2 are
+        // transformed together and both forward transforms follow the update.
+        // Lines longer than 256 take this order too: for them the compiler
+        // allocates fewer registers to it than to the two-phase order below.
+        V a[E], b[E];
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const size_t o = base + PL::rot(m) * TT;
+            a[m] = valid ? Z1[o] : mk<V>(0, 0);
+            b[m] = valid ? Z2[o] : mk<V>(0, 0);
+        });
+        fft_line<PL, +1>(a, t, tw, acc);
+        fft_line<PL, +1>(b, t, tw, acc);
+        // a <- t1' (or the next Y1 source), b <- t2' (or next Y2 source)
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const size_t o = base + PL::rot(m) * TT;
+            const bool in = valid && mask[o] != 0;
+            const V p = valid ? psi[o] : mk<V>(0, 0);
+            // explicit fmaf: the same operations whatever the kernel inlining this
+            // body leaves the compiler free to contract (the launch modes of the
+            // mixed-size engine agree bit for bit)
+            const V w = mk<V>(fmaf(pb, a[m].x, -ib * p.x), fmaf(pb, a[m].y, -ib * p.y));
+            const V t1n = ps_apply<SPACE>(w, in);
+            const V t2n = b[m];
+            V pn = mk<V>(fmaf(beta, t1n.x, p.x), fmaf(beta, t1n.y, p.y));
+            pn = mk<V>(fmaf(-beta, t2n.x, pn.x), fmaf(-beta, t2n.y, pn.y));
+            if (valid) psi[o] = pn;
+            if constexpr (LAST) {
+                a[m] = t1n;
+                b[m] = t2n;
+            } else {
+                const V s1 = TEXTBOOK ? pn : t1n;
+                const V q = ps_apply<SPACE>(TEXTBOOK ? pn : t2n, in);
+                a[m] = s1;
+                b[m] = mk<V>(fmaf(cb, q.x, pn.x * ib), fmaf(cb, q.y, pn.y * ib));
+            }
+        });
         if constexpr (LAST) {
-            a[m] = t1n;
-            b[m] = t2n;
+            if (valid) {
+                sfor<E>([&](auto mc) {
+                    constexpr int m = decltype(mc)::value;
+                    const size_t o = base + PL::rot(m) * TT;
+                    if (t1out) t1out[o] = a[m];
+                    if (t2out) t2out[o] = b[m];
+                });
+            }
         } else {
-            const V s1 = TEXTBOOK ? pn : t1n;
-            const V q = ps_apply<SPACE>(TEXTBOOK ? pn : t2n, in);
-            a[m] = s1;
-            b[m] = mk<V>(fmaf(cb, q.x, pn.x * ib), fmaf(cb, q.y, pn.y * ib));
-        }
-    });
-    if constexpr (LAST) {
-        if (valid) {
-            sfor<E>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                const size_t o = base + PL::rot(m) * TT;
-                if (t1out) t1out[o] = a[m];
-                if (t2out) t2out[o] = b[m];
-            });
+            fft_line<PL, -1>(a, t, tw, acc);
+            fft_line<PL, -1>(b, t, tw, acc);
+            if (valid) {
+                sfor<E>([&](auto mc) {
+                    constexpr int m = decltype(mc)::value;
+                    const size_t o = base + PL::rot(m) * TT;
+                    Y1[o] = a[m];
+                    Y2[o] = b[m];
+                });
+            }
         }
     } else {
-        fft_line<PL, -1>(a, t, tw, acc);
-        fft_line<PL, -1>(b, t, tw, acc);
-        if (valid) {
-            sfor<E>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                const size_t o = base + PL::rot(m) * TT;
-                Y1[o] = a[m];
-                Y2[o] = b[m];
-            });
+        // Reference variant: two row arrays live at a time.  t1' is finished and
+        // transformed into Y1 before Z2 is loaded; psi + beta t1' is carried to
+        // the second half.  Explicit fmaf throughout: the same operations
+        // whatever kernel inlines this body.
+        static_assert(E <= 32, "support bits in one word");
+        V a[E], acc1[E];
+        unsigned inbits = 0u;  // bit m: element m lies in the support
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            a[m] = valid ? Z1[base + PL::rot(m) * TT] : mk<V>(0, 0);
+        });
+        fft_line<PL, +1>(a, t, tw, acc);
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const size_t o = base + PL::rot(m) * TT;
+            const bool inm = valid && mask[o] != 0;
+            inbits |= (inm ? 1u : 0u) << m;
+            const V p = valid ? psi[o] : mk<V>(0, 0);
+            const V w = mk<V>(fmaf(pb, a[m].x, -ib * p.x), fmaf(pb, a[m].y, -ib * p.y));
+            const V t1n = ps_apply<SPACE>(w, inm);
+            acc1[m] = mk<V>(fmaf(beta, t1n.x, p.x), fmaf(beta, t1n.y, p.y));  // psi + beta t1'
+            a[m] = t1n;
+        });
+        if constexpr (LAST) {
+            if (valid && t1out) {
+                sfor<E>([&](auto mc) {
+                    constexpr int m = decltype(mc)::value;
+                    t1out[base + PL::rot(m) * TT] = a[m];
+                });
+            }
+        } else {
+            // next Y1 = DFT_r(t1')
+            fft_line<PL, -1>(a, t, tw, acc);
+            if (valid) {
+                sfor<E>([&](auto mc) {
+                    constexpr int m = decltype(mc)::value;
+                    Y1[base + PL::rot(m) * TT] = a[m];
+                });
+            }
+        }
+        // second half: t2' = IDFT_r(Z2), psi' = (psi + beta t1') - beta t2'.
+        // The barrier keeps the scheduler from hoisting the Z2 loads above the
+        // first half (which would make three row arrays live at once).
+        __builtin_amdgcn_sched_barrier(0);
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            a[m] = valid ? Z2[base + PL::rot(m) * TT] : mk<V>(0, 0);
+        });
+        fft_line<PL, +1>(a, t, tw, acc);
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const size_t o = base + PL::rot(m) * TT;
+            const V t2n = a[m];
+            const V pn = mk<V>(fmaf(-beta, t2n.x, acc1[m].x), fmaf(-beta, t2n.y, acc1[m].y));
+            if (valid) psi[o] = pn;
+            if constexpr (LAST) {
+                if (valid && t2out) t2out[o] = t2n;
+            } else {
+                const V q = ps_apply<SPACE>(t2n, ((inbits >> m) & 1u) != 0u);
+                a[m] = mk<V>(fmaf(cb, q.x, pn.x * ib), fmaf(cb, q.y, pn.y * ib));
+            }
+        });
+        if constexpr (!LAST) {
+            // next Y2 = DFT_r(psi'/beta + (1 - 1/beta) P_S t2')
+            fft_line<PL, -1>(a, t, tw, acc);
+            if (valid) {
+                sfor<E>([&](auto mc) {
+                    constexpr int m = decltype(mc)::value;
+                    Y2[base + PL::rot(m) * TT] = a[m];
+                });
+            }
         }
     }
 }

@@ -45,14 +45,17 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_fft_contig(c2<T>* __restric
     }
 }
 
-template <int N, int DIR, typename T, bool SHIFT>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_fft_strided(
+// W = 2: tiles of twice the lines with twice the threads, so each row of a
+// tile is a 2*C*sizeof(c2<T>)-byte segment (128 bytes for 512-point float
+// lines instead of 64).
+template <int N, int DIR, typename T, bool SHIFT, int W = 1>
+__global__ __launch_bounds__(W * SizeCfg<N>::TPB) void k_fft_strided(
     c2<T>* __restrict__ data, int64_t nlines, int64_t es, int64_t G, int64_t GS, T scale,
     int do_scale)
 {
     using V = c2<T>;
     using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, C = SizeCfg<N>::C;
+    constexpr int E = PL::E, C = W * SizeCfg<N>::C;
     __shared__ V lds[N * C];
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
@@ -216,16 +219,29 @@ static void launch_strided(cpra_hip_ctx* h, void* data, int64_t lines, int64_t e
                            int64_t GS, int dir, bool shift, double scale)
 {
     using V = c2<T>;
-    const unsigned grid = (unsigned)(lines / SizeCfg<N>::C);
     const int ds = scale != 1.0;
     V* d = static_cast<V*>(data);
-#define CPRA_LS(DIR, SH)                                                                       \
-    hipLaunchKernelGGL((k_fft_strided<N, DIR, T, SH>), dim3(grid), dim3(SizeCfg<N>::TPB), 0,  \
-                       h->stream, d, lines, es, G, GS, (T)scale, ds)
+    // 512-point float lines: double-width tiles (see k_fft_strided)
+    constexpr int C2 = 2 * SizeCfg<N>::C;
+    const bool wide = N == 512 && sizeof(T) == 4 && G % C2 == 0 && lines % C2 == 0;
+#define CPRA_LS(DIR, SH, WW)                                                                        \
+    hipLaunchKernelGGL((k_fft_strided<N, DIR, T, SH, WW>), dim3((unsigned)(lines / (WW * SizeCfg<N>::C))), \
+                       dim3(WW * SizeCfg<N>::TPB), 0, h->stream, d, lines, es, G, GS, (T)scale, ds)
+    if constexpr (N == 512 && sizeof(T) == 4) {
+        if (wide) {
+            if (dir < 0) {
+                if (shift) CPRA_LS(-1, true, 2); else CPRA_LS(-1, false, 2);
+            } else {
+                if (shift) CPRA_LS(+1, true, 2); else CPRA_LS(+1, false, 2);
+            }
+            check_launch("k_fft_strided");
+            return;
+        }
+    }
     if (dir < 0) {
-        if (shift) CPRA_LS(-1, true); else CPRA_LS(-1, false);
+        if (shift) CPRA_LS(-1, true, 1); else CPRA_LS(-1, false, 1);
     } else {
-        if (shift) CPRA_LS(+1, true); else CPRA_LS(+1, false);
+        if (shift) CPRA_LS(+1, true, 1); else CPRA_LS(+1, false, 1);
     }
 #undef CPRA_LS
     check_launch("k_fft_strided");

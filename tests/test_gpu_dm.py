@@ -126,7 +126,8 @@ def volume_problem(L, M, N, ab, seed):
 
 
 @pytest.mark.parametrize("L,M,N,B,ab", [(64, 64, 64, 1, 1), (64, 64, 64, 2, 1),
-                                        (64, 128, 192, 2, 2), (128, 64, 64, 1, 1)])
+                                        (64, 128, 192, 2, 2), (128, 64, 64, 1, 1),
+                                        (64, 512, 64, 1, 1), (512, 64, 64, 1, 1)])
 @pytest.mark.parametrize("iters", [1, 3])
 def test_dm3d_reference_variant_matches_oracle(hipmod, oracle, gpu, L, M, N, B, ab, iters):
     """The 3D difference map of src/CONV_MKL.cpp:47-63 (t1, t2 carried) on the

@@ -56,7 +56,9 @@ def test_driver_matches_oracle(oracle, gpu, tmp_path, M):
             rec = np.fromfile(tmp_path / f"rec.bin.{k}", dtype=np.float32)
             e = np.fromfile(tmp_path / f"err.bin.{k}", dtype=np.float32)
             assert rec.size == P * M * N and e.size == P
-            assert relerr(rec.reshape(P, M, N), psi[:, i].real) < 1e-4, k
+            # FP32 rounding accumulates over the chained DM iterations
+            # (2e-5 per iteration, as in test_gpu_dm)
+            assert relerr(rec.reshape(P, M, N), psi[:, i].real) < 2e-5 * (preit + epi * iters), k
             np.testing.assert_allclose(e, err[:, i], rtol=2e-3, atol=1e-6)
             k += 1
     assert not (tmp_path / f"rec.bin.{k}").exists()
