@@ -286,12 +286,12 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
                               const unsigned* rowbits, bool tiled)
 {
     const int64_t lines = g.B * g.M;
-    constexpr int LPB = SizeCfg<N>::LPB;
+    constexpr int LPB = HioRowCfg<N>::LPB;
     const unsigned grid = (unsigned)((lines + LPB - 1) / LPB);
     auto* z = static_cast<const c2<float>*>(Z);
     auto* y = static_cast<c2<float>*>(Y);
     const float invN = 1.0f / (float)N;
-    const dim3 blk(SizeCfg<N>::TPB);
+    const dim3 blk(HioRowCfg<N>::TPB);
     const bool exact = lines % LPB == 0;
     if (tiled) {
         if (g.M != N || N % LPB != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major rows: square patterns only");
@@ -558,8 +558,8 @@ static void dm_entry_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p
                        const uint8_t* mask, c2<float>* Y1, c2<float>* Y2)
 {
     const int64_t lines = g.B * g.M;
-    constexpr int LPB = SizeCfg<N>::LPB;
-    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(SizeCfg<N>::TPB);
+    constexpr int LPB = DmRowCfg<N>::LPB;
+    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(DmRowCfg<N>::TPB);
 #define CPRA_DME(TB, SP) hipLaunchKernelGGL((k_dm_entry<N, TB, SP>), grid, blk, 0, st, psi, t1, t2, mask, Y1, Y2, lines, p->beta)
     if (p->textbook) {
         if (p->space_mode == 0) CPRA_DME(true, SPACE_MKL);
@@ -580,8 +580,8 @@ static void dm_rows_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p,
                       c2<float>* t2)
 {
     const int64_t lines = g.B * g.M;
-    constexpr int LPB = SizeCfg<N>::LPB;
-    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(SizeCfg<N>::TPB);
+    constexpr int LPB = DmRowCfg<N>::LPB;
+    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(DmRowCfg<N>::TPB);
 #define CPRA_DMR(TB, SP, LA) hipLaunchKernelGGL((k_dm_rows<N, TB, SP, LA>), grid, blk, 0, st, Z1, Z2, psi, mask, Y1, Y2, t1, t2, lines, p->beta)
 #define CPRA_DMR2(TB, SP) do { if (last) CPRA_DMR(TB, SP, true); else CPRA_DMR(TB, SP, false); } while (0)
     if (p->textbook) {

@@ -37,9 +37,11 @@ __device__ __forceinline__ unsigned tile_index(int xcd_remap)
 // column-kernel block then reads and writes one contiguous N*C block instead
 // of N row segments of C*8 bytes N*8 bytes apart, and a row-kernel wave's
 // load of one register over its 64/T rows is one contiguous run.
-template <int N> struct Tiled {
+// TT: threads per row line of the kernel that addresses the layout (the
+// layout itself depends on C only).
+template <int N, int TT = SizeCfg<N>::P::T> struct Tiled {
     static constexpr int C = SizeCfg<N>::C;
-    static constexpr int T = SizeCfg<N>::P::T;  // threads per row line
+    static constexpr int T = TT;
     static_assert(C % T == 0 || T % C == 0, "tile width and row threads must nest");
     // offset in a pattern of element c = K*T + t of row r, split into a
     // compile-time part per register K (a wave-uniform base: the access
@@ -316,14 +318,15 @@ enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
 // TILED: Z and Y in the tile-major layout of N x N patterns (Tiled<N>), the
 // mask row-major; a block's LPB lines then lie in one pattern (LPB | N).
 template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
+__global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
     int64_t nlines, float beta, float invN, int real_positive,
     const unsigned* __restrict__ rowbits)
 {
     using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    using PL = typename HioRowCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = HioRowCfg<N>::LPB;
+    using TL = Tiled<N, TT>;
     constexpr int SPAN = LdsRow<V, N>::span();
     __shared__ V lds[LPB * SPAN];
     const int tid = threadIdx.x;
@@ -341,12 +344,12 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
     // tiled state: pattern and first row of the block; element m at the
     // register base kb(m) plus the lane offset sl
     const size_t pbase = TILED ? ((size_t)blockIdx.x * LPB / N) * N * N
-                                     + (((size_t)blockIdx.x * LPB) % N) * Tiled<N>::C
+                                     + (((size_t)blockIdx.x * LPB) % N) * TL::C
                                : bbase;
-    const unsigned sl = TILED ? Tiled<N>::lane(t, ll) : (unsigned)lo;
+    const unsigned sl = TILED ? TL::lane(t, ll) : (unsigned)lo;
     auto kb = [&](auto mc) -> size_t {
         constexpr int m = decltype(mc)::value;
-        if constexpr (TILED) return Tiled<N>::template kpart<PL::rot(m)>();
+        if constexpr (TILED) return TL::template kpart<PL::rot(m)>();
         else return (size_t)(PL::rot(m) * TT);
     };
     const c2<float>* Zb = Z + pbase;
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
         constexpr int m = decltype(mc)::value;
         const int o = lo + PL::rot(m) * TT;
         if constexpr (TILED) {
-            constexpr unsigned kbytes = (unsigned)(Tiled<N>::template kpart<PL::rot(m)>() * sizeof(V));
+            constexpr unsigned kbytes = (unsigned)(TL::template kpart<PL::rot(m)>() * sizeof(V));
             xp[m] = valid ? zr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
             xo[m] = valid ? yr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
         } else {
@@ -391,7 +394,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_eng_rows_hio(
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             if constexpr (TILED) {
-                constexpr unsigned kbytes = (unsigned)(Tiled<N>::template kpart<PL::rot(m)>() * sizeof(V));
+                constexpr unsigned kbytes = (unsigned)(TL::template kpart<PL::rot(m)>() * sizeof(V));
                 yr.st(xo[m], sl * (unsigned)sizeof(V), kbytes);
             } else {
                 (Yb + kb(mc))[sl] = xo[m];
@@ -585,14 +588,14 @@ template <int SPACE> __device__ __forceinline__ c2<float> ps_apply(c2<float> z, 
 // Entry: Y1 = DFT_r(t1 | psi), Y2 = DFT_r(psi/beta + (1-1/beta) P_S(t2 | psi)).
 // Y1 and Y2 are the two halves of one 2B-pattern array (Y2 = Y1 + B*M*N).
 template <int N, bool TEXTBOOK, int SPACE>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_entry(
+__global__ __launch_bounds__(DmRowCfg<N>::TPB) void k_dm_entry(
     const c2<float>* __restrict__ psi, const c2<float>* __restrict__ t1,
     const c2<float>* __restrict__ t2, const uint8_t* __restrict__ mask,
     c2<float>* __restrict__ Y1, c2<float>* __restrict__ Y2, int64_t nlines, float beta)
 {
     using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    using PL = typename DmRowCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = DmRowCfg<N>::LPB;
     constexpr int SPAN = LdsRow<V, N>::span();
     __shared__ V lds[LPB * SPAN];
     const int tid = threadIdx.x;
@@ -633,9 +636,9 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_entry(
 //   psi' = (psi + beta t1') - beta t2'                          :109-110
 // then Y1 = DFT_r(t1' | psi'), Y2 = DFT_r(psi'/beta + (1-1/beta) P_S(t2' | psi'))
 // for the next iteration; LAST writes psi', t1', t2' in pattern space instead.
-// The body handles the LPB lines of block `vb` with the caller's LDS
-// (LPB * LdsRow<V, N>::span() elements); k_dm_rows and the mixed-size kernel
-// run it.
+// The body handles the LPB lines of block `vb` (DmRowCfg<N>) with the
+// caller's LDS (LPB * LdsRow<V, N>::span() elements); k_dm_rows and the
+// mixed-size kernel run it.
 template <int N, bool TEXTBOOK, int SPACE, bool LAST>
 __device__ __forceinline__ void dm_rows_block(
     const unsigned vb, c2<float>* lds, const c2<float>* __restrict__ Z1, const c2<float>* __restrict__ Z2,
@@ -644,8 +647,8 @@ __device__ __forceinline__ void dm_rows_block(
     int64_t nlines, float beta)
 {
     using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
+    using PL = typename DmRowCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = DmRowCfg<N>::LPB;
     constexpr int SPAN = LdsRow<V, N>::span();
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
@@ -792,13 +795,13 @@ __device__ __forceinline__ void dm_rows_block(
 }
 
 template <int N, bool TEXTBOOK, int SPACE, bool LAST>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_dm_rows(
+__global__ __launch_bounds__(DmRowCfg<N>::TPB) void k_dm_rows(
     const c2<float>* __restrict__ Z1, const c2<float>* __restrict__ Z2,
     c2<float>* __restrict__ psi, const uint8_t* __restrict__ mask, c2<float>* __restrict__ Y1,
     c2<float>* __restrict__ Y2, c2<float>* __restrict__ t1out, c2<float>* __restrict__ t2out,
     int64_t nlines, float beta)
 {
-    __shared__ c2<float> lds[SizeCfg<N>::LPB * LdsRow<c2<float>, N>::span()];
+    __shared__ c2<float> lds[DmRowCfg<N>::LPB * LdsRow<c2<float>, N>::span()];
     dm_rows_block<N, TEXTBOOK, SPACE, LAST>(blockIdx.x, lds, Z1, Z2, psi, mask, Y1, Y2, t1out, t2out,
                                             nlines, beta);
 }

@@ -107,9 +107,9 @@ template <int N> static SizeGeo geo_n()
 {
     SizeGeo g;
     g.C = SizeCfg<N>::C;
-    g.LPB = SizeCfg<N>::LPB;
+    g.LPB = DmRowCfg<N>::LPB;
     g.col_lds = sizeof(c2<float>) * (size_t)N * SizeCfg<N>::C;
-    g.row_lds = sizeof(c2<float>) * (size_t)SizeCfg<N>::LPB * LdsRow<c2<float>, N>::span();
+    g.row_lds = sizeof(c2<float>) * (size_t)DmRowCfg<N>::LPB * LdsRow<c2<float>, N>::span();
     return g;
 }
 
@@ -131,9 +131,9 @@ static void entry_n(hipStream_t st, const MixedClass& c, float beta)
 {
     const int64_t lines = (int64_t)c.count * N;
     const size_t half = (size_t)c.count * N * N;
-    constexpr int LPB = SizeCfg<N>::LPB;
+    constexpr int LPB = DmRowCfg<N>::LPB;
     hipLaunchKernelGGL((k_dm_entry<N, true, SPACE>), dim3((unsigned)((lines + LPB - 1) / LPB)),
-                       dim3(SizeCfg<N>::TPB), 0, st, c.psi, nullptr, nullptr, c.mask, c.Y, c.Y + half, lines, beta);
+                       dim3(DmRowCfg<N>::TPB), 0, st, c.psi, nullptr, nullptr, c.mask, c.Y, c.Y + half, lines, beta);
 }
 
 template <int N, int SPACE, bool LAST>
@@ -142,9 +142,9 @@ static void rows_n(hipStream_t st, const MixedClass& c, int j0, int cnt, float b
     // patterns [j0, j0 + cnt) of the class
     const size_t pp = (size_t)N * N, half = (size_t)c.count * pp;
     const int64_t lines = (int64_t)cnt * N;
-    constexpr int LPB = SizeCfg<N>::LPB;
+    constexpr int LPB = DmRowCfg<N>::LPB;
     hipLaunchKernelGGL((k_dm_rows<N, true, SPACE, LAST>), dim3((unsigned)((lines + LPB - 1) / LPB)),
-                       dim3(SizeCfg<N>::TPB), 0, st, c.Z + j0 * pp, c.Z + half + j0 * pp, c.psi + j0 * pp,
+                       dim3(DmRowCfg<N>::TPB), 0, st, c.Z + j0 * pp, c.Z + half + j0 * pp, c.psi + j0 * pp,
                        c.mask + j0 * pp, c.Y + j0 * pp, c.Y + half + j0 * pp, nullptr, nullptr, lines, beta);
 }
 

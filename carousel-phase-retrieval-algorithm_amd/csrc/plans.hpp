@@ -58,4 +58,41 @@ template <> struct SizeCfg<512> {
     static constexpr int TPB = 256, LPB = TPB / P::T, C = TPB / P::T;
 };
 
+// Plans of the difference-map row kernels (k_dm_entry and dm_rows_block).
+// They hold two to four row arrays at a time, so where the plan above keeps
+// 24 (192, 384) or 16 (256, 512) values per thread they take 12 or 8 instead:
+// fewer registers per array, one more LDS exchange per transform.  The other
+// row and column kernels keep the plans above (the column kernels' tile width
+// C = 256 / T would narrow with a larger T).
+template <int N> struct DmRowCfg {
+    using P = typename SizeCfg<N>::P;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+template <> struct DmRowCfg<192> {
+    using P = Plan<192, 12, 12, 4, 4>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+template <> struct DmRowCfg<256> {
+    using P = Plan<256, 8, 8, 8, 4>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+template <> struct DmRowCfg<384> {
+    using P = Plan<384, 12, 6, 4, 4, 4>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+template <> struct DmRowCfg<512> {
+    using P = Plan<512, 8, 8, 8, 8>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+
+// Plan of the HIO row kernel (k_eng_rows_hio).  The 256-point rows keep the
+// one-exchange 16 x 16 plan: 8 values per thread (Plan<256, 8, 8, 8, 4>)
+// lowers the kernel from 125 to 96 VGPRs, but its two extra LDS exchanges
+// cost what the fifth wave per SIMD gains (149.4 against 148.7 µs per
+// 512-pattern pass).
+template <int N> struct HioRowCfg {
+    using P = typename SizeCfg<N>::P;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+
 }  // namespace cpra

@@ -99,6 +99,8 @@ int main()
     both<Plan<256, 16, 16, 16>>("Plan<256, 16, 16, 16>");
     both<Plan<256, 8, 8, 8, 4>>("Plan<256, 8, 8, 8, 4>");
     both<Plan<384, 24, 6, 8, 8>>("Plan<384, 24, 6, 8, 8>");
+    both<Plan<384, 12, 6, 4, 4, 4>>("Plan<384, 12, 6, 4, 4, 4>");
+    both<Plan<512, 8, 8, 8, 8>>("Plan<512, 8, 8, 8, 8>");
     both<Plan<512, 16, 16, 16, 2>>("Plan<512, 16, 16, 16, 2>");
     both<Plan<512, 32, 32, 16>>("Plan<512, 32, 32, 16>");
     return 0;
