@@ -480,8 +480,10 @@ def bench_ref_benchmark(hip, synth, ge, epi=10, iters=10, fp64=True):
     over HipImpl (bin/cpra_benchmark), run as its README documents it
     ("benchmark 10 10": 10 episodes of 10 iterations at M = N = L = 512,
     P = 288), with the fused engine and with the reference's primitive
-    sequence, plus the conventional mode with 10 iterations.  Value: the
-    program's own headline, the estimated time for 100 initial guesses."""
+    sequence, plus the conventional mode with 10 iterations.  The variant
+    that records Step B's sweep as a graph and replays it (--graph) is timed
+    beside them.  Value: the program's own headline, the estimated time for
+    100 initial guesses."""
     import re
     import subprocess
 
@@ -501,6 +503,8 @@ def bench_ref_benchmark(hip, synth, ge, epi=10, iters=10, fp64=True):
     log(f"bench[ref_benchmark]: fused {fused}")
     prim = run("--primitives", str(epi), str(iters))
     log(f"bench[ref_benchmark]: primitives {prim}")
+    graphed = run("--graph", str(epi), str(iters))
+    log(f"bench[ref_benchmark]: graph-replayed sweep {graphed}")
     r = subprocess.run([exe, str(iters)], capture_output=True, text=True, timeout=900)
     conv = re.search(r"^elapsed time: ([0-9.eE+-]+)s", r.stdout, re.M)
     return {
@@ -509,6 +513,7 @@ def bench_ref_benchmark(hip, synth, ge, epi=10, iters=10, fp64=True):
         "config": {"workload": f"bin/cpra_benchmark {epi} {iters}: M = N = L = 512, P = 288, "
                                "1000 pre-iterations, one 512^3 3D iteration (src/benchmark_mkl.cpp)"},
         "fused_engine": fused,
+        "fused_engine_graph_replayed_sweep": graphed,
         "primitive_sequence": prim,
         "conventional_mode_s": float(conv.group(1)) if conv else None,
     }

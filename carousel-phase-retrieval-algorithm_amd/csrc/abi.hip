@@ -393,6 +393,15 @@ int cpra_hip_destroy(cpra_hip_handle h)
     if (!h) return CPRA_HIP_ERR_ARG;
     (void)hipStreamSynchronize(h->stream);
     cpra_engine_release(h);
+    if (h->capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+    }
+    for (auto& kv : h->user_graphs) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
+    }
     for (auto& kv : h->twiddles) (void)hipFree(kv.second);
     h->ws_z.release();
     h->ws_state.release();
@@ -462,7 +471,70 @@ int cpra_hip_sync(cpra_hip_handle h)
 {
     return guarded(h, [&]() -> int {
         CPRA_ARG(h, "cpra_hip_sync: NULL handle");
+        CPRA_ARG(!h->capturing, "cpra_hip_sync: the handle's stream is being captured");
         CPRA_CHECK_HIP(hipStreamSynchronize(h->stream));
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_graph_begin(cpra_hip_handle h)
+{
+    return guarded(h, [&]() -> int {
+        CPRA_ARG(h, "cpra_hip_graph_begin: NULL handle");
+        CPRA_ARG(!h->capturing, "cpra_hip_graph_begin: a capture is already open");
+        CPRA_ARG(!h->timing, "cpra_hip_graph_begin: kernel timing is on");
+        // relaxed: a workspace that still grows during the capture may call
+        // hipMalloc (the allocation happens at capture time)
+        CPRA_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        h->capturing = true;
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_graph_end(cpra_hip_handle h, uint64_t* graph_id)
+{
+    return guarded(h, [&]() -> int {
+        CPRA_ARG(h && graph_id, "cpra_hip_graph_end: NULL argument");
+        CPRA_ARG(h->capturing, "cpra_hip_graph_end: no capture is open");
+        h->capturing = false;
+        hipGraph_t g = nullptr;
+        CPRA_CHECK_HIP(hipStreamEndCapture(h->stream, &g));
+        hipGraphExec_t e = nullptr;
+        const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        if (err != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            fail(CPRA_HIP_ERR_HIP, "cpra_hip_graph_end: instantiation failed: %s", hipGetErrorString(err));
+        }
+        const uint64_t id = h->next_graph++;
+        h->user_graphs[id] = {g, e};
+        *graph_id = id;
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_graph_launch(cpra_hip_handle h, uint64_t graph_id)
+{
+    return guarded(h, [&]() -> int {
+        CPRA_ARG(h, "cpra_hip_graph_launch: NULL handle");
+        CPRA_ARG(!h->capturing, "cpra_hip_graph_launch: a capture is open");
+        auto it = h->user_graphs.find(graph_id);
+        CPRA_ARG(it != h->user_graphs.end(), "cpra_hip_graph_launch: unknown graph %llu",
+                 (unsigned long long)graph_id);
+        CPRA_CHECK_HIP(hipGraphLaunch(it->second.second, h->stream));
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_graph_destroy(cpra_hip_handle h, uint64_t graph_id)
+{
+    return guarded(h, [&]() -> int {
+        CPRA_ARG(h, "cpra_hip_graph_destroy: NULL handle");
+        auto it = h->user_graphs.find(graph_id);
+        CPRA_ARG(it != h->user_graphs.end(), "cpra_hip_graph_destroy: unknown graph %llu",
+                 (unsigned long long)graph_id);
+        CPRA_CHECK_HIP(hipGraphExecDestroy(it->second.second));
+        CPRA_CHECK_HIP(hipGraphDestroy(it->second.first));
+        h->user_graphs.erase(it);
         return CPRA_HIP_OK;
     });
 }
@@ -674,6 +746,8 @@ int cpra_hip_converge_error(cpra_hip_handle h, void* old_, const void* new_, voi
 {
     return guarded(h, [&]() -> int {
         CPRA_ARG(h, "cpra_hip_converge_error: NULL handle");
+        CPRA_ARG(!h->capturing, "cpra_hip_converge_error: its host result needs a synchronisation, "
+                                "which a stream capture cannot hold");
         if (batch == 0 || m == 0 || n == 0 || l == 0) return CPRA_HIP_FALSE;
         CPRA_ARG(old_ && new_ && host_err, "cpra_hip_converge_error: NULL pointer");
         CPRA_ARG(batch <= 65535, "cpra_hip_converge_error: batch > 65535");

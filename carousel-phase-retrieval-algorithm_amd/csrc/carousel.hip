@@ -240,7 +240,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             done = 1;
         }
         const int rest = p->outer_iters - done;
-        if (p->use_graph && !h->timing && !debug_launches() && rest >= 2) {
+        if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest >= 2) {
             hipStream_t gst = nullptr;
             CPRA_CHECK_HIP(hipStreamCreateWithFlags(&gst, hipStreamNonBlocking));
             hipEvent_t ev0, ev1;

@@ -92,6 +92,11 @@ struct cpra_hip_ctx {
     // tuning choice per (kind, n) set by the autotuner (0 = default)
     std::map<uint64_t, int> tuning;
     cpra::EngineGraph* graph = nullptr;
+    // graphs captured from the handle's stream by the caller
+    // (cpra_hip_graph_begin / _end); engines run eagerly while capturing
+    bool capturing = false;
+    uint64_t next_graph = 1;
+    std::map<uint64_t, std::pair<hipGraph_t, hipGraphExec_t>> user_graphs;
     // per-kernel event timing of the engines (eager launches only)
     bool timing = false;
     double kt_ms[16] = {};

@@ -682,6 +682,8 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                  "cpra_hip_hio_run: amp_batch must divide batch");
         CPRA_ARG(p->iters >= 0, "cpra_hip_hio_run: iters < 0");
         CPRA_ARG(p->sw_every >= 0, "cpra_hip_hio_run: sw_every < 0");
+        CPRA_ARG(!(h->capturing && rfactor_host), "cpra_hip_hio_run: the R-factor copy-out needs a "
+                                                  "synchronisation, which a stream capture cannot hold");
         engine_check_geometry(h, "cpra_hip_hio_run");
         const Geo g = geo_of(h);
         const int64_t total = (int64_t)g.M * g.N * g.B;
@@ -712,7 +714,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         const int colv = col_variant_code(col_variant(h, g.M));
         const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
         int done = 0;
-        bool use_graph = p->use_graph && !h->timing && p->iters >= chunk && chunk > 0;
+        bool use_graph = p->use_graph && !h->timing && !h->capturing && p->iters >= chunk && chunk > 0;
         KernelTimer kt;
         hipStream_t gst = st;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -870,7 +872,7 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
         if (!last) ypass(s, Y, -1);
     };
     const int body = p->iters - 1;
-    if (p->use_graph && !h->timing && body >= 2) {
+    if (p->use_graph && !h->timing && !h->capturing && body >= 2) {
         hipStream_t gst = aux_stream(h);
         hipEvent_t ev0, ev1;
         CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
@@ -928,7 +930,7 @@ int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const floa
                            Y + total, static_cast<c2<float>*>(t1), static_cast<c2<float>*>(t2));
         };
         const int body = p->iters - 1;  // the last iteration writes t1, t2 in pattern space
-        if (p->use_graph && body >= 2) {
+        if (p->use_graph && !h->timing && !h->capturing && body >= 2) {
             hipStream_t gst = aux_stream(h);
             hipEvent_t ev0, ev1;
             CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
@@ -1050,6 +1052,7 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0,
                  "cpra_hip_autotune_2d: amp_batch must divide batch");
         CPRA_ARG(iters >= 1 && iters <= 1000, "cpra_hip_autotune_2d: iters out of range");
+        CPRA_ARG(!h->capturing, "cpra_hip_autotune_2d: the handle's stream is being captured");
         engine_check_geometry(h, "cpra_hip_autotune_2d");
         const Geo g = geo_of(h);
         const int64_t total = (int64_t)g.M * g.N * g.B;

@@ -579,6 +579,9 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void k_merge_heavy(
 // Builds the gather list of (angles, n, P) in the handle (synchronises once).
 void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
 {
+    if (h->capturing)
+        fail(CPRA_HIP_ERR_ARG, "carousel merge: building the gather list needs a synchronisation, "
+                               "which a stream capture cannot hold");
     check_merge_geometry(n, n, P, n, "carousel merge");
     if (n % 16) fail(CPRA_HIP_ERR_UNSUPPORTED, "carousel merge: n=%d not a multiple of 16", n);
     const int cols = n * n;

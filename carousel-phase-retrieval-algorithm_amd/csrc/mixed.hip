@@ -323,7 +323,7 @@ int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes,
         r.entry(st);
         const int body = p->iters - 1;
         KernelTimer kt;
-        if (p->use_graph && !h->timing && !debug_launches() && body >= 2) {
+        if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && body >= 2) {
             hipStream_t gst = nullptr;
             CPRA_CHECK_HIP(hipStreamCreateWithFlags(&gst, hipStreamNonBlocking));
             hipEvent_t ev0, ev1;

@@ -26,9 +26,10 @@
 // Inputs are uniform random (Initialize), as in the reference.  The
 // difference-map iterations of Steps A and B run through the fused engine
 // cpra_hip_dm_run; --primitives runs the reference's sequence of eleven
-// primitive calls per iteration instead (:47-62).  Additions: --size M
+// primitive calls per iteration instead (:47-62); --graph records Step B's
+// sweep once as a graph and replays it EPI times.  Additions: --size M
 // (default 512; M = N = L, P = 9M/16) for short runs, --seed S (the
-// reference draws from unseeded generators), --primitives.
+// reference draws from unseeded generators), --primitives, --graph.
 #include <chrono>
 #include <complex>
 #include <cstdio>
@@ -52,6 +53,7 @@ namespace {
 struct Config {
     uint64_t M = 512;
     bool primitives = false;
+    bool graph = false;
     unsigned long long seed = 0;
 };
 
@@ -155,8 +157,14 @@ void cpra_sample(const Config& cfg, int epi, int iter)
     const auto endA = std::chrono::system_clock::now();
     print_finished("step A computation", endA, endA - start);
 
-    // Step B (:72-125)
-    for (int e = 0; e < epi; ++e) {
+    // Step B (:72-125).  --graph: a sweep touches the same buffers in every
+    // episode, so it can be recorded once (cpra_hip_graph_begin/_end: the DM
+    // engine runs eagerly inside the capture) and replayed EPI times, with
+    // the recording and instantiation inside the timed step.  The sweep's DM
+    // iterations on single 512 x 512 patterns are bound by their kernels'
+    // own duration (64-128 blocks each), not by launches, so this does not
+    // shorten the step; it stays an option.
+    auto sweep = [&]() {
         for (uint64_t p = 0; p < P; ++p) {
             cf* cur = rg + px * p;
             if (p == P - 1) {
@@ -171,6 +179,18 @@ void cpra_sample(const Config& cfg, int epi, int iter)
             chk(compute->Memcpy(t2, cur, sizeof(cf) * px), compute.get(), "Memcpy");
             dm(p, iter);
         }
+    };
+    if (cfg.graph && epi > 0) {
+        uint64_t gid = 0;
+        if (cpra_hip_graph_begin(h) < 0 || (sweep(), cpra_hip_graph_end(h, &gid)) < 0)
+            throw std::runtime_error(std::string("graph capture: ") + cpra_hip_last_error(h));
+        for (int e = 0; e < epi; ++e)
+            if (cpra_hip_graph_launch(h, gid) < 0)
+                throw std::runtime_error(std::string("graph launch: ") + cpra_hip_last_error(h));
+        chk(compute->Sync(), compute.get(), "Sync");
+        (void)cpra_hip_graph_destroy(h, gid);
+    } else {
+        for (int e = 0; e < epi; ++e) sweep();
     }
     chk(compute->Sync(), compute.get(), "Sync");
     const auto endB = std::chrono::system_clock::now();
@@ -277,7 +297,7 @@ void conventional_sample(const Config& cfg, int iter)
 [[noreturn]] void usage(const std::string& msg)
 {
     std::cerr << "cpra_benchmark: " << msg << "\n"
-              << "usage: cpra_benchmark [--size M] [--seed S] [--primitives] EPI ITER   (CPRA benchmark)\n"
+              << "usage: cpra_benchmark [--size M] [--seed S] [--primitives] [--graph] EPI ITER   (CPRA benchmark)\n"
               << "       cpra_benchmark [--size M] [--seed S] ITER                      (conventional benchmark)\n";
     std::exit(2);
 }
@@ -305,6 +325,8 @@ int main(int argc, char* argv[])
             ++k;
         } else if (a == "--primitives") {
             cfg.primitives = true;
+        } else if (a == "--graph") {
+            cfg.graph = true;
         } else if (a == "-h" || a == "--help") {
             usage("help");
         } else {

@@ -132,6 +132,10 @@ _SIGS = {
     "cpra_hip_dm_default_params": (None, [C.POINTER(DmParams)]),
     "cpra_hip_dm_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
     "cpra_hip_dm3d_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
+    "cpra_hip_graph_begin": (C.c_int, [_H]),
+    "cpra_hip_graph_end": (C.c_int, [_H, C.POINTER(C.c_uint64)]),
+    "cpra_hip_graph_launch": (C.c_int, [_H, C.c_uint64]),
+    "cpra_hip_graph_destroy": (C.c_int, [_H, C.c_uint64]),
     "cpra_hip_rfactor": (C.c_int, [_H, _P, _P, _U64, _P]),
     "cpra_hip_support_from_constraint": (C.c_int, [_H, _P, _P, _U64]),
     "cpra_hip_real2d_to_3d": (C.c_int, [_H, _P, _P, _P, _U64, _U64, _U64, _U64]),
@@ -400,6 +404,21 @@ class HipImpl:
 
     def Sync(self) -> bool:
         return self._check(lib().cpra_hip_sync(self._h))
+
+    # ---- graph capture of the handle's stream ---------------------------
+    def graph_begin(self) -> bool:
+        return self._check(lib().cpra_hip_graph_begin(self._h))
+
+    def graph_end(self) -> int:
+        gid = C.c_uint64(0)
+        self._check(lib().cpra_hip_graph_end(self._h, C.byref(gid)))
+        return gid.value
+
+    def graph_launch(self, graph_id: int) -> bool:
+        return self._check(lib().cpra_hip_graph_launch(self._h, graph_id))
+
+    def graph_destroy(self, graph_id: int) -> bool:
+        return self._check(lib().cpra_hip_graph_destroy(self._h, graph_id))
 
     # -- fused engines (include/cpra_hip_engine.h) -------------------------
     def hio_run(self, x, amp, support, params: Optional[HioParams] = None, amp_batch=None,

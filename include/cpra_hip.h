@@ -76,6 +76,20 @@ int cpra_hip_free(cpra_hip_handle h, void* ptr);
 int cpra_hip_memcpy(cpra_hip_handle h, void* dst, const void* src, uint64_t bytes);
 /* Replaces CudaImpl::Sync (include/CUDA_Impl.cuh:99-103). */
 int cpra_hip_sync(cpra_hip_handle h);
+/* Graph capture of the handle's stream (north-star addition; SURVEY's
+ * graph_capture / graph_launch): every call made on the handle between
+ * cpra_hip_graph_begin and cpra_hip_graph_end is recorded, not run, and the
+ * instantiated graph replays it with cpra_hip_graph_launch on the handle's
+ * stream.  The engines launch eagerly inside a capture (no nested graphs);
+ * calls that need a host synchronisation (cpra_hip_sync, ConvergeError, the
+ * R-factor copy-out, the autotuner, the carousel's gather-list build) are
+ * refused while a capture is open.  Pointers and sizes are fixed at capture;
+ * workspaces should be sized by one eager run first.  Replaces nothing in the
+ * reference, whose backends launch every call eagerly. */
+int cpra_hip_graph_begin(cpra_hip_handle h);
+int cpra_hip_graph_end(cpra_hip_handle h, uint64_t* graph_id);
+int cpra_hip_graph_launch(cpra_hip_handle h, uint64_t graph_id);
+int cpra_hip_graph_destroy(cpra_hip_handle h, uint64_t graph_id);
 
 /* Replaces CpraImpl::Initialize (include/CPRA_Impl.hpp:23; CUDA
  * include/detail/CUDA_Impl.inl:8-18): fills num values of T with uniform

@@ -64,3 +64,12 @@ def test_hio_params_layout_matches_header(hipmod):
     assert C.sizeof(hipmod.HioParams) == 40
     p = hipmod.default_hio_params()
     assert p.iters == 200 and abs(p.beta - 0.9) < 1e-7 and p.sw_every == 20
+
+
+def test_graph_capture_entry_points_validate(hipmod):
+    L = hipmod.lib()
+    gid = C.c_uint64(0)
+    assert L.cpra_hip_graph_begin(None) == hipmod.ERR_ARG
+    assert L.cpra_hip_graph_end(None, C.byref(gid)) == hipmod.ERR_ARG
+    assert L.cpra_hip_graph_launch(None, 1) == hipmod.ERR_ARG
+    assert L.cpra_hip_graph_destroy(None, 1) == hipmod.ERR_ARG

@@ -111,7 +111,7 @@ def test_conv_driver_matches_oracle(oracle, gpu, tmp_path, L, M, N):
 BENCH = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_benchmark")
 
 
-@pytest.mark.parametrize("extra", [[], ["--primitives"]])
+@pytest.mark.parametrize("extra", [[], ["--primitives"], ["--graph"]])
 def test_benchmark_protocol_runs(gpu, extra):
     """bin/cpra_benchmark EPI ITER (src/benchmark_mkl.cpp:15-188) at a reduced
     size prints the reference's step lines with positive times."""

@@ -232,3 +232,53 @@ def test_fft_known_answers_f32(hipmod, gpu):
         y = d.numpy()
         assert abs(y[M // 2, N // 2 + k] - M * N / 2) < 0.05
         assert abs(y[M // 2, N // 2 - k] - M * N / 2) < 0.05
+
+
+def test_graph_capture_replays_primitive_sequence(hipmod, gpu):
+    """cpra_hip_graph_begin/_end record the calls made on a handle (here the
+    reference's DM iteration as primitives, src/CPRA_MKL.cpp:94-110, plus one
+    fused engine call, which runs eagerly inside the capture); three replays
+    give the bits of three eager runs.  Calls that need a host
+    synchronisation are refused while the capture is open."""
+    n, B = 64, 2
+    rng = np.random.default_rng(5)
+    shp = (B, n, n)
+    x0 = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(np.complex64)
+    A = rng.random(shp).astype(np.float32)
+    S = rng.random(shp).astype(np.float32)
+    px = B * n * n
+    outs = []
+    for mode in ("eager", "graph"):
+        with hipmod.HipImpl(n, n, 1, B) as h:
+            psi, t1, t2 = (hipmod.to_device(x0) for _ in range(3))
+            dA, dS = hipmod.to_device(A), hipmod.to_device(S)
+            mask = hipmod.zeros(shp, np.uint8)
+            h.support_from_constraint(dS, mask, px)
+
+            def body():
+                h.Forward2D(t1)
+                h.DataConstraint(t1, dA, px, B)
+                h.Backward2D(t1)
+                h.MergeAddData(psi, t1, -1 / 0.9, 1 + 1 / 0.9, px)
+                h.SpaceConstraint(t1, dS, px, B)
+                h.dm_run(psi, dA, mask, t1=t1, t2=t2, iters=3, beta=0.9, textbook=0, use_graph=1)
+
+            h.dm_run(psi, dA, mask, t1=t1, t2=t2, iters=1, use_graph=0)  # size the workspaces
+            h.Sync()
+            if mode == "eager":
+                for _ in range(3):
+                    body()
+            else:
+                assert h.graph_begin()
+                body()
+                with pytest.raises(hipmod.CpraHipError) as e:
+                    h.Sync()
+                assert e.value.code == hipmod.ERR_ARG
+                gid = h.graph_end()
+                for _ in range(3):
+                    assert h.graph_launch(gid)
+                assert h.graph_destroy(gid)
+            h.Sync()
+            outs.append([a.numpy() for a in (psi, t1, t2)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
