@@ -84,7 +84,9 @@ __global__ void k_sum_partials(const double* __restrict__ part, int nblk, int64_
 //  GD_LOSS:      loss partials only (Y untouched)
 enum GdMode { GD_STEP = 0, GD_STEP_LOSS = 1, GD_LOSS = 2 };
 
-template <int M, int MODE>
+// TILED (2D, square patterns): Y and A in the tile-major layout (each
+// block's M x C tile contiguous, row stride C; Tiled<M> in engine_kernels.hpp).
+template <int M, int MODE, bool TILED = false>
 __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_gd_cols(c2<float>* Y, const float* __restrict__ A,
                                                             int64_t amp_batch, int64_t W, float invM,
                                                             float step, float* __restrict__ loss_part,
@@ -102,12 +104,16 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_gd_cols(c2<float>* Y, const
     const unsigned b = tile / tiles_per_pattern;
     const int64_t col0 = (int64_t)(tile - b * tiles_per_pattern) * C;
     const size_t pat = (size_t)M * (size_t)W;
-    c2<float>* Yp = Y + b * pat + col0;
-    const float* Ap = A + (size_t)(b % (unsigned)amp_batch) * pat + col0;
+    const size_t blk0 = TILED ? (size_t)tile * M * C : b * pat + col0;
+    const size_t ablk0 = (size_t)(b % (unsigned)amp_batch) * pat +
+                         (TILED ? (size_t)(tile - b * tiles_per_pattern) * M * C : (size_t)col0);
+    c2<float>* Yp = Y + blk0;
+    const float* Ap = A + ablk0;
     // element m of this lane sits at line position t + rot(m)*T: a uniform
-    // base per m plus one per-lane offset
-    const unsigned toff = (unsigned)t * (unsigned)W + (unsigned)c;  // < T*W <= 2^24
-    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)W; };
+    // base per m plus one per-lane offset (row stride W, or C when tiled)
+    const unsigned RS = TILED ? (unsigned)C : (unsigned)W;
+    const unsigned toff = (unsigned)t * RS + (unsigned)c;  // < T*W <= 2^24
+    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)RS; };
     V v[E];
     float av[E];
     sfor<E>([&](auto mc) {
@@ -259,16 +265,25 @@ static void check_even(const Dims& d, const char* who)
 
 // Fused gradient descent, 2D (lines along M, W = N) or 3D (lines along L,
 // W = M*N).  Entry: the inner axes are transformed in place (Y); exit: back.
-template <int LEN>
-static void gd_launch(hipStream_t st, int mode, unsigned grid, c2<float>* Y, const float* A,
-                      int64_t amp_batch, int64_t W, float invM, float step, float* part, int remap)
+template <int LEN, bool TL>
+static void gd_launch_t(hipStream_t st, int mode, unsigned grid, c2<float>* Y, const float* A,
+                        int64_t amp_batch, int64_t W, float invM, float step, float* part, int remap)
 {
     if (mode == GD_STEP)
-        hipLaunchKernelGGL((k_gd_cols<LEN, GD_STEP>), dim3(grid), dim3(SizeCfg<LEN>::TPB), 0, st, Y, A, amp_batch, W, invM, step, part, remap);
+        hipLaunchKernelGGL((k_gd_cols<LEN, GD_STEP, TL>), dim3(grid), dim3(SizeCfg<LEN>::TPB), 0, st, Y, A, amp_batch, W, invM, step, part, remap);
     else if (mode == GD_STEP_LOSS)
-        hipLaunchKernelGGL((k_gd_cols<LEN, GD_STEP_LOSS>), dim3(grid), dim3(SizeCfg<LEN>::TPB), 0, st, Y, A, amp_batch, W, invM, step, part, remap);
+        hipLaunchKernelGGL((k_gd_cols<LEN, GD_STEP_LOSS, TL>), dim3(grid), dim3(SizeCfg<LEN>::TPB), 0, st, Y, A, amp_batch, W, invM, step, part, remap);
     else
-        hipLaunchKernelGGL((k_gd_cols<LEN, GD_LOSS>), dim3(grid), dim3(SizeCfg<LEN>::TPB), 0, st, Y, A, amp_batch, W, invM, step, part, remap);
+        hipLaunchKernelGGL((k_gd_cols<LEN, GD_LOSS, TL>), dim3(grid), dim3(SizeCfg<LEN>::TPB), 0, st, Y, A, amp_batch, W, invM, step, part, remap);
+}
+
+template <int LEN>
+static void gd_launch(hipStream_t st, int mode, unsigned grid, c2<float>* Y, const float* A,
+                      int64_t amp_batch, int64_t W, float invM, float step, float* part, int remap,
+                      bool tiled)
+{
+    if (tiled) gd_launch_t<LEN, true>(st, mode, grid, Y, A, amp_batch, W, invM, step, part, remap);
+    else gd_launch_t<LEN, false>(st, mode, grid, Y, A, amp_batch, W, invM, step, part, remap);
 }
 
 static int line_tile(int len)
@@ -304,8 +319,21 @@ static void gd_run(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batc
     CPRA_ARG(tiles < (1ll << 31), "%s: batch too large", who);
     hipStream_t st = h->stream;
     c2<float>* Y = static_cast<c2<float>*>(x);
+    const float* Ause = amp;
+    // 2D square patterns: the state lives tile-major in a workspace (the
+    // column pass then reads and writes contiguous tiles), with a tile-major
+    // copy of the amplitudes; the tuning key tiled=0 keeps it in x
+    const bool tiled = d.rank == 2 && d.M == d.N && engine_tiled_state(h);
     // enter: Y = centred DFT over the inner axes
-    if (d.rank == 2) {
+    if (tiled) {
+        const size_t px = (size_t)d.M * d.N;
+        h->ws_state.ensure(sizeof(c2<float>) * px * (size_t)B);
+        h->ws_amp_t.ensure(sizeof(float) * px * amp_batch);
+        Y = static_cast<c2<float>*>(h->ws_state.ptr);
+        Ause = static_cast<const float*>(h->ws_amp_t.ptr);
+        engine_amp_retile(st, d.N, (int64_t)amp_batch, amp, static_cast<float*>(h->ws_amp_t.ptr));
+        engine_retile(st, d.N, B, -1, x, Y);
+    } else if (d.rank == 2) {
         fft_pass_contig(h, x, d.N, B * d.M, -1, true, 1.0, false);
     } else {
         fft_pass_contig(h, x, d.N, B * d.L * d.M, -1, true, 1.0, false);
@@ -318,17 +346,19 @@ static void gd_run(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batc
     const float invM = 1.0f / (float)len;
     const float step = (float)(2.0 * (double)mu * (double)W);  // 2 mu * (size of the inner axes)
     const unsigned grid = (unsigned)tiles;
-    const int remap = (grid % 8u == 0u) ? 1 : 0;
+    // XCD-aware tile order helps the strided row-major tiles; contiguous
+    // tiles stream best in dispatch order
+    const int remap = (!tiled && grid % 8u == 0u) ? 1 : 0;
     KernelTimer kt;
     auto launch = [&](int mode, float stp) {
         kt.begin(h, st, KT_GD);
         switch (len) {
-        case 64: gd_launch<64>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
-        case 128: gd_launch<128>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
-        case 192: gd_launch<192>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
-        case 256: gd_launch<256>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
-        case 384: gd_launch<384>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
-        case 512: gd_launch<512>(st, mode, grid, Y, amp, (int64_t)amp_batch, W, invM, stp, part, remap); break;
+        case 64: gd_launch<64>(st, mode, grid, Y, Ause, (int64_t)amp_batch, W, invM, stp, part, remap, tiled); break;
+        case 128: gd_launch<128>(st, mode, grid, Y, Ause, (int64_t)amp_batch, W, invM, stp, part, remap, tiled); break;
+        case 192: gd_launch<192>(st, mode, grid, Y, Ause, (int64_t)amp_batch, W, invM, stp, part, remap, tiled); break;
+        case 256: gd_launch<256>(st, mode, grid, Y, Ause, (int64_t)amp_batch, W, invM, stp, part, remap, tiled); break;
+        case 384: gd_launch<384>(st, mode, grid, Y, Ause, (int64_t)amp_batch, W, invM, stp, part, remap, tiled); break;
+        case 512: gd_launch<512>(st, mode, grid, Y, Ause, (int64_t)amp_batch, W, invM, stp, part, remap, tiled); break;
         }
         check_launch("k_gd_cols");
         kt.end(h, st);
@@ -340,7 +370,9 @@ static void gd_run(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batc
         check_launch("k_loss_fin");
     }
     // leave: inverse over the inner axes, scaled by 1/(their size)
-    if (d.rank == 2) {
+    if (tiled) {
+        engine_retile(st, d.N, B, +1, Y, x);
+    } else if (d.rank == 2) {
         fft_pass_contig(h, x, d.N, B * d.M, +1, true, 1.0 / (double)d.N, false);
     } else {
         fft_pass_contig(h, x, d.N, B * d.L * d.M, +1, true, 1.0, false);

@@ -185,6 +185,13 @@ enum KernelKind {
     KT_MIXED = 8, KT_PENCIL = 9
 };
 
+// engine2d.hip: tile-major layout of square patterns (Tiled<N> in
+// engine_kernels.hpp): whether the handle uses it, the row pass between the
+// row-major x and the tiled state (dir -1: x -> DFT_r into the tiled state;
+// +1: back, scaled 1/n), and the tiled copy of amplitudes
+bool engine_tiled_state(const cpra_hip_ctx* h);
+void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void* in, void* out);
+void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out);
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)
 void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support);
 void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,

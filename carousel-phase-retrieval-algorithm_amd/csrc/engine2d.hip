@@ -362,6 +362,11 @@ static void launch_retile(hipStream_t st, const Geo& g, int dir, const void* in,
     check_launch("k_rows_retile");
 }
 
+void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void* in, void* out)
+{
+    launch_retile(st, Geo{n, n, patterns}, dir, in, out);
+}
+
 static void launch_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
 {
     const int64_t total = patterns * n * n;
@@ -505,6 +510,13 @@ static bool tiled_state(const cpra_hip_ctx* h)
     if (h->m != h->n) return false;
     auto it = h->tuning.find((uint64_t)4 << 32);
     return it == h->tuning.end() || it->second != 0;
+}
+
+bool engine_tiled_state(const cpra_hip_ctx* h) { return tiled_state(h); }
+
+void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
+{
+    launch_amp_retile(st, n, patterns, in, out);
 }
 
 static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,

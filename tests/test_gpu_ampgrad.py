@@ -236,3 +236,24 @@ def test_torch_autograd_harness(gpu):
                        capture_output=True, text=True, timeout=600)
     print(r.stdout, r.stderr[-3000:])
     assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+
+
+@pytest.mark.parametrize("n", [64, 256, 384, 512])
+def test_fused_gd_tiled_state_matches_row_major(hipmod, gpu, n):
+    """The 2D descent keeps its state tile-major by default (square patterns);
+    the tuning key tiled=0 keeps it row-major in x.  Same arithmetic per
+    column, so the two agree to FP32 rounding (the entry and exit row passes
+    are separate kernels), the loss included."""
+    B = 2
+    x, A = disc_problem(B, n, 40 + n, noise=0.3, P=B)
+    A32 = A.astype(np.float32)
+    mu = 0.5 / float((A32.astype(float) ** 2).max())
+    outs = []
+    for tl in (0, 1):
+        xd = hipmod.to_device(x.astype(np.complex64))
+        with hipmod.HipImpl(n, n, 1, B) as h:
+            assert h.set_tuning(f"tiled={tl};")
+            L = h.amp_gd_run(xd, hipmod.to_device(A32), 8, mu)
+        outs.append((xd.numpy(), L))
+    assert relerr(outs[1][0], outs[0][0]) < 2e-6
+    assert np.abs(outs[1][1] - outs[0][1]).max() <= 1e-5 * np.abs(outs[0][1]).max()
