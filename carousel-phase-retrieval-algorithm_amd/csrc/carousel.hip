@@ -21,7 +21,8 @@
 // (the row transforms cancel against the HIO engine's).  Step 4 is five passes,
 // not seven: the z-line kernel runs IDFT_z, the constraint and DFT_z on each
 // line in registers, between the inverse and forward (y, x) passes.  The
-// outer body (12 launches for inner = 1) is captured once in a hipGraph and
+// outer body (12 launches for inner = 1; 10 for n <= 128, where one kernel per
+// direction transforms whole z-planes in LDS) is captured once in a hipGraph and
 // replayed; its working set (the projections, the transposed slices and the
 // volume: 56 MB at 180 x 128^2 and 128^3) stays in the Infinity Cache.
 #include <cmath>
@@ -87,6 +88,109 @@ static int zline_tile(int n)
     return 0;
 }
 
+// Centred 2D transform along x and y of every z-plane of the volume [N][N][N],
+// one workgroup per plane, for N <= 128, where an N x N complex64 plane fits in
+// LDS (136 KiB at 128 with the padding below).  This replaces two volume
+// passes with one cache round trip.
+// LDS layouts, with row span SP = N + N/16 and the LdsRow padding of one value
+// per 16 (pad(i) = i + i/16):
+//   NAT(y, x) = y*SP + pad(x)  (x-lines contiguous)
+//   TRN(y, x) = x*SP + pad(y)  (y-lines contiguous)
+// The plane is staged through LDS with whole-row (512-byte) global accesses;
+// loading each line straight into registers (64-byte segments) measured
+// slower.  An axis step reads each line from the layout where it is
+// contiguous.  The line's T threads sit in one wave, so the FFT exchanges in
+// place within the line's own span with wave-level ordering only.  After a
+// workgroup barrier the step writes the other layout, or its own span when no
+// transpose follows.  With 8 lines x 8 threads per wave at N = 128, both the
+// in-place reads and the transposed writes touch 64 distinct 8-byte slots
+// modulo 512 bytes.
+// XFIRST runs x then y, the order of the inverse passes; otherwise y then x,
+// the order of the forward passes.  Each line goes through the same plan,
+// twiddles and centring fold as in the one-axis kernels of fft.hip.
+template <int N, int DIR, bool XFIRST>
+__global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d(c2<float>* V)
+{
+    using Vt = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, TPB = N * TT, SP = N + N / 16, K = N * N / TPB;
+    static_assert(64 % TT == 0 && TPB % 64 == 0 && N % 8 == 0, "plane layout");
+    __shared__ Vt lds[N * SP];
+    const int tid = threadIdx.x;
+    Vt* Pp = V + (size_t)blockIdx.x * (N * N);
+    auto pad = [](int i) { return i + (i >> 4); };
+
+    Vt g[K];
+    if constexpr (XFIRST) {
+        // coalesced rows into NAT
+        sfor<K>([&](auto kc) { constexpr int k = decltype(kc)::value; g[k] = Pp[tid + k * TPB]; });
+        sfor<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const int i = tid + k * TPB;
+            lds[(i / N) * SP + pad(i % N)] = g[k];
+        });
+    } else {
+        // 8 x 8 blocks (8 consecutive x per row, 8 rows per wave) into TRN
+        const int wv = tid >> 6, xl = tid & 7, yl = (tid >> 3) & 7;
+        auto blk_y = [&](int k) { return ((wv + k * (TPB / 64)) / (N / 8)) * 8 + yl; };
+        auto blk_x = [&](int k) { return ((wv + k * (TPB / 64)) % (N / 8)) * 8 + xl; };
+        sfor<K>([&](auto kc) { constexpr int k = decltype(kc)::value; g[k] = Pp[blk_y(k) * N + blk_x(k)]; });
+        sfor<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            lds[blk_x(k) * SP + pad(blk_y(k))] = g[k];
+        });
+    }
+    __syncthreads();
+
+    const int t = tid % TT, line = tid / TT;
+    Vt* lb = lds + line * SP;
+    auto axis = [&](bool transpose_out) {
+        Vt v[E];
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            v[m] = lb[pad(t + PL::rot(m) * TT)];
+        });
+        LdsRow<Vt, N, true> acc{lb};
+        fft_line<PL, DIR>(v, t, g_tw<N>(), acc);
+        __syncthreads();  // every exchange done before the transposed writes
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const int pos = t + PL::rot(m) * TT;
+            if (transpose_out) lds[pos * SP + pad(line)] = v[m];
+            else lb[pad(pos)] = v[m];
+        });
+        __syncthreads();
+    };
+    // XFIRST: NAT -> (x) -> TRN -> (y) -> NAT; else TRN -> (y) -> NAT -> (x) -> NAT
+    axis(true);
+    axis(XFIRST);
+
+    sfor<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int i = tid + k * TPB;
+        Pp[i] = lds[(i / N) * SP + pad(i % N)];
+    });
+}
+
+// true when the plane kernel covers size n (the plane fits in LDS)
+static bool plane2d_launch(hipStream_t st, int n, int dir, c2<float>* V)
+{
+    const dim3 grid((unsigned)n);
+    switch (n) {
+    case 64:
+        if (dir > 0) hipLaunchKernelGGL((k_plane2d<64, +1, true>), grid, dim3(64 * SizeCfg<64>::P::T), 0, st, V);
+        else hipLaunchKernelGGL((k_plane2d<64, -1, false>), grid, dim3(64 * SizeCfg<64>::P::T), 0, st, V);
+        break;
+    case 128:
+        if (dir > 0) hipLaunchKernelGGL((k_plane2d<128, +1, true>), grid, dim3(128 * SizeCfg<128>::P::T), 0, st, V);
+        else hipLaunchKernelGGL((k_plane2d<128, -1, false>), grid, dim3(128 * SizeCfg<128>::P::T), 0, st, V);
+        break;
+    default: return false;
+    }
+    check_launch("k_plane2d");
+    return true;
+}
+
 template <int L>
 static void zline_n(hipStream_t st, int mode, c2<float>* V, const uint8_t* S, int64_t W, float scale)
 {
@@ -126,13 +230,17 @@ struct Carousel {
     int inner;
 
     // inverse (x, y) passes, fused z-line constraint, forward (y, x) passes
+    // (n <= 128: one whole-plane kernel per direction for the x and y passes)
     void volume_constraint(int mode)
     {
         const int64_t W = (int64_t)n * n;
-        fft_pass_contig(h, vol, n, (int64_t)n * n, +1, true, 1.0, false);            // x
-        fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, +1, true, 1.0, false);  // y
+        const bool plane = engine_plane_fft(h);
+        if (!(plane && plane2d_launch(h->stream, n, +1, vol))) {
+            fft_pass_contig(h, vol, n, (int64_t)n * n, +1, true, 1.0, false);            // x
+            fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, +1, true, 1.0, false);  // y
+        }
         zline_launch(h->stream, n, mode, vol, sup3d);                                // z
-        if (mode == ZL_ROUNDTRIP) {
+        if (mode == ZL_ROUNDTRIP && !(plane && plane2d_launch(h->stream, n, -1, vol))) {
             fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, -1, true, 1.0, false);
             fft_pass_contig(h, vol, n, (int64_t)n * n, -1, true, 1.0, false);
         }

@@ -190,6 +190,8 @@ enum KernelKind {
 // row-major x and the tiled state (dir -1: x -> DFT_r into the tiled state;
 // +1: back, scaled 1/n), and the tiled copy of amplitudes
 bool engine_tiled_state(const cpra_hip_ctx* h);
+// whole-plane x/y transforms in the carousel's volume constraint (tuning plane=)
+bool engine_plane_fft(const cpra_hip_ctx* h);
 void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void* in, void* out);
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out);
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)

@@ -514,6 +514,14 @@ static bool tiled_state(const cpra_hip_ctx* h)
 
 bool engine_tiled_state(const cpra_hip_ctx* h) { return tiled_state(h); }
 
+// Whole-plane 2D transforms in the carousel's volume constraint (n <= 128),
+// tuning key 5 (1 = on, the default; 0 = one pass per axis).
+bool engine_plane_fft(const cpra_hip_ctx* h)
+{
+    auto it = h->tuning.find((uint64_t)5 << 32);
+    return it == h->tuning.end() || it->second != 0;
+}
+
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
 {
     launch_amp_retile(st, n, patterns, in, out);
@@ -1205,8 +1213,8 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
     });
 }
 
-// Applies "col_<M>=<variant>;" entries of a previous autotune report (other
-// keys are ignored) and writes the current choices back when out != NULL.
+// Applies the col_<M>=, rows_sparse=, chunk=, tiled= and plane= entries of a
+// spec such as an autotune report; other keys are skipped.
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
 {
     return eguard(h, [&]() -> int {
@@ -1223,6 +1231,9 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
                 p += used;
             } else if (std::sscanf(p, "tiled=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)4 << 32] = v ? 1 : 0;
+                p += used;
+            } else if (std::sscanf(p, "plane=%d;%n", &v, &used) == 1 && used > 0) {
+                h->tuning[(uint64_t)5 << 32] = v ? 1 : 0;
                 p += used;
             } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
                 CPRA_ARG(col_variant_valid(v), "cpra_hip_set_tuning: variant %d out of range", v);
@@ -1243,9 +1254,9 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
     return eguard(h, [&]() -> int {
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
-        std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;", (int)h->m,
+        std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;", (int)h->m,
                       col_variant_code(cv), sparse_rows_enabled(h) ? 1 : 0,
-                      (long long)iteration_chunk(h), tiled_state(h) ? 1 : 0);
+                      (long long)iteration_chunk(h), tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0);
         return CPRA_HIP_OK;
     });
 }

@@ -109,11 +109,14 @@ int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t*
  * report (may be NULL). */
 int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                          uint8_t* support, int iters, char* report, uint64_t report_len);
-/* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1>;" and "chunk=<patterns>;"
- * choices of an autotune report (chunk: patterns per launch of the iteration
- * kernels, 0 = whole batch). */
+/* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1>;", "chunk=<patterns>;"
+ * and "tiled=<0|1>;" choices of an autotune report (chunk: patterns per launch
+ * of the iteration kernels, 0 = whole batch; tiled: tile-major state for
+ * square patterns), and "plane=<0|1>;" (whole-plane LDS transforms in the
+ * carousel's volume constraint for n <= 128; 0 = one pass per axis). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
-/* Current choices as "col_<m>=<v>;rows_sparse=<0|1>;chunk=<patterns>;". */
+/* Current choices as
+ * "col_<m>=<v>;rows_sparse=<0|1>;chunk=<patterns>;tiled=<0|1>;plane=<0|1>;". */
 int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 
 /* Per-kernel timing of the engines: when enabled, engine runs launch
@@ -121,7 +124,7 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
  * stream it runs on; cpra_hip_get_kernel_times returns, per kernel kind
  * (0 = column pass, 1 = row pass, 2 = one shrinkwrap update, 3 = fused
  * gradient-descent pass, 4 = 2D->3D merge, 5 = 3D volume constraint (its
- * five FFT passes), 6 = slice extraction, 7 = column pass of the carousel's
+ * five FFT passes, three kernels for n <= 128), 6 = slice extraction, 7 = column pass of the carousel's
  * projections, 8 = one whole mixed-batch iteration of the per-class and
  * per-pattern modes; the fused mode records 0 and 1; 9 = the y-line pencil
  * passes of the 3D difference map), the summed milliseconds

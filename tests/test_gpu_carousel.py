@@ -25,11 +25,13 @@ def synth(pkg):
     return synth
 
 
-def run_gpu(hipmod, d, outer, inner=1, use_graph=1, want_rfactor=False):
+def run_gpu(hipmod, d, outer, inner=1, use_graph=1, want_rfactor=False, tuning=None):
     P, n = d["x0"].shape[0], d["x0"].shape[-1]
     x = hipmod.to_device(d["x0"])
     vol = hipmod.zeros((n, n, n), np.complex64)
     with hipmod.HipImpl(n, n, 1, P) as h:
+        if tuning:
+            assert h.set_tuning(tuning)
         rf = h.carousel3d_run(x, hipmod.to_device(d["amp"]), hipmod.to_device(d["support2d"]),
                               hipmod.to_device(d["angles"]), hipmod.to_device(d["support3d"]), vol,
                               outer_iters=outer, inner_iters=inner, beta=0.9, use_graph=use_graph,
@@ -61,6 +63,24 @@ def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth):
     xe, ve, _ = run_gpu(hipmod, d, 6, use_graph=0)
     assert np.array_equal(xg, xe)
     assert np.array_equal(vg, ve)
+
+
+@pytest.mark.parametrize("n,P", [(64, 36), (128, 60)])
+def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P):
+    """For n <= 128 the volume constraint's x and y transforms run as one
+    whole-plane LDS kernel per direction (tuning plane=1, the default); plane=0
+    runs the one-axis passes.  Each line goes through the same plan and
+    twiddles, so the results agree to rounding after several iterations."""
+    d = synth.make_carousel(n, P, config=53)
+    x1, v1, _ = run_gpu(hipmod, d, 3, use_graph=0, tuning="plane=1;")
+    x0, v0, _ = run_gpu(hipmod, d, 3, use_graph=0, tuning="plane=0;")
+    ex, ev = relerr(x1, x0), relerr(v1, v0)
+    print(f"plane vs passes n={n}: x {ex:.2e} vol {ev:.2e} bitwise={np.array_equal(x1, x0)}")
+    assert ex < 1e-5 and ev < 1e-5
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        assert h.get_tuning()["plane"] == "1"
+        assert h.set_tuning("plane=0;")
+        assert h.get_tuning()["plane"] == "0"
 
 
 def test_carousel_rfactor_falls(hipmod, gpu, synth):
