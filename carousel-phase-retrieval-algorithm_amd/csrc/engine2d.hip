@@ -60,6 +60,26 @@ __global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
         p[i] = v;
 }
 
+// mask = b >= frac * max_pattern(b), 16 pixels per thread and step: four
+// 16-byte loads and one 16-byte store (per % 16 == 0, both arrays 16-byte
+// aligned); k_sw_thresh below takes unaligned caller buffers
+__global__ __launch_bounds__(256) void k_sw_thresh16(const float4* __restrict__ bl, const unsigned* __restrict__ keys,
+                                                     uint4* __restrict__ mask, int64_t per16, int64_t n16, float frac)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n16;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const float thr = frac * key2f(keys[v / per16]);
+        unsigned w[4];
+        sfor<4>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const float4 b = bl[v * 4 + q];
+            w[q] = (b.x >= thr ? 1u : 0u) | (b.y >= thr ? 1u << 8 : 0u) | (b.z >= thr ? 1u << 16 : 0u) |
+                   (b.w >= thr ? 1u << 24 : 0u);
+        });
+        mask[v] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 // mask = b >= frac * max_pattern(b)
 __global__ void k_sw_thresh(const float* __restrict__ bl, const unsigned* __restrict__ keys,
                             uint8_t* __restrict__ mask, int64_t per, int64_t total, float frac)
@@ -445,7 +465,13 @@ static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, 
     launch_cols(h, st, g, COL_BLUR, W, W, nullptr, 1, scale, 0, nullptr, gr, gc);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(g.B, 256)), dim3(256), 0, st, keys, 0u, g.B);
     launch_sw_rows(st, g, 3, false, W, nullptr, bl, keys);
-    hipLaunchKernelGGL(k_sw_thresh, dim3(grid_1d(total, 256, 65536)), dim3(256), 0, st, bl, keys, mask, px, total, frac);
+    if (px % 16 == 0 && reinterpret_cast<uintptr_t>(bl) % 16 == 0 && reinterpret_cast<uintptr_t>(mask) % 16 == 0)
+        hipLaunchKernelGGL(k_sw_thresh16, dim3(grid_1d(total / 16, 256, 65536)), dim3(256), 0, st,
+                           reinterpret_cast<const float4*>(bl), keys, reinterpret_cast<uint4*>(mask), px / 16,
+                           total / 16, frac);
+    else
+        hipLaunchKernelGGL(k_sw_thresh, dim3(grid_1d(total, 256, 65536)), dim3(256), 0, st, bl, keys, mask, px,
+                           total, frac);
     check_launch("k_sw_thresh");
 }
 
