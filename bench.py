@@ -49,11 +49,12 @@ def log(msg):
 def measured_traffic(kernel: str, tuning: dict):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known traffic), taken
-    with the same column variant and patterns per launch as `tuning`."""
+    with the same column variant, patterns per launch and row mode as `tuning`
+    (on the initial support of the workload, as the event-timed iterations)."""
     path = os.path.join(ROOT, "profiles", "hio_512x256sq_pmc.md")
     if not os.path.exists(path):
         return None
-    want = ";".join(f"{k}={tuning.get(k)}" for k in ("col_256", "chunk"))
+    want = ";".join(f"{k}={tuning.get(k)}" for k in ("col_256", "chunk", "rows_sparse_used"))
     for line in open(path):
         parts = line.split()
         if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel
@@ -698,22 +699,33 @@ def main():
     its = args.steps / elapsed
     log(f"bench: {args.steps} iterations in {elapsed:.3f} s -> {its:.1f} it/s")
 
-    # per-kernel durations with HIP events (eager launches, same workload)
+    # per-kernel durations with HIP events (eager launches, same workload),
+    # from the initial state, so that the support (and with it the row mode
+    # and the rows the row kernel processes) is the same in every run
+    xt = hip.to_device(data["x0"])
+    st_ = hip.to_device(data["support"])
     h.set_kernel_timing(True)
-    h.hio_run(x, amp, sup, iters=args.timing_iters, use_graph=0, **sched)
+    h.hio_run(xt, amp, st_, iters=args.timing_iters, use_graph=0, **sched)
     h.Sync()
     kt = h.kernel_times()
     h.set_kernel_timing(False)
+    xt.free()
+    st_.free()
     px = P * n * n
     tuning = h.get_tuning()
+    # sparse rows: the row kernel reads and writes only the rows that hold a
+    # support pixel (the column kernel finishes the others)
+    row_frac = float(data["support"].reshape(P, n, n).any(axis=2).mean())
+    sparse_used = tuning.get("rows_sparse_used") == "1"
+    row_bpp = ROW_BYTES_PER_PX * (row_frac if sparse_used else 1.0)
     kern = {}
-    for name, bpp in (("column", COL_BYTES_PER_PX), ("row", ROW_BYTES_PER_PX)):
+    for name, bpp in (("column", COL_BYTES_PER_PX), ("row", row_bpp)):
         ms, cnt = kt[name]
         per_iter = ms / args.timing_iters            # all launches of one iteration
         per_launch = ms / max(cnt, 1)
         ach = px * bpp / (per_iter * 1e-3) / 1e9
         kern[name] = {"avg_ms": per_launch, "launches": cnt, "ms_per_iteration": per_iter,
-                      "bytes_per_launch": px * bpp * args.timing_iters // max(cnt, 1),
+                      "bytes_per_launch": int(px * bpp * args.timing_iters // max(cnt, 1)),
                       "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS}
     sw_ms, sw_n = kt["shrinkwrap"]
     dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
@@ -723,6 +735,7 @@ def main():
             "traffic": None, "kernel": f"k_eng_{'cols' if dom == 'column' else 'rows_hio'}<{n}>",
             "kernels": kern,
             "shrinkwrap_update_ms": sw_ms / max(sw_n, 1),
+            "rows": {"sparse": sparse_used, "support_row_fraction": round(row_frac, 4)},
             "tuning": tuning}
     # counters are per launch of the 512 x 256^2 workload at a given tuning
     if P == 512 and n == 256:

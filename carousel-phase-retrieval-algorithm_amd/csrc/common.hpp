@@ -91,7 +91,12 @@ struct cpra_hip_ctx {
     int64_t mplan_entries = 0;
     // tuning choice per (kind, n) set by the autotuner (0 = default)
     std::map<uint64_t, int> tuning;
-    cpra::EngineGraph* graph = nullptr;
+    // captured HIO chunk bodies: [0] dense rows, [1] sparse rows
+    cpra::EngineGraph* graph[2] = {nullptr, nullptr};
+    // sparse rows for engine_hio_iteration (decided by engine_hio_prepare)
+    bool hio_sparse = false;
+    // row mode of the last chunk of the last hio_run (1 = sparse)
+    int hio_sparse_used = 0;
     // graphs captured from the handle's stream by the caller
     // (cpra_hip_graph_begin / _end); engines run eagerly while capturing
     bool capturing = false;

@@ -331,20 +331,35 @@ __global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
     __shared__ V lds[LPB * SPAN];
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    bool valid = EXACT || line < nlines;
+    // Sparse rows of tile-major patterns: consecutive blocks (which the
+    // dispatcher deals to different XCDs) take the same row block of
+    // consecutive patterns, so every XCD gets its share of the support rows
+    // whatever their position in the pattern.  The grid is whole patterns.
+    unsigned vb = blockIdx.x;
+    if constexpr (SPARSE && TILED) {
+        constexpr unsigned BPP = N / LPB;
+        const unsigned npat = gridDim.x / BPP;
+        vb = (blockIdx.x % npat) * BPP + blockIdx.x / npat;
+    }
+    const int64_t line = (int64_t)vb * LPB + ll;
+    // inb: the line exists (loads); valid: it is also written (sparse rows
+    // skip the lines the column kernel finished).  The loads test inb only,
+    // which is a compile-time true for EXACT grids: per-lane conditions on
+    // them cost the scheduler its batching of the loads (a third slower).
+    const bool inb = EXACT || line < nlines;
+    bool valid = inb;
     if constexpr (SPARSE) {
         valid = valid && ((rowbits[line >> 5] >> (line & 31)) & 1u);
         if (__ballot(valid) == 0ull) return;
     }
     static_assert(!TILED || (EXACT && N % LPB == 0), "tiled rows: whole blocks inside a pattern");
     // block-uniform base, per-lane 32-bit offsets
-    const size_t bbase = (size_t)blockIdx.x * LPB * N;
+    const size_t bbase = (size_t)vb * LPB * N;
     const int lo = ll * N + t;
     // tiled state: pattern and first row of the block; element m at the
     // register base kb(m) plus the lane offset sl
-    const size_t pbase = TILED ? ((size_t)blockIdx.x * LPB / N) * N * N
-                                     + (((size_t)blockIdx.x * LPB) % N) * TL::C
+    const size_t pbase = TILED ? ((size_t)vb * LPB / N) * N * N
+                                     + (((size_t)vb * LPB) % N) * TL::C
                                : bbase;
     const unsigned sl = TILED ? TL::lane(t, ll) : (unsigned)lo;
     auto kb = [&](auto mc) -> size_t {
@@ -365,13 +380,13 @@ __global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
         const int o = lo + PL::rot(m) * TT;
         if constexpr (TILED) {
             constexpr unsigned kbytes = (unsigned)(TL::template kpart<PL::rot(m)>() * sizeof(V));
-            xp[m] = valid ? zr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
-            xo[m] = valid ? yr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
+            xp[m] = inb ? zr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
+            xo[m] = inb ? yr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
         } else {
-            xp[m] = valid ? (Zb + kb(mc))[sl] : mk<V>(0, 0);
-            xo[m] = valid ? (Yb + kb(mc))[sl] : mk<V>(0, 0);
+            xp[m] = inb ? (Zb + kb(mc))[sl] : mk<V>(0, 0);
+            xo[m] = inb ? (Yb + kb(mc))[sl] : mk<V>(0, 0);
         }
-        sm[m] = valid ? Sb[o] : (uint8_t)0;
+        sm[m] = inb ? Sb[o] : (uint8_t)0;
     });
     fft_line<PL, +1>(xp, t, tw, acc);
     fft_line<PL, +1>(xo, t, tw, acc);

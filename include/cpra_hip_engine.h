@@ -109,14 +109,17 @@ int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t*
  * report (may be NULL). */
 int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                          uint8_t* support, int iters, char* report, uint64_t report_len);
-/* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1>;", "chunk=<patterns>;"
- * and "tiled=<0|1>;" choices of an autotune report (chunk: patterns per launch
- * of the iteration kernels, 0 = whole batch; tiled: tile-major state for
- * square patterns), and "plane=<0|1>;" (whole-plane LDS transforms in the
- * carousel's volume constraint for n <= 128; 0 = one pass per axis). */
+/* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1|2>;", "chunk=<patterns>;"
+ * and "tiled=<0|1>;" choices of an autotune report (rows_sparse: dense rows,
+ * sparse rows, or adaptive, the default: sparse while at most 80 % of the rows
+ * hold a support pixel, decided after every shrinkwrap update; chunk: patterns
+ * per launch of the iteration kernels, 0 = whole batch; tiled: tile-major
+ * state for square patterns), and "plane=<0|1>;" (whole-plane LDS transforms
+ * in the carousel's volume constraint for n <= 128; 0 = one pass per axis). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
-/* Current choices as
- * "col_<m>=<v>;rows_sparse=<0|1>;chunk=<patterns>;tiled=<0|1>;plane=<0|1>;". */
+/* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
+ * tiled=<0|1>;plane=<0|1>;rows_sparse_used=<0|1>;" (the last: the row mode
+ * of the last chunk of the last cpra_hip_hio_run). */
 int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 
 /* Per-kernel timing of the engines: when enabled, engine runs launch

@@ -317,6 +317,62 @@ def test_tiled_state_matches_row_major(hipmod, gpu, synth, n):
     assert np.sum(t["sup"] != r["sup"]) <= max(1, t["sup"].size // 10000)
 
 
+@pytest.mark.parametrize("n", [256, 192])
+def test_adaptive_rows_follow_the_support(hipmod, gpu, synth, n):
+    """rows_sparse=2 (the default) runs sparse rows while at most 80 % of the
+    rows hold a support pixel and dense rows otherwise, deciding again after
+    every shrinkwrap update: its results are bit for bit those of the forced
+    mode it picks, and rows_sparse_used reports the choice."""
+    P = 3
+    data = synth.make_2d_batch(P, n, n / 10, n / 7, 1e6, config=133)
+    tight = (data["obj"] > 0).astype(np.uint8)          # the object discs: few rows
+    full = np.ones_like(tight)                           # every row
+    assert tight.any(axis=2).mean() < 0.8
+    amp = hipmod.to_device(data["amp"])
+    kw = dict(beta=0.9, real_positive=1, sw_every=0)
+
+    def run(mode, support, graph):
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            if mode is not None:
+                assert h.set_tuning(f"rows_sparse={mode};")
+            assert h.get_tuning()["rows_sparse"] == str(2 if mode is None else mode)
+            x = hipmod.to_device(data["x0"])
+            h.hio_run(x, amp, hipmod.to_device(support), iters=4, use_graph=graph, **kw)
+            h.Sync()
+            return x.numpy(), h.get_tuning()["rows_sparse_used"]
+
+    for graph in (0, 1):
+        xa, used = run(None, tight, graph)
+        assert used == "1"
+        assert np.array_equal(xa, run(1, tight, graph)[0])
+        xa, used = run(None, full, graph)
+        assert used == "0"
+        assert np.array_equal(xa, run(0, full, graph)[0])
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        with pytest.raises(hipmod.CpraHipError):
+            h.set_tuning("rows_sparse=3;")
+
+
+def test_adaptive_rows_through_shrinkwrap_match_dense(hipmod, gpu, synth):
+    """A schedule whose support changes at every update (the adaptive mode
+    switching between its graphs) agrees with dense rows to FP32 rounding over
+    a short run (the sparse update of the empty rows rounds differently)."""
+    n, P = 128, 4
+    data = synth.make_2d_batch(P, n, 15, 25, 1e6, config=134)
+    amp = hipmod.to_device(data["amp"])
+    out = {}
+    for mode in (0, 2):
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            assert h.set_tuning(f"rows_sparse={mode};")
+            x = hipmod.to_device(data["x0"])
+            sup = hipmod.to_device(data["support"])
+            h.hio_run(x, amp, sup, iters=4, beta=0.9, real_positive=1, sw_every=2, use_graph=1)
+            h.Sync()
+            out[mode] = (x.numpy(), sup.numpy())
+    assert relerr(out[2][0], out[0][0]) < 1e-4
+    assert np.mean(out[2][1] != out[0][1]) < 1e-3
+
+
 def test_tiled_state_rejected_for_non_square(hipmod, gpu, synth):
     """Non-square patterns keep the row-major state whatever the tuning says."""
     with hipmod.HipImpl(64, 128, 1, 2) as h:
