@@ -743,7 +743,7 @@ def main():
         if tb is not None:
             roof["traffic"] = tb
             roof["traffic_source"] = ("profiles/hio_512x256sq_pmc.md (rocprofv3 FETCH_SIZE + "
-                                      "WRITE_SIZE, same column variant and chunk)")
+                                      "WRITE_SIZE, same column variant, chunk and row mode)")
     iter_algo_frac = px * HIO_ALGO_BYTES_PER_PX * its / 1e9 / HBM_PEAK_GBS
     h.close()
 
