@@ -40,7 +40,12 @@ void cpra_hip_hio_default_params(cpra_hip_hio_params* p);
  * After iteration i (1-based) with i % sw_every == 0, update k = i/sw_every-1
  * of K = iters/sw_every uses sigma_k = sigma0 + (sigma1 - sigma0) k/(K-1).
  * If rfactor_host != NULL it receives, per pattern, the R-factor
- * sum_{A>0} | |Forward2D(x)| - A | / sum_{A>0} A of the final x. */
+ * sum_{A>0} | |Forward2D(x)| - A | / sum_{A>0} A of the final x.
+ * In the adaptive row mode (tuning rows_sparse=2, the default) the call
+ * synchronises the handle's stream at its start and after every shrinkwrap
+ * update, to read back how many rows hold support; rows_sparse=0 or 1 keeps
+ * it asynchronous, and inside a stream capture the adaptive mode runs dense
+ * rows without synchronising. */
 int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                      uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host);
 
