@@ -29,7 +29,8 @@
 // primitive calls per iteration instead (:47-62); --graph records Step B's
 // sweep once as a graph and replays it EPI times.  Additions: --size M
 // (default 512; M = N = L, P = 9M/16) for short runs, --seed S (the
-// reference draws from unseeded generators), --primitives, --graph.
+// reference draws from unseeded generators), --tuning SPEC (kernel variants,
+// as cpra_hip_set_tuning takes them), --primitives, --graph.
 #include <chrono>
 #include <complex>
 #include <cstdio>
@@ -55,7 +56,14 @@ struct Config {
     bool primitives = false;
     bool graph = false;
     unsigned long long seed = 0;
+    std::string tuning;  // cpra_hip_set_tuning spec for the engines' handles
 };
+
+void apply_tuning(const Config& cfg, cpra_hip_handle h)
+{
+    if (!cfg.tuning.empty() && cpra_hip_set_tuning(h, cfg.tuning.c_str()) < 0)
+        throw std::runtime_error(std::string("--tuning: ") + cpra_hip_last_error(h));
+}
 
 constexpr float BETA = 0.9f;
 
@@ -102,6 +110,7 @@ void cpra_sample(const Config& cfg, int epi, int iter)
     auto compute = NewHIPImpl<float>(M, N, L, 1);
     auto* hip = static_cast<HipImpl<float>*>(compute.get());
     cpra_hip_handle h = hip->handle();
+    apply_tuning(cfg, h);
     auto alloc = NewHIPAllocator();
     cf* rg = (cf*)alloc->allocate(sizeof(cf) * px * P);
     cf* t1 = (cf*)alloc->allocate(sizeof(cf) * px);
@@ -136,7 +145,9 @@ void cpra_sample(const Config& cfg, int epi, int iter)
         if (iters <= 0) return;
         if (fused) {
             dp.iters = iters;
-            dp.use_graph = iters > 16;  // short runs: eager launches beat a graph build
+            // eager launches: on a single 512 x 512 pattern they measured
+            // 18.6 us per iteration against 19.1-19.9 us for graph replays
+            dp.use_graph = 0;
             const int rc = cpra_hip_dm_run(h, rg + px * p, t1, t2, dataC + px * p, 1, mask + px * p, &dp);
             if (rc == CPRA_HIP_ERR_UNSUPPORTED) {
                 fused = false;
@@ -207,6 +218,7 @@ void cpra_sample(const Config& cfg, int epi, int iter)
     // Step D setup (:147-155)
     auto c3 = NewHIPImpl<float>(M, N, L, 1);
     cpra_hip_handle h3 = static_cast<HipImpl<float>*>(c3.get())->handle();
+    apply_tuning(cfg, h3);
     const uint64_t vox = M * N * L;
     cf* rg3 = (cf*)alloc->allocate(sizeof(cf) * vox);
     cf* cdc3 = (cf*)alloc->allocate(sizeof(cf) * vox);
@@ -245,6 +257,7 @@ void conventional_sample(const Config& cfg, int iter)
     auto compute = NewHIPImpl<float>(M, N, L, 1);
     CpraImpl<float>* c = compute.get();
     cpra_hip_handle h = static_cast<HipImpl<float>*>(c)->handle();
+    apply_tuning(cfg, h);
     auto alloc = NewHIPAllocator();
     cf* rg = (cf*)alloc->allocate(sizeof(cf) * vox);
     cf* t1 = (cf*)alloc->allocate(sizeof(cf) * vox);
@@ -297,8 +310,9 @@ void conventional_sample(const Config& cfg, int iter)
 [[noreturn]] void usage(const std::string& msg)
 {
     std::cerr << "cpra_benchmark: " << msg << "\n"
-              << "usage: cpra_benchmark [--size M] [--seed S] [--primitives] [--graph] EPI ITER   (CPRA benchmark)\n"
-              << "       cpra_benchmark [--size M] [--seed S] ITER                      (conventional benchmark)\n";
+              << "usage: cpra_benchmark [--size M] [--seed S] [--tuning SPEC] [--primitives] [--graph] EPI ITER\n"
+              << "                                                   (CPRA benchmark)\n"
+              << "       cpra_benchmark [--size M] [--seed S] [--tuning SPEC] ITER   (conventional benchmark)\n";
     std::exit(2);
 }
 
@@ -323,6 +337,9 @@ int main(int argc, char* argv[])
             if (a == "--size") cfg.M = (uint64_t)v;
             else cfg.seed = (unsigned long long)v;
             ++k;
+        } else if (a == "--tuning") {
+            if (k + 1 >= argc) usage("--tuning needs a spec such as col_512=1;");
+            cfg.tuning = argv[++k];
         } else if (a == "--primitives") {
             cfg.primitives = true;
         } else if (a == "--graph") {

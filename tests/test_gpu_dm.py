@@ -50,6 +50,28 @@ def test_dm_reference_variant_matches_oracle(hipmod, oracle, gpu, synth, n, P, B
     assert relerr(t2.numpy(), r2) < tol
 
 
+@pytest.mark.parametrize("iters", [3, 17, 18, 40])
+def test_dm_graph_replays_match_eager(hipmod, gpu, synth, iters):
+    """cpra_hip_dm_run with a graph replays a body of up to 16 iterations and
+    runs the remainder eagerly: bit for bit the eager run, for run lengths
+    below, at and past whole bodies."""
+    n, P, B = 128, 1, 2
+    data = synth.make_2d_batch(P, n, 15, 20, 1e6, config=161)
+    mask = hipmod.to_device(np.tile((data["obj"] > 0).astype(np.uint8), (B, 1, 1)))
+    amp = hipmod.to_device(data["amp"])
+    init = [starts(data, B, s) for s in (4, 5, 6)]
+    outs = []
+    for g in (0, 1):
+        psi, t1, t2 = (hipmod.to_device(a) for a in init)
+        with hipmod.HipImpl(n, n, 1, B) as h:
+            assert h.dm_run(psi, amp, mask, t1=t1, t2=t2, iters=iters, beta=0.9, textbook=0,
+                            space_mode=0, use_graph=g)
+            h.Sync()
+        outs.append([a.numpy() for a in (psi, t1, t2)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("space_mode", [0, 1, 2])
 def test_dm_textbook_steps_match_oracle(hipmod, oracle, gpu, synth, space_mode):
     n, P = 128, 3
