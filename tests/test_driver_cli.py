@@ -62,7 +62,7 @@ BENCH = os.path.join(BIN, "cpra_benchmark")
 
 
 @pytest.mark.parametrize("args", [[], ["1", "2", "3"], ["--size", "7", "1", "1"], ["x"],
-                                  ["--size"], ["1", "-2"]])
+                                  ["--size"], ["1", "-2"], ["--tuning"]])
 def test_benchmark_rejects_bad_arguments(pkg, args):
     """bin/cpra_benchmark (src/benchmark_mkl.cpp:256-282): one positional
     argument runs the conventional benchmark, two the CPRA benchmark; anything

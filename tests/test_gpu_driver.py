@@ -111,7 +111,7 @@ def test_conv_driver_matches_oracle(oracle, gpu, tmp_path, L, M, N):
 BENCH = os.path.join(ROOT, "carousel-phase-retrieval-algorithm_amd", "bin", "cpra_benchmark")
 
 
-@pytest.mark.parametrize("extra", [[], ["--primitives"], ["--graph"]])
+@pytest.mark.parametrize("extra", [[], ["--primitives"], ["--graph"], ["--tuning", "col_64=1;"]])
 def test_benchmark_protocol_runs(gpu, extra):
     """bin/cpra_benchmark EPI ITER (src/benchmark_mkl.cpp:15-188) at a reduced
     size prints the reference's step lines with positive times."""
@@ -128,6 +128,12 @@ def test_benchmark_protocol_runs(gpu, extra):
     times = [float(ln.split("elapsed time:")[1].rstrip("s")) for ln in out.splitlines()
              if ln.startswith("elapsed time:")]
     assert len(times) == 3 and all(t > 0 for t in times)
+
+
+def test_benchmark_rejects_bad_tuning(gpu):
+    r = subprocess.run([BENCH, "--size", "64", "--tuning", "col_64=9;", "1", "1"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1 and "--tuning" in r.stderr, (r.stdout, r.stderr)
 
 
 def test_conventional_benchmark_runs(gpu):
