@@ -74,8 +74,12 @@ __global__ __launch_bounds__(256) void k_mixed_cols(const MixedArgs a)
 #undef CPRA_MC
 }
 
+// Three waves per SIMD: the 384-point body alone would take 181 VGPRs (two
+// waves for every class); capped at 168 it spills 56 bytes per lane and the
+// batch of 128^2 to 384^2 runs 685 instead of 600 it/s.  The column kernel
+// keeps its 212 VGPRs: capped, it spills 196 bytes per lane and runs slower.
 template <int SPACE, bool LAST>
-__global__ __launch_bounds__(256) void k_mixed_dm_rows(const MixedArgs a)
+__global__ __launch_bounds__(256, 3) void k_mixed_dm_rows(const MixedArgs a)
 {
     extern __shared__ unsigned char smem[];
     c2<float>* lds = reinterpret_cast<c2<float>*>(smem);
