@@ -95,8 +95,8 @@ struct cpra_hip_ctx {
     cpra::EngineGraph* graph[2] = {nullptr, nullptr};
     // sparse rows for engine_hio_iteration (decided by engine_hio_prepare)
     bool hio_sparse = false;
-    // row mode of the last chunk of the last hio_run (1 = sparse)
-    int hio_sparse_used = 0;
+    // the support rows have been listed (the adaptive mode's flag is valid)
+    bool rows_listed = false;
     // graphs captured from the handle's stream by the caller
     // (cpra_hip_graph_begin / _end); engines run eagerly while capturing
     bool capturing = false;

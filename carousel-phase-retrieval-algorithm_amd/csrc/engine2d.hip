@@ -53,6 +53,30 @@ __global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict_
     if (lane16 == 0 && row < nrows && any) atomicOr(&rowbits[row >> 5], 1u << (row & 31));
 }
 
+// Adaptive rows, one block: when more than 4/5 of the nrows row bits are set,
+// sets all of them (the sparse kernels then process every row) and the
+// natural-order flag.
+__global__ __launch_bounds__(1024) void k_rows_decide(unsigned* __restrict__ bits, int64_t words,
+                                                      int64_t nrows, unsigned* __restrict__ flag)
+{
+    __shared__ unsigned part[16];
+    __shared__ int dense;
+    unsigned c = 0;
+    for (int64_t i = threadIdx.x; i < words; i += blockDim.x) c += __popc(bits[i]);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        dense = 5 * t > 4 * nrows ? 1 : 0;
+        flag[0] = (unsigned)dense;
+    }
+    __syncthreads();
+    if (dense)
+        for (int64_t i = threadIdx.x; i < words; i += blockDim.x) bits[i] = 0xffffffffu;
+}
+
 __global__ void k_fill_u32(unsigned* p, unsigned v, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -303,7 +327,7 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
 template <int N>
 static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
                               const uint8_t* mask, float beta, int rp, int er,
-                              const unsigned* rowbits, bool tiled)
+                              const unsigned* rowbits, const unsigned* order, bool tiled)
 {
     const int64_t lines = g.B * g.M;
     constexpr int LPB = HioRowCfg<N>::LPB;
@@ -316,7 +340,7 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
     if (tiled) {
         if (g.M != N || N % LPB != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major rows: square patterns only");
 #define CPRA_ROWST(MODE, SP) \
-    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, true, SP, true>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, true, SP, true>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits, order)
         if (er) {
             if (rowbits) CPRA_ROWST(ROW_ER, true); else CPRA_ROWST(ROW_ER, false);
         } else {
@@ -326,7 +350,7 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
         return;
     }
 #define CPRA_ROWS(MODE, EX, SP) \
-    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX, SP>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits)
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX, SP>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits, order)
 #define CPRA_ROWS2(MODE, SP) \
     do { if (exact) CPRA_ROWS(MODE, true, SP); else CPRA_ROWS(MODE, false, SP); } while (0)
     if (er) {
@@ -340,15 +364,16 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
 
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
                             const uint8_t* mask, float beta, int rp, int er,
-                            const unsigned* rowbits = nullptr, bool tiled = false)
+                            const unsigned* rowbits = nullptr, const unsigned* order = nullptr,
+                            bool tiled = false)
 {
     switch (g.N) {
-    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
-    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
-    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
-    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
-    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
-    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, tiled); break;
+    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
+    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
+    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
+    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
+    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
+    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
     }
     check_launch("k_eng_rows_hio");
 }
@@ -492,8 +517,8 @@ static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, co
 
 
 // Sparse rows (tuning key 2): 0 = dense, 1 = sparse, 2 = adaptive (the
-// default): sparse while at most kSparseRowsMax of the rows hold a support
-// pixel, decided again after every shrinkwrap update.
+// default): sparse while at most 4/5 of the rows hold a support pixel,
+// decided on the device again after every shrinkwrap update.
 enum SparseMode { SPARSE_OFF = 0, SPARSE_ON = 1, SPARSE_AUTO = 2 };
 static int sparse_mode(const cpra_hip_ctx* h)
 {
@@ -503,54 +528,67 @@ static int sparse_mode(const cpra_hip_ctx* h)
 
 // Measured at 512 x 256^2 (tile-major state): sparse rows take 219 and 237 us
 // per iteration against 291 dense at 35 % and 50 % support rows, and 319
-// against 295 when every row holds support.
-constexpr double kSparseRowsMax = 0.8;
+// against 295 when every row holds support.  The adaptive mode therefore runs
+// the sparse kernels with every row bit set, in the natural block order, once
+// more than 4/5 of the rows hold support (k_rows_decide); they then do the
+// dense kernels' work at the dense kernels' speed.
 
-// Per pattern M/32 words: bit r set when row r holds a support pixel.
+// Per pattern M/32 words: bit r set when row r holds a support pixel; one
+// more word after them: the natural-order flag of the adaptive mode.
+static size_t row_words(const cpra_hip_ctx* h)
+{
+    return ((size_t)h->batch * h->m + 31) / 32;
+}
+
 static unsigned* row_bits(cpra_hip_ctx* h)
 {
-    const Geo g = geo_of(h);
-    const size_t words = ((size_t)g.B * g.M + 31) / 32;
-    h->ws_rows.ensure(4 * words + 256);
+    h->ws_rows.ensure(4 * row_words(h) + 256);
     return static_cast<unsigned*>(h->ws_rows.ptr);
 }
 
+static unsigned* row_order_flag(cpra_hip_ctx* h) { return row_bits(h) + row_words(h); }
+
+// Row bits of the support (the natural-order flag cleared).
 static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* mask)
 {
     const Geo g = geo_of(h);
     unsigned* bits = row_bits(h);
     const int64_t nrows = g.B * g.M;
-    const int64_t words = (nrows + 31) / 32;
+    const int64_t words = (int64_t)row_words(h) + 1;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(words, 256, 4096)), dim3(256), 0, st, bits, 0u, words);
     hipLaunchKernelGGL(k_support_rows, dim3((unsigned)((nrows + 15) / 16)), dim3(256), 0, st, mask, nrows, g.N, bits);
     check_launch("k_support_rows");
 }
 
-// Fraction of the batch's rows that hold a support pixel, from the row bits
-// written on st (synchronises st).
-static double support_row_fraction(cpra_hip_ctx* h, hipStream_t st)
-{
-    const Geo g = geo_of(h);
-    const int64_t nrows = g.B * g.M;
-    const size_t words = (size_t)((nrows + 31) / 32);
-    std::vector<unsigned> w(words);
-    CPRA_CHECK_HIP(hipMemcpyAsync(w.data(), row_bits(h), sizeof(unsigned) * words, hipMemcpyDeviceToHost, st));
-    CPRA_CHECK_HIP(hipStreamSynchronize(st));
-    int64_t c = 0;
-    for (unsigned v : w) c += __builtin_popcount(v);
-    return (double)c / (double)nrows;
-}
-
-// Lists the support rows (unless dense) and returns whether the next
-// iterations run sparse rows.  The adaptive mode reads the row count back,
-// which a stream capture cannot hold: it stays dense there.
+// Lists the support rows for the next iterations and returns whether they run
+// the sparse kernels (the adaptive mode decides dense or sparse on the device:
+// no synchronisation, so it also runs inside a stream capture).
 static bool decide_sparse(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support)
 {
     const int mode = sparse_mode(h);
-    if (mode == SPARSE_OFF || (mode == SPARSE_AUTO && h->capturing)) return false;
+    if (mode == SPARSE_OFF) return false;
     support_rows_launch(h, st, support);
-    if (mode == SPARSE_ON) return true;
-    return support_row_fraction(h, st) <= kSparseRowsMax;
+    if (mode == SPARSE_AUTO) {
+        const int64_t words = (int64_t)row_words(h);
+        hipLaunchKernelGGL(k_rows_decide, dim3(1), dim3(1024), 0, st, row_bits(h), words,
+                           (int64_t)h->batch * (int64_t)h->m, row_order_flag(h));
+        check_launch("k_rows_decide");
+    }
+    h->rows_listed = true;
+    return true;
+}
+
+// Row mode of the iterations that last ran: 1 sparse, 0 dense.  The adaptive
+// mode's choice is read from the device (synchronises the handle's stream).
+static int rows_sparse_used(cpra_hip_ctx* h)
+{
+    const int mode = sparse_mode(h);
+    if (mode != SPARSE_AUTO) return mode == SPARSE_ON ? 1 : 0;
+    if (!h->rows_listed || h->capturing) return 0;
+    unsigned flag = 0;
+    CPRA_CHECK_HIP(hipMemcpyAsync(&flag, row_order_flag(h), sizeof(flag), hipMemcpyDeviceToHost, h->stream));
+    CPRA_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return flag ? 0 : 1;
 }
 
 // Patterns per launch of the iteration kernels (0 = the whole batch).  With
@@ -602,13 +640,14 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
         void* xc = static_cast<c2<float>*>(x) + p0 * px;
         const uint8_t* mc = mask + p0 * px;
         const unsigned* bc = bits ? bits + p0 * g.M / 32 : nullptr;
+        const unsigned* order = bits ? row_order_flag(h) : nullptr;
         if (kt) kt->begin(h, st, KT_COL);
         launch_cols(h, st, gc, COL_MOD, xc, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero,
                     nullptr, 0.f, 0.f, bc, p->beta * (float)g.N, p->error_reduction, (int)p0, tiled);
         if (kt) kt->end(h, st);
         if (kt) kt->begin(h, st, KT_ROW);
         launch_rows_hio(st, gc, h->ws_z.ptr, xc, mc, p->beta, p->real_positive, p->error_reduction,
-                        bc, tiled);
+                        bc, order, tiled);
         if (kt) kt->end(h, st);
     }
 }
@@ -793,7 +832,6 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
             fft_pass_contig(h, x, g.N, g.B * g.M, -1, true, 1.0, false);
         }
         bool sparse = decide_sparse(h, st, support);
-        h->hio_sparse_used = sparse ? 1 : 0;
         const int colv = col_variant_code(col_variant(h, g.M));
         const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
         int done = 0;
@@ -867,10 +905,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 kt.end(h, gst);
                 // the new support's rows, and the row mode for the next chunk
                 // (after the last update only the exit follows)
-                if (done < p->iters) {
-                    sparse = decide_sparse(h, gst, support);
-                    h->hio_sparse_used = sparse ? 1 : 0;
-                }
+                if (done < p->iters) sparse = decide_sparse(h, gst, support);
                 h->stream = save;
                 ++k;
             }
@@ -1341,7 +1376,7 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
         const ColVariant cv = col_variant(h, (int)h->m);
         std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;rows_sparse_used=%d;",
                       (int)h->m, col_variant_code(cv), sparse_mode(h), (long long)iteration_chunk(h),
-                      tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, h->hio_sparse_used);
+                      tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, rows_sparse_used(h));
         return CPRA_HIP_OK;
     });
 }

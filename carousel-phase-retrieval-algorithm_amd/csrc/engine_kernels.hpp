@@ -321,7 +321,7 @@ template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false>
 __global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
     int64_t nlines, float beta, float invN, int real_positive,
-    const unsigned* __restrict__ rowbits)
+    const unsigned* __restrict__ rowbits, const unsigned* __restrict__ natural_order = nullptr)
 {
     using V = c2<float>;
     using PL = typename HioRowCfg<N>::P;
@@ -335,11 +335,15 @@ __global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
     // dispatcher deals to different XCDs) take the same row block of
     // consecutive patterns, so every XCD gets its share of the support rows
     // whatever their position in the pattern.  The grid is whole patterns.
+    // natural_order[0] != 0 (rows judged dense, every row bit set): the
+    // natural order, which keeps a pattern's rows together.
     unsigned vb = blockIdx.x;
     if constexpr (SPARSE && TILED) {
-        constexpr unsigned BPP = N / LPB;
-        const unsigned npat = gridDim.x / BPP;
-        vb = (blockIdx.x % npat) * BPP + blockIdx.x / npat;
+        if (natural_order == nullptr || natural_order[0] == 0u) {
+            constexpr unsigned BPP = N / LPB;
+            const unsigned npat = gridDim.x / BPP;
+            vb = (blockIdx.x % npat) * BPP + blockIdx.x / npat;
+        }
     }
     const int64_t line = (int64_t)vb * LPB + ll;
     // inb: the line exists (loads); valid: it is also written (sparse rows

@@ -41,11 +41,8 @@ void cpra_hip_hio_default_params(cpra_hip_hio_params* p);
  * of K = iters/sw_every uses sigma_k = sigma0 + (sigma1 - sigma0) k/(K-1).
  * If rfactor_host != NULL it receives, per pattern, the R-factor
  * sum_{A>0} | |Forward2D(x)| - A | / sum_{A>0} A of the final x.
- * In the adaptive row mode (tuning rows_sparse=2, the default) the call
- * synchronises the handle's stream at its start and after every shrinkwrap
- * update, to read back how many rows hold support; rows_sparse=0 or 1 keeps
- * it asynchronous, and inside a stream capture the adaptive mode runs dense
- * rows without synchronising. */
+ * The call is asynchronous unless rfactor_host is given; the row mode
+ * (tuning rows_sparse, adaptive by default) is decided on the device. */
 int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                      uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host);
 
@@ -117,14 +114,16 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
 /* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1|2>;", "chunk=<patterns>;"
  * and "tiled=<0|1>;" choices of an autotune report (rows_sparse: dense rows,
  * sparse rows, or adaptive, the default: sparse while at most 80 % of the rows
- * hold a support pixel, decided after every shrinkwrap update; chunk: patterns
+ * hold a support pixel, decided on the device after every shrinkwrap update;
+ * chunk: patterns
  * per launch of the iteration kernels, 0 = whole batch; tiled: tile-major
  * state for square patterns), and "plane=<0|1>;" (whole-plane LDS transforms
  * in the carousel's volume constraint for n <= 128; 0 = one pass per axis). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
  * tiled=<0|1>;plane=<0|1>;rows_sparse_used=<0|1>;" (the last: the row mode
- * of the last chunk of the last cpra_hip_hio_run). */
+ * of the last iterations run; in the adaptive mode it is read from the
+ * device, which synchronises the handle's stream). */
 int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 
 /* Per-kernel timing of the engines: when enabled, engine runs launch

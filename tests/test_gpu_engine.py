@@ -320,9 +320,10 @@ def test_tiled_state_matches_row_major(hipmod, gpu, synth, n):
 @pytest.mark.parametrize("n", [256, 192])
 def test_adaptive_rows_follow_the_support(hipmod, gpu, synth, n):
     """rows_sparse=2 (the default) runs sparse rows while at most 80 % of the
-    rows hold a support pixel and dense rows otherwise, deciding again after
-    every shrinkwrap update: its results are bit for bit those of the forced
-    mode it picks, and rows_sparse_used reports the choice."""
+    rows hold a support pixel and dense rows otherwise, deciding on the device
+    again after every shrinkwrap update.  Either way it runs the sparse
+    kernels (dense: with every row listed), so its results are bit for bit
+    those of rows_sparse=1, and rows_sparse_used reports the choice."""
     P = 3
     data = synth.make_2d_batch(P, n, n / 10, n / 7, 1e6, config=133)
     tight = (data["obj"] > 0).astype(np.uint8)          # the object discs: few rows
@@ -345,9 +346,12 @@ def test_adaptive_rows_follow_the_support(hipmod, gpu, synth, n):
         xa, used = run(None, tight, graph)
         assert used == "1"
         assert np.array_equal(xa, run(1, tight, graph)[0])
+        # dense: the sparse kernels with every row listed (in the natural
+        # block order), so the bits of rows_sparse=1 on a support that covers
+        # every row
         xa, used = run(None, full, graph)
         assert used == "0"
-        assert np.array_equal(xa, run(0, full, graph)[0])
+        assert np.array_equal(xa, run(1, full, graph)[0])
     with hipmod.HipImpl(n, n, 1, P) as h:
         with pytest.raises(hipmod.CpraHipError):
             h.set_tuning("rows_sparse=3;")

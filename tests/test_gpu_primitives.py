@@ -282,3 +282,36 @@ def test_graph_capture_replays_primitive_sequence(hipmod, gpu):
             outs.append([a.numpy() for a in (psi, t1, t2)])
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+def test_graph_capture_of_adaptive_hio(hipmod, gpu):
+    """cpra_hip_hio_run inside a caller's capture: the adaptive row mode
+    decides on the device (no synchronisation), so a captured run with
+    shrinkwrap updates replays to the bits of the same run made eagerly."""
+    n, B = 64, 3
+    rng = np.random.default_rng(9)
+    shp = (B, n, n)
+    x0 = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(np.complex64)
+    A = rng.random(shp).astype(np.float32)
+    yy, xx = np.mgrid[0:n, 0:n]
+    sup0 = np.broadcast_to((((yy - n / 2) ** 2 + (xx - n / 2) ** 2) < 15 ** 2), shp).astype(np.uint8)
+    kw = dict(iters=6, beta=0.9, real_positive=1, sw_every=3, use_graph=1)
+    outs = []
+    for mode in ("eager", "graph"):
+        with hipmod.HipImpl(n, n, 1, B) as h:
+            x, sup, dA = hipmod.to_device(x0), hipmod.to_device(sup0), hipmod.to_device(A)
+            h.hio_run(hipmod.to_device(x0), dA, hipmod.to_device(sup0), iters=1, sw_every=0)  # workspaces
+            h.Sync()
+            if mode == "eager":
+                h.hio_run(x, dA, sup, **kw)
+            else:
+                assert h.graph_begin()
+                h.hio_run(x, dA, sup, **kw)
+                gid = h.graph_end()
+                assert h.graph_launch(gid)
+                h.Sync()
+                assert h.graph_destroy(gid)
+            h.Sync()
+            outs.append((x.numpy(), sup.numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
