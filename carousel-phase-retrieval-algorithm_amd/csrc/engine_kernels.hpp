@@ -239,6 +239,16 @@ __device__ __forceinline__ void eng_cols_tile(
         const unsigned* rb = rowbits + (size_t)b * (M / 32);
         c2<float>* Yw = const_cast<c2<float>*>(Yp);
         static_assert(E % 4 == 0, "E must be a multiple of 4");
+        // every row of the pattern listed (the adaptive mode's dense choice):
+        // the dense kernel's stores, without the per-row tests
+        unsigned every = 0xffffffffu;
+        for (int w = 0; w < (M + 31) / 32; ++w) every &= rb[w];
+        if (M % 32 == 0 && every == 0xffffffffu) {
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                (Zp + rowbase(m))[toff] = v[m];
+            });
+        } else
         sfor<E / 4>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             bool sup[4];
@@ -336,7 +346,10 @@ __global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
     // consecutive patterns, so every XCD gets its share of the support rows
     // whatever their position in the pattern.  The grid is whole patterns.
     // natural_order[0] != 0 (rows judged dense, every row bit set): the
-    // natural order, which keeps a pattern's rows together.
+    // natural order, which was faster there (152 against 188 us per pass at
+    // 512 x 256^2).  Rotating each pattern's row blocks by the pattern index
+    // instead balances the XCDs without reading the flag, but measured slower
+    // than the better of the two orders at every support fraction tried.
     unsigned vb = blockIdx.x;
     if constexpr (SPARSE && TILED) {
         if (natural_order == nullptr || natural_order[0] == 0u) {
