@@ -234,7 +234,7 @@ static bool col_variant_valid(int v) { return v >= 0 && v < 8 && (v & 5) != 5; }
 template <int M, int MODE, int MAP, bool SP, bool TL>
 static void launch_cols_map(hipStream_t st, const Geo& g, int xcd_req, const void* Y, void* Z,
                             const float* A, int64_t amp_batch, float scale, int skip,
-                            float* rf_part, float gr, float gc, const unsigned* rowflag,
+                            float* rf_part, const unsigned* rowflag,
                             float beta_n, int er, int p0)
 {
     using G = ColGeom<M, MAP>;
@@ -242,13 +242,13 @@ static void launch_cols_map(hipStream_t st, const Geo& g, int xcd_req, const voi
     const int xcd = (xcd_req && grid % 8 == 0) ? 1 : 0;
     hipLaunchKernelGGL((k_eng_cols<M, MODE, MAP, SP, TL>), dim3(grid), dim3(G::TPB), 0, st,
                        static_cast<const c2<float>*>(Y), static_cast<c2<float>*>(Z), A, amp_batch,
-                       g.N, scale, skip, rf_part, gr, gc, xcd, rowflag, beta_n, er, p0);
+                       g.N, scale, skip, rf_part, xcd, rowflag, beta_n, er, p0);
 }
 
 template <int M, int MODE, bool SP, bool TL>
 static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, const void* Y,
                            void* Z, const float* A, int64_t amp_batch, float scale, int skip,
-                           float* rf_part, float gr, float gc, const unsigned* rowflag,
+                           float* rf_part, const unsigned* rowflag,
                            float beta_n, int er, int p0)
 {
     int map = cv.map;
@@ -257,7 +257,7 @@ static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, c
     // the MAP_BLOCK tile width
     if (map == MAP_WIDE && (TL || MODE == COL_MOD_RF || g.N % ColGeom<M, MAP_WIDE>::C != 0))
         map = MAP_BLOCK;
-#define CPRA_COLMAP(MP) launch_cols_map<M, MODE, MP, SP, TL>(st, g, cv.xcd, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0)
+#define CPRA_COLMAP(MP) launch_cols_map<M, MODE, MP, SP, TL>(st, g, cv.xcd, Y, Z, A, amp_batch, scale, skip, rf_part, rowflag, beta_n, er, p0)
     if (map == MAP_WAVE) CPRA_COLMAP(MAP_WAVE);
     else if constexpr (!TL) {
         if (map == MAP_WIDE) CPRA_COLMAP(MAP_WIDE);
@@ -271,28 +271,26 @@ static void launch_cols_mm(hipStream_t st, const Geo& g, const ColVariant& cv, c
 template <int M>
 static void launch_cols_m(hipStream_t st, const Geo& g, const ColVariant& cv, int mode,
                           const void* Y, void* Z, const float* A, int64_t amp_batch, float scale,
-                          int skip, float* rf_part, float gr, float gc, const unsigned* rowflag,
+                          int skip, float* rf_part, const unsigned* rowflag,
                           float beta_n, int er, int p0, bool tiled)
 {
     if (tiled) {
         if (mode == COL_MOD && rowflag)
-            launch_cols_mm<M, COL_MOD, true, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0);
+            launch_cols_mm<M, COL_MOD, true, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, rowflag, beta_n, er, p0);
         else if (mode == COL_MOD)
-            launch_cols_mm<M, COL_MOD, false, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+            launch_cols_mm<M, COL_MOD, false, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, nullptr, 0.f, 0, p0);
         else if (mode == COL_MOD_RF)
-            launch_cols_mm<M, COL_MOD_RF, false, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+            launch_cols_mm<M, COL_MOD_RF, false, true>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, nullptr, 0.f, 0, p0);
         else
             fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major column pass: mode %d", mode);
         return;
     }
     if (mode == COL_MOD && rowflag)
-        launch_cols_mm<M, COL_MOD, true, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0);
+        launch_cols_mm<M, COL_MOD, true, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, rowflag, beta_n, er, p0);
     else if (mode == COL_MOD)
-        launch_cols_mm<M, COL_MOD, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
-    else if (mode == COL_MOD_RF)
-        launch_cols_mm<M, COL_MOD_RF, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        launch_cols_mm<M, COL_MOD, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, nullptr, 0.f, 0, p0);
     else
-        launch_cols_mm<M, COL_BLUR, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, nullptr, 0.f, 0, p0);
+        launch_cols_mm<M, COL_MOD_RF, false, false>(st, g, cv, Y, Z, A, amp_batch, scale, skip, rf_part, nullptr, 0.f, 0, p0);
 }
 
 static ColVariant col_variant(const cpra_hip_ctx* h, int M)
@@ -305,13 +303,12 @@ static ColVariant col_variant(const cpra_hip_ctx* h, int M)
 
 static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode, const void* Y,
                         void* Z, const float* A, int64_t amp_batch, float scale, int skip,
-                        float* rf_part, float gr = 0.f, float gc = 0.f,
-                        const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0,
+                        float* rf_part, const unsigned* rowflag = nullptr, float beta_n = 0.f, int er = 0,
                         int p0 = 0, bool tiled = false, const ColVariant* force = nullptr)
 {
     const ColVariant cv = force ? *force : col_variant(h, g.M);
     if (tiled && g.M != g.N) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major state needs square patterns");
-#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, gr, gc, rowflag, beta_n, er, p0, tiled)
+#define CPRA_COLS(MM) launch_cols_m<MM>(st, g, cv, mode, Y, Z, A, amp_batch, scale, skip, rf_part, rowflag, beta_n, er, p0, tiled)
     switch (g.M) {
     case 64: CPRA_COLS(64); break;
     case 128: CPRA_COLS(128); break;
@@ -427,44 +424,63 @@ static void launch_amp_retile(hipStream_t st, int n, int64_t patterns, const flo
     check_launch("k_amp_retile");
 }
 
+static Geo geo_of(cpra_hip_ctx* h) { return Geo{(int)h->m, (int)h->n, (int64_t)h->batch}; }
+
 template <int N>
-static void launch_sw_rows_n(hipStream_t st, const Geo& g, int stage, bool from_state,
-                             const void* in, void* out, float* bl, unsigned* keys, bool tiled)
+static void launch_sw_rowblur_n(hipStream_t st, const Geo& g, bool from_state, bool tiled, const void* in,
+                                float* r1, float gc)
 {
     const int64_t lines = g.B * g.M;
     const unsigned grid = (unsigned)((lines + SizeCfg<N>::LPB - 1) / SizeCfg<N>::LPB);
     const dim3 blk(SizeCfg<N>::TPB);
     auto* i = static_cast<const c2<float>*>(in);
-    if (stage == 1) {
-        auto* o = static_cast<c2<float>*>(out);
-        if (from_state && tiled)
-            hipLaunchKernelGGL((k_sw_mag<N, true, true>), dim3(grid), blk, 0, st, i, o, lines, 1.0f / (float)N);
-        else if (from_state)
-            hipLaunchKernelGGL((k_sw_mag<N, true>), dim3(grid), blk, 0, st, i, o, lines, 1.0f / (float)N);
-        else
-            hipLaunchKernelGGL((k_sw_mag<N, false>), dim3(grid), blk, 0, st, i, o, lines, 1.0f);
-    } else {
-        hipLaunchKernelGGL((k_sw_blurmax<N>), dim3(grid), blk, 0, st, i, bl, keys, lines, g.M);
-    }
+    const float invN = 1.0f / (float)N;
+    if (from_state && tiled)
+        hipLaunchKernelGGL((k_sw_rowblur<N, true, true>), dim3(grid), blk, 0, st, i, r1, lines, invN, gc);
+    else if (from_state)
+        hipLaunchKernelGGL((k_sw_rowblur<N, true>), dim3(grid), blk, 0, st, i, r1, lines, invN, gc);
+    else
+        hipLaunchKernelGGL((k_sw_rowblur<N, false>), dim3(grid), blk, 0, st, i, r1, lines, invN, gc);
 }
 
-static void launch_sw_rows(hipStream_t st, const Geo& g, int stage, bool from_state,
-                           const void* in, void* out, float* bl, unsigned* keys,
-                           bool tiled = false)
+static void launch_sw_rowblur(hipStream_t st, const Geo& g, bool from_state, bool tiled, const void* in,
+                              float* r1, float gc)
 {
     if (tiled && g.M != g.N) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major state needs square patterns");
     switch (g.N) {
-    case 64: launch_sw_rows_n<64>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
-    case 128: launch_sw_rows_n<128>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
-    case 192: launch_sw_rows_n<192>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
-    case 256: launch_sw_rows_n<256>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
-    case 384: launch_sw_rows_n<384>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
-    case 512: launch_sw_rows_n<512>(st, g, stage, from_state, in, out, bl, keys, tiled); break;
+    case 64: launch_sw_rowblur_n<64>(st, g, from_state, tiled, in, r1, gc); break;
+    case 128: launch_sw_rowblur_n<128>(st, g, from_state, tiled, in, r1, gc); break;
+    case 192: launch_sw_rowblur_n<192>(st, g, from_state, tiled, in, r1, gc); break;
+    case 256: launch_sw_rowblur_n<256>(st, g, from_state, tiled, in, r1, gc); break;
+    case 384: launch_sw_rowblur_n<384>(st, g, from_state, tiled, in, r1, gc); break;
+    case 512: launch_sw_rowblur_n<512>(st, g, from_state, tiled, in, r1, gc); break;
     }
-    check_launch("k_sw_rows");
+    check_launch("k_sw_rowblur");
 }
 
-static Geo geo_of(cpra_hip_ctx* h) { return Geo{(int)h->m, (int)h->n, (int64_t)h->batch}; }
+template <int M>
+static void launch_sw_colblur_m(hipStream_t st, const Geo& g, const float* r1, float* bl, unsigned* keys,
+                                float gr)
+{
+    constexpr int C = SizeCfg<M>::C;
+    if (g.N % C != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "shrinkwrap: N=%d not a multiple of the %d-column tile", g.N, C);
+    const unsigned grid = (unsigned)(g.B * (g.N / C));
+    hipLaunchKernelGGL((k_sw_colblur<M>), dim3(grid), dim3(SizeCfg<M>::TPB), 0, st, r1, bl, keys, g.N,
+                       1.0f / (float)M, gr);
+}
+
+static void launch_sw_colblur(hipStream_t st, const Geo& g, const float* r1, float* bl, unsigned* keys, float gr)
+{
+    switch (g.M) {
+    case 64: launch_sw_colblur_m<64>(st, g, r1, bl, keys, gr); break;
+    case 128: launch_sw_colblur_m<128>(st, g, r1, bl, keys, gr); break;
+    case 192: launch_sw_colblur_m<192>(st, g, r1, bl, keys, gr); break;
+    case 256: launch_sw_colblur_m<256>(st, g, r1, bl, keys, gr); break;
+    case 384: launch_sw_colblur_m<384>(st, g, r1, bl, keys, gr); break;
+    case 512: launch_sw_colblur_m<512>(st, g, r1, bl, keys, gr); break;
+    }
+    check_launch("k_sw_colblur");
+}
 
 // Shrinkwrap on the state (from_state) or on a real-space x; writes the mask
 // and, if blur_out != NULL, copies the blurred image there.
@@ -478,18 +494,17 @@ static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, 
     h->ws_z.ensure(sizeof(c2<float>) * total);
     h->ws_blur.ensure(sizeof(float) * total);
     h->ws_keys.ensure(sizeof(unsigned) * g.B);
-    void* W = h->ws_z.ptr;
+    float* r1 = static_cast<float*>(h->ws_z.ptr);  // the row-blurred image
     float* bl = blur_out ? blur_out : static_cast<float*>(h->ws_blur.ptr);
     unsigned* keys = static_cast<unsigned*>(h->ws_keys.ptr);
-    launch_sw_rows(st, g, 1, from_state, src, W, nullptr, nullptr, tiled);
-    // G(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-M/2)/M)^2 + ((kc-N/2)/N)^2])
+    // G(kr, kc) = exp(-2 pi^2 sigma^2 [((kr-M/2)/M)^2 + ((kc-N/2)/N)^2]),
+    // applied as the row factor then the column factor
     const double k = -2.0 * M_PI * M_PI * (double)sigma * (double)sigma;
     const float gr = (float)(k / ((double)g.M * g.M));
     const float gc = (float)(k / ((double)g.N * g.N));
-    const float scale = (float)(1.0 / ((double)g.M * g.N));
-    launch_cols(h, st, g, COL_BLUR, W, W, nullptr, 1, scale, 0, nullptr, gr, gc);
+    launch_sw_rowblur(st, g, from_state, tiled, src, r1, gc);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(g.B, 256)), dim3(256), 0, st, keys, 0u, g.B);
-    launch_sw_rows(st, g, 3, false, W, nullptr, bl, keys);
+    launch_sw_colblur(st, g, r1, bl, keys, gr);
     if (px % 16 == 0 && reinterpret_cast<uintptr_t>(bl) % 16 == 0 && reinterpret_cast<uintptr_t>(mask) % 16 == 0)
         hipLaunchKernelGGL(k_sw_thresh16, dim3(grid_1d(total / 16, 256, 65536)), dim3(256), 0, st,
                            reinterpret_cast<const float4*>(bl), keys, reinterpret_cast<uint4*>(mask), px / 16,
@@ -509,7 +524,7 @@ static void rfactor_launch(cpra_hip_ctx* h, hipStream_t st, const void* Ysrc, co
     h->ws_part.ensure(sizeof(float) * 2 * (size_t)bpp * g.B);
     float* part = static_cast<float*>(h->ws_part.ptr);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
-    launch_cols(h, st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part, 0.f, 0.f,
+    launch_cols(h, st, g, COL_MOD_RF, Ysrc, h->ws_z.ptr, amp, amp_batch, scale, skip, part,
                 nullptr, 0.f, 0, 0, tiled);
     hipLaunchKernelGGL(k_rf_fin, dim3(grid_1d(g.B, 64)), dim3(64), 0, st, part, bpp, g.B, rf_dev);
     check_launch("k_rf_fin");
@@ -643,7 +658,7 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
         const unsigned* order = bits ? row_order_flag(h) : nullptr;
         if (kt) kt->begin(h, st, KT_COL);
         launch_cols(h, st, gc, COL_MOD, xc, h->ws_z.ptr, amp, amp_batch, scale, p->dc_skip_zero,
-                    nullptr, 0.f, 0.f, bc, p->beta * (float)g.N, p->error_reduction, (int)p0, tiled);
+                    nullptr, bc, p->beta * (float)g.N, p->error_reduction, (int)p0, tiled);
         if (kt) kt->end(h, st);
         if (kt) kt->begin(h, st, KT_ROW);
         launch_rows_hio(st, gc, h->ws_z.ptr, xc, mc, p->beta, p->real_positive, p->error_reduction,
@@ -989,7 +1004,7 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
     auto one = [&](hipStream_t s, bool last) {
         kt.begin(h, s, KT_COL);
         launch_cols(h, s, gz2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr,
-                    0.f, 0.f, nullptr, 0.f, 0, 0, false, &zcv);
+                    nullptr, 0.f, 0, 0, false, &zcv);
         kt.end(h, s);
         ypass(s, Z, +1);
         kt.begin(h, s, KT_ROW);

@@ -88,7 +88,7 @@ struct BufC64 {
     }
 };
 
-enum ColMode { COL_MOD = 0, COL_MOD_RF = 1, COL_BLUR = 2 };
+enum ColMode { COL_MOD = 0, COL_MOD_RF = 1 };
 enum ColMap { MAP_BLOCK = 0, MAP_WAVE = 1, MAP_WIDE = 2 };
 
 // Thread <-> (column, t) mapping of the column kernels.  A block owns C
@@ -126,7 +126,6 @@ template <int M, int MAP> struct ColGeom {
 // Column kernel: centred column DFT of the row-spectral state Y, then
 //  COL_MOD / COL_MOD_RF: modulus projection with A (scale 1/(MN)); RF also
 //    accumulates the R-factor sums of the incoming iterate per block;
-//  COL_BLUR: multiply by the centred Gaussian transfer function (scale 1/(MN));
 // then the inverse column DFT into Z.  Y and Z may alias (each block reads
 // its columns fully before writing them).
 // SPARSE (HIO iterations): rows of a pattern that hold no support pixel get
@@ -145,7 +144,7 @@ template <int M, int MODE, int MAP, bool SPARSE, bool TILED = false>
 __device__ __forceinline__ void eng_cols_tile(
     const unsigned tile, c2<float>* lds, float* red, const c2<float>* Y, c2<float>* Z,
     const float* __restrict__ A, int64_t amp_batch, int N, float scale, int skip_zero,
-    float* __restrict__ rf_part, float gcoef_r, float gcoef_c, const unsigned* __restrict__ rowbits,
+    float* __restrict__ rf_part, const unsigned* __restrict__ rowbits,
     float beta_n, int error_reduction, int pattern0)
 {
     using V = c2<float>;
@@ -158,7 +157,6 @@ __device__ __forceinline__ void eng_cols_tile(
     const unsigned tiles_per_pattern = (unsigned)N / C;
     const unsigned b = tile / tiles_per_pattern;
     const int col0 = (int)(tile - b * tiles_per_pattern) * C;
-    const int col = col0 + c;
     const size_t pat = (size_t)M * (size_t)N;
     static_assert(!TILED || MAP != MAP_WIDE, "tiled state uses the layout's tile width");
     // row stride and the block's first element: row-major (stride N, column
@@ -178,49 +176,35 @@ __device__ __forceinline__ void eng_cols_tile(
         constexpr int m = decltype(mc)::value;
         v[m] = (Yp + rowbase(m))[toff];
     });
-    if constexpr (MODE != COL_BLUR) {
-        // the amplitudes are needed after the forward FFT; load them now so
-        // their latency overlaps the transform
-        // pattern0: index of this launch's first pattern in the whole batch
-        // (chunked launches), for the amplitude broadcast b % amp_batch
-        const size_t abase = (size_t)(((unsigned)pattern0 + b) % (unsigned)amp_batch) * pat;
-        const float* Ap = A + abase + (TILED ? (size_t)(tile - b * tiles_per_pattern) * M * C : (size_t)col0);
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            av[m] = (Ap + rowbase(m))[toff];
-        });
-    }
+    // the amplitudes are needed after the forward FFT; load them now so their
+    // latency overlaps the transform.  pattern0: index of this launch's first
+    // pattern in the whole batch (chunked launches), for the amplitude
+    // broadcast b % amp_batch
+    const size_t abase = (size_t)(((unsigned)pattern0 + b) % (unsigned)amp_batch) * pat;
+    const float* Ap = A + abase + (TILED ? (size_t)(tile - b * tiles_per_pattern) * M * C : (size_t)col0);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        av[m] = (Ap + rowbase(m))[toff];
+    });
     float rnum = 0.f, rden = 0.f;
     auto body = [&](auto& acc) {
         fft_line<PL, -1>(v, t, g_tw<M>(), acc);
-        if constexpr (MODE == COL_BLUR) {
-            const float dc = (float)(col - N / 2);
-            const float gc = gcoef_c * dc * dc;
-            sfor<E>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                const int row = t + PL::rot(m) * TT;
-                const float dr = (float)(row - M / 2);
-                const float g = expf(gcoef_r * dr * dr + gc) * scale;
-                v[m] = cscale(v[m], g);
-            });
-        } else {
-            sfor<E>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                const float a = av[m];
-                const V x = v[m];
-                const float m2 = x.x * x.x + x.y * x.y;
-                if constexpr (MODE == COL_MOD_RF) {
-                    if (a > 0.f) {
-                        rnum += fabsf(sqrtf(m2) - a);
-                        rden += a;
-                    }
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const float a = av[m];
+            const V x = v[m];
+            const float m2 = x.x * x.x + x.y * x.y;
+            if constexpr (MODE == COL_MOD_RF) {
+                if (a > 0.f) {
+                    rnum += fabsf(sqrtf(m2) - a);
+                    rden += a;
                 }
-                // x * A/|x| (the reference's polar(A, arg x), MKL_Impl.inl:102-103)
-                float s = scale;
-                if (!(skip_zero && a == 0.f) && m2 > 0.f) s = scale * a * rsqrtf(m2);
-                v[m] = cscale(x, s);
-            });
-        }
+            }
+            // x * A/|x| (the reference's polar(A, arg x), MKL_Impl.inl:102-103)
+            float s = scale;
+            if (!(skip_zero && a == 0.f) && m2 > 0.f) s = scale * a * rsqrtf(m2);
+            v[m] = cscale(x, s);
+        });
         fft_line<PL, +1>(v, t, g_tw<M>(), acc);
     };
     if constexpr (MAP != MAP_WAVE) {
@@ -298,7 +282,7 @@ __device__ __forceinline__ void eng_cols_tile(
 template <int M, int MODE, int MAP, bool SPARSE = false, bool TILED = false>
 __global__ __launch_bounds__((ColGeom<M, MAP>::TPB)) void k_eng_cols(
     const c2<float>* Y, c2<float>* Z, const float* __restrict__ A, int64_t amp_batch, int N,
-    float scale, int skip_zero, float* __restrict__ rf_part, float gcoef_r, float gcoef_c,
+    float scale, int skip_zero, float* __restrict__ rf_part,
     int xcd_remap, const unsigned* __restrict__ rowbits = nullptr, float beta_n = 0.f,
     int error_reduction = 0, int pattern0 = 0)
 {
@@ -306,11 +290,11 @@ __global__ __launch_bounds__((ColGeom<M, MAP>::TPB)) void k_eng_cols(
     if constexpr (MODE == COL_MOD_RF) {
         __shared__ float red[2 * ColGeom<M, MAP>::TPB];
         eng_cols_tile<M, MODE, MAP, SPARSE, TILED>(tile_index(xcd_remap), lds, red, Y, Z, A, amp_batch, N, scale,
-                                            skip_zero, rf_part, gcoef_r, gcoef_c, rowbits, beta_n,
+                                            skip_zero, rf_part, rowbits, beta_n,
                                             error_reduction, pattern0);
     } else {
         eng_cols_tile<M, MODE, MAP, SPARSE, TILED>(tile_index(xcd_remap), lds, nullptr, Y, Z, A, amp_batch, N,
-                                            scale, skip_zero, rf_part, gcoef_r, gcoef_c, rowbits, beta_n,
+                                            scale, skip_zero, rf_part, rowbits, beta_n,
                                             error_reduction, pattern0);
     }
 }
@@ -491,13 +475,20 @@ __global__ __launch_bounds__(256) void k_amp_retile(const float* __restrict__ in
     out[i] = in[b * N * N + rr * N + tile * C + cc];
 }
 
-// Shrinkwrap, row stage 1: W = DFT_r(|x|), x = IDFT_r(Y)/N (FROM_STATE) or x
-// read directly.
-// TILED (with FROM_STATE): the state Y is tile-major (Tiled<N>), W row-major.
+// Shrinkwrap blur, separable form.  The centred Gaussian transfer function
+// factors, G(kr, kc) = g_r(kr) g_c(kc) with g(k) = exp(-2 pi^2 sigma^2
+// ((k - n/2)/n)^2), so b = Backward2D(G . Forward2D(|x|)) is a blur of every
+// row by g_c followed by a blur of every column by g_r.  The row stage keeps
+// only the real part (the row blur of a real row is real), so the image
+// between the stages and the result are real: 12 + 8 bytes per pixel instead
+// of the 16 + 16 + 12 of the 2D spectrum.
+//
+// Row stage: r1 = Re IDFT_r(g_c . DFT_r(|x|)) / N, with x = IDFT_r(Y)/N
+// (FROM_STATE, Y tile-major when TILED) or x read directly; r1 row-major.
 template <int N, bool FROM_STATE, bool TILED = false>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __restrict__ Y,
-                                                           c2<float>* __restrict__ W,
-                                                           int64_t nlines, float invN)
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>* __restrict__ Y,
+                                                               float* __restrict__ r1, int64_t nlines,
+                                                               float invN, float gcoef_c)
 {
     using V = c2<float>;
     using PL = typename SizeCfg<N>::P;
@@ -527,69 +518,70 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_mag(const c2<float>* __r
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const float sc = FROM_STATE ? invN : 1.f;
-        const float a = sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc;
-        v[m] = mk<V>(a, 0.f);
+        v[m] = mk<V>(sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc, 0.f);
     });
     fft_line<PL, -1>(v, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float d = (float)(t + PL::rot(m) * TT - N / 2);
+        v[m] = cscale(v[m], expf(gcoef_c * d * d) * invN);
+    });
+    fft_line<PL, +1>(v, t, tw, acc);
     if (valid) {
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            W[base + t + PL::rot(m) * TT] = v[m];
+            r1[base + t + PL::rot(m) * TT] = v[m].x;
         });
     }
 }
 
-// Shrinkwrap, row stage 3: blurred image b = Re IDFT_r(W) and per-pattern
-// maximum (reduced in the wave, then per block, then one atomic per block
-// when the block's lines belong to one pattern).
-template <int N>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_blurmax(const c2<float>* __restrict__ W,
+// Column stage: b = Re IDFT_c(g_r . DFT_c(r1)) / M on tiles of C columns of
+// one pattern (r1 and b row-major [B][M][N]), and the per-pattern maximum of
+// b as an order-preserving key (one atomicMax per block).
+template <int M>
+__global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __restrict__ r1,
                                                                float* __restrict__ bl,
-                                                               unsigned* __restrict__ keys,
-                                                               int64_t nlines, int M)
+                                                               unsigned* __restrict__ keys, int N,
+                                                               float invM, float gcoef_r)
 {
     using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
-    constexpr int SPAN = LdsRow<V, N>::span();
-    __shared__ V lds[LPB * SPAN];
-    __shared__ unsigned bmax[LPB];
+    using PL = typename SizeCfg<M>::P;
+    constexpr int E = PL::E, TT = PL::T, C = SizeCfg<M>::C, TPB = SizeCfg<M>::TPB;
+    __shared__ V lds[M * C];
+    __shared__ float wmax[TPB / 64];
     const int tid = threadIdx.x;
-    const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    const bool valid = line < nlines;
-    const int64_t base = (valid ? line : 0) * N;
-    LdsRow<V, N, true> acc{lds + ll * SPAN};
-    const V* tw = g_tw<N>();
+    const int c = tid % C, t = tid / C;
+    const unsigned tiles_per_pattern = (unsigned)N / C;
+    const unsigned b = blockIdx.x / tiles_per_pattern;
+    const int col0 = (int)(blockIdx.x - b * tiles_per_pattern) * C;
+    const size_t off = (size_t)b * M * N + col0 + c;
+    auto at = [&](int m) { return off + (size_t)(t + PL::rot(m) * TT) * (size_t)N; };
     V v[E];
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        v[m] = valid ? W[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+        v[m] = mk<V>(r1[at(m)], 0.f);
     });
-    fft_line<PL, +1>(v, t, tw, acc);
+    LdsCol<V, C> acc{lds + c};
+    fft_line<PL, -1>(v, t, g_tw<M>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float d = (float)(t + PL::rot(m) * TT - M / 2);
+        v[m] = cscale(v[m], expf(gcoef_r * d * d) * invM);
+    });
+    fft_line<PL, +1>(v, t, g_tw<M>(), acc);
     float mx = -INFINITY;
-    if (valid) {
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            bl[base + t + PL::rot(m) * TT] = v[m].x;
-            mx = fmaxf(mx, v[m].x);
-        });
-    }
-    // max over the line's T lanes (consecutive lanes of one wave)
-    for (int off = TT / 2; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if (t == 0) bmax[ll] = valid ? f2key(mx) : 0u;
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        bl[at(m)] = v[m].x;
+        mx = fmaxf(mx, v[m].x);
+    });
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((tid & 63) == 0) wmax[tid >> 6] = mx;
     __syncthreads();
     if (tid == 0) {
-        const int64_t first = (int64_t)blockIdx.x * LPB;
-        const int64_t last = first + LPB - 1 < nlines ? first + LPB - 1 : nlines - 1;
-        if (first / M == last / M) {
-            unsigned k = 0u;
-            for (int i = 0; i < LPB; ++i) k = bmax[i] > k ? bmax[i] : k;
-            atomicMax(&keys[first / M], k);
-        } else {
-            for (int i = 0; i < LPB && first + i < nlines; ++i)
-                atomicMax(&keys[(first + i) / M], bmax[i]);
-        }
+        float bm = wmax[0];
+        for (int w = 1; w < TPB / 64; ++w) bm = fmaxf(bm, wmax[w]);
+        atomicMax(&keys[b], f2key(bm));
     }
 }
 

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_mixed_cols(const MixedArgs a)
     const unsigned tile = blockIdx.x - c.cb0;
 #define CPRA_MC(NN)                                                                                       \
     eng_cols_tile<NN, COL_MOD, MAP_BLOCK, false>(tile, lds, nullptr, c.Y, c.Z, c.A, c.count, NN, c.scale, \
-                                                 a.skip_zero, nullptr, 0.f, 0.f, nullptr, 0.f, 0, 0)
+                                                 a.skip_zero, nullptr, nullptr, 0.f, 0, 0)
     switch (c.n) {
     case 64: CPRA_MC(64); break;
     case 128: CPRA_MC(128); break;
@@ -158,7 +158,7 @@ static void cols_n(hipStream_t st, const MixedClass& c, const c2<float>* Y, c2<f
 {
     const unsigned grid = (unsigned)(B * N / SizeCfg<N>::C);
     hipLaunchKernelGGL((k_eng_cols<N, COL_MOD, MAP_BLOCK, false>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, Y, Z,
-                       A, amp_batch, N, c.scale, skip, nullptr, 0.f, 0.f, 0, nullptr, 0.f, 0, 0);
+                       A, amp_batch, N, c.scale, skip, nullptr, 0, nullptr, 0.f, 0, 0);
 }
 
 #define CPRA_NSWITCH(NV, CALL)                         \

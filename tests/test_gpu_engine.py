@@ -101,6 +101,29 @@ def test_shrinkwrap_matches_oracle(hipmod, oracle, gpu, synth, n, B):
         assert np.array_equal(got, (b >= thr_gpu).astype(np.uint8))
 
 
+@pytest.mark.parametrize("m,n", [(64, 128), (192, 64)])
+def test_shrinkwrap_non_square_matches_oracle(hipmod, oracle, gpu, m, n):
+    """The separable blur on m x n patterns: rows blurred with the n-point
+    factor, columns with the m-point factor."""
+    B = 2
+    rng = np.random.default_rng(m + n)
+    yy, xx = np.mgrid[0:m, 0:n]
+    disc = ((yy - m / 2) / (m / 5)) ** 2 + ((xx - n / 2) / (n / 5)) ** 2 < 1
+    obj = (rng.random((B, m, n)) * disc).astype(np.complex64)
+    x = hipmod.to_device(obj)
+    sup = hipmod.zeros((B, m, n), np.uint8)
+    blur = hipmod.zeros((B, m, n), np.float32)
+    with hipmod.HipImpl(m, n, 1, B) as h:
+        assert h.shrinkwrap(x, sup, 2.0, 0.1, blur)
+        h.Sync()
+    s_ref, b_ref = oracle.shrinkwrap(obj, 2.0, 0.1, want_blur=True)
+    b = blur.numpy()
+    assert relerr(b, b_ref) < 1e-5
+    thr = 0.1 * b_ref.max(axis=(1, 2), keepdims=True)
+    near = np.abs(b_ref - thr) <= 1e-5 * b_ref.max(axis=(1, 2), keepdims=True)
+    assert np.array_equal(sup.numpy()[~near], s_ref[~near])
+
+
 @pytest.mark.parametrize("n,B", [(64, 4), (256, 2), (384, 1)])
 def test_rfactor_matches_oracle(hipmod, oracle, gpu, synth, n, B):
     data = synth.make_2d_batch(B, n, n * 0.12, n * 0.18, 1e6, config=100 + n)
@@ -254,9 +277,8 @@ def test_autotune_picks_variant_and_restores_state(hipmod, oracle, gpu, synth):
 @pytest.mark.parametrize("n", [128, 256, 384, 512])
 def test_wide_column_tiles_bitwise_equal_to_block_tiles(hipmod, gpu, synth, n):
     """The 32-column (512-thread) column tiles run each column's transform with
-    the same arithmetic as the 16-column tiles, so HIO with shrinkwrap (the
-    blur runs through the column kernel) and the difference map agree bit for
-    bit between the two thread mappings."""
+    the same arithmetic as the 16-column tiles, so HIO with shrinkwrap and the
+    difference map agree bit for bit between the two thread mappings."""
     P = 3
     data = synth.make_2d_batch(P, n, n / 12, n / 8, 1e6, config=131)
     amp = hipmod.to_device(data["amp"])
