@@ -464,9 +464,12 @@ static void launch_sw_colblur_m(hipStream_t st, const Geo& g, const float* r1, f
 {
     constexpr int C = SizeCfg<M>::C;
     if (g.N % C != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "shrinkwrap: N=%d not a multiple of the %d-column tile", g.N, C);
-    const unsigned grid = (unsigned)(g.B * (g.N / C));
-    hipLaunchKernelGGL((k_sw_colblur<M>), dim3(grid), dim3(SizeCfg<M>::TPB), 0, st, r1, bl, keys, g.N,
-                       1.0f / (float)M, gr);
+    if (g.N % (2 * C) == 0)  // two real columns per transform
+        hipLaunchKernelGGL((k_sw_colblur<M, true>), dim3((unsigned)(g.B * (g.N / (2 * C)))),
+                           dim3(SizeCfg<M>::TPB), 0, st, r1, bl, keys, g.N, 1.0f / (float)M, gr);
+    else
+        hipLaunchKernelGGL((k_sw_colblur<M, false>), dim3((unsigned)(g.B * (g.N / C))), dim3(SizeCfg<M>::TPB),
+                           0, st, r1, bl, keys, g.N, 1.0f / (float)M, gr);
 }
 
 static void launch_sw_colblur(hipStream_t st, const Geo& g, const float* r1, float* bl, unsigned* keys, float gr)

@@ -535,10 +535,13 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>*
     }
 }
 
-// Column stage: b = Re IDFT_c(g_r . DFT_c(r1)) / M on tiles of C columns of
-// one pattern (r1 and b row-major [B][M][N]), and the per-pattern maximum of
-// b as an order-preserving key (one atomicMax per block).
-template <int M>
+// Column stage: b = Re IDFT_c(g_r . DFT_c(r1)) / M on tiles of one pattern
+// (r1 and b row-major [B][M][N]), and the per-pattern maximum of b as an
+// order-preserving key (one atomicMax per block).  PAIR: each line of the
+// transform carries two adjacent real columns, a + i b; g_r is real and even
+// about the centre, so the blur of a + i b is blur(a) + i blur(b): half the
+// transforms, and 8-byte loads (2C columns per block, N % 2C == 0).
+template <int M, bool PAIR>
 __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __restrict__ r1,
                                                                float* __restrict__ bl,
                                                                unsigned* __restrict__ keys, int N,
@@ -547,19 +550,25 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __r
     using V = c2<float>;
     using PL = typename SizeCfg<M>::P;
     constexpr int E = PL::E, TT = PL::T, C = SizeCfg<M>::C, TPB = SizeCfg<M>::TPB;
+    constexpr int CW = PAIR ? 2 * C : C;  // columns per block
     __shared__ V lds[M * C];
     __shared__ float wmax[TPB / 64];
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
-    const unsigned tiles_per_pattern = (unsigned)N / C;
+    const unsigned tiles_per_pattern = (unsigned)N / CW;
     const unsigned b = blockIdx.x / tiles_per_pattern;
-    const int col0 = (int)(blockIdx.x - b * tiles_per_pattern) * C;
-    const size_t off = (size_t)b * M * N + col0 + c;
+    const int col0 = (int)(blockIdx.x - b * tiles_per_pattern) * CW;
+    const size_t off = (size_t)b * M * N + col0 + (PAIR ? 2 * c : c);
     auto at = [&](int m) { return off + (size_t)(t + PL::rot(m) * TT) * (size_t)N; };
     V v[E];
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        v[m] = mk<V>(r1[at(m)], 0.f);
+        if constexpr (PAIR) {
+            const float2 p = *reinterpret_cast<const float2*>(r1 + at(m));
+            v[m] = mk<V>(p.x, p.y);
+        } else {
+            v[m] = mk<V>(r1[at(m)], 0.f);
+        }
     });
     LdsCol<V, C> acc{lds + c};
     fft_line<PL, -1>(v, t, g_tw<M>(), acc);
@@ -574,6 +583,10 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __r
         constexpr int m = decltype(mc)::value;
         bl[at(m)] = v[m].x;
         mx = fmaxf(mx, v[m].x);
+        if constexpr (PAIR) {
+            bl[at(m) + 1] = v[m].y;
+            mx = fmaxf(mx, v[m].y);
+        }
     });
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if ((tid & 63) == 0) wmax[tid >> 6] = mx;
