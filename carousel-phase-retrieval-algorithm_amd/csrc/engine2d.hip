@@ -431,7 +431,8 @@ static void launch_sw_rowblur_n(hipStream_t st, const Geo& g, bool from_state, b
                                 float* r1, float gc)
 {
     const int64_t lines = g.B * g.M;
-    const unsigned grid = (unsigned)((lines + SizeCfg<N>::LPB - 1) / SizeCfg<N>::LPB);
+    // two halves of LPB lines per block
+    const unsigned grid = (unsigned)((lines + 2 * SizeCfg<N>::LPB - 1) / (2 * SizeCfg<N>::LPB));
     const dim3 blk(SizeCfg<N>::TPB);
     auto* i = static_cast<const c2<float>*>(in);
     const float invN = 1.0f / (float)N;

@@ -485,6 +485,11 @@ __global__ __launch_bounds__(256) void k_amp_retile(const float* __restrict__ in
 //
 // Row stage: r1 = Re IDFT_r(g_c . DFT_r(|x|)) / N, with x = IDFT_r(Y)/N
 // (FROM_STATE, Y tile-major when TILED) or x read directly; r1 row-major.
+// A block takes 2 LPB lines: each thread group transforms line A (first half)
+// and line B (second half) to |x| on its own, then blurs them together as
+// |x_A| + i |x_B| (g_c is real and even about the centre, so the real and
+// imaginary parts come out as the two blurred rows): four transforms per two
+// rows instead of six.
 template <int N, bool FROM_STATE, bool TILED = false>
 __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>* __restrict__ Y,
                                                                float* __restrict__ r1, int64_t nlines,
@@ -497,28 +502,41 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>*
     __shared__ V lds[LPB * SPAN];
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
-    const int64_t line = (int64_t)blockIdx.x * LPB + ll;
-    const bool valid = line < nlines;
-    const int64_t base = (valid ? line : 0) * N;
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
-    V v[E];
-    // tiled: a block's LPB lines lie in one pattern (LPB | N, whole patterns)
-    const size_t line0 = (size_t)blockIdx.x * LPB;
-    const c2<float>* Yp = Y + (line0 / N) * N * N + (line0 % N) * Tiled<N>::C;
-    const unsigned yl = TILED ? Tiled<N>::lane(t, ll) : 0u;
+    // |x| of line `line` of the half starting at line0 (a half's LPB lines lie
+    // in one pattern: LPB | N)
+    auto magnitude = [&](size_t line0, V (&v)[E]) -> bool {
+        const int64_t line = (int64_t)line0 + ll;
+        const bool valid = line < nlines;
+        const int64_t base = (valid ? line : 0) * N;
+        const c2<float>* Yp = Y + (line0 / N) * N * N + (line0 % N) * Tiled<N>::C;
+        const unsigned yl = TILED ? Tiled<N>::lane(t, ll) : 0u;
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if constexpr (TILED)
+                v[m] = valid ? (Yp + Tiled<N>::template kpart<PL::rot(m)>())[yl] : mk<V>(0, 0);
+            else
+                v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
+        });
+        if constexpr (FROM_STATE) fft_line<PL, +1>(v, t, tw, acc);
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const float sc = FROM_STATE ? invN : 1.f;
+            v[m] = mk<V>(sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc, 0.f);
+        });
+        return valid;
+    };
+    const size_t lineA0 = (size_t)blockIdx.x * (2 * LPB), lineB0 = lineA0 + LPB;
+    V v[E], w[E];
+    const bool okA = magnitude(lineA0, v);
+    const bool okB = (int64_t)lineB0 < nlines ? magnitude(lineB0, w) : false;
+    if ((int64_t)lineB0 >= nlines) {  // block-uniform: no second half
+        sfor<E>([&](auto mc) { w[decltype(mc)::value] = mk<V>(0, 0); });
+    }
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        if constexpr (TILED)
-            v[m] = valid ? (Yp + Tiled<N>::template kpart<PL::rot(m)>())[yl] : mk<V>(0, 0);
-        else
-            v[m] = valid ? Y[base + t + PL::rot(m) * TT] : mk<V>(0, 0);
-    });
-    if constexpr (FROM_STATE) fft_line<PL, +1>(v, t, tw, acc);
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const float sc = FROM_STATE ? invN : 1.f;
-        v[m] = mk<V>(sqrtf(v[m].x * v[m].x + v[m].y * v[m].y) * sc, 0.f);
+        v[m] = mk<V>(v[m].x, w[m].x);
     });
     fft_line<PL, -1>(v, t, tw, acc);
     sfor<E>([&](auto mc) {
@@ -527,12 +545,12 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>*
         v[m] = cscale(v[m], expf(gcoef_c * d * d) * invN);
     });
     fft_line<PL, +1>(v, t, tw, acc);
-    if (valid) {
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            r1[base + t + PL::rot(m) * TT] = v[m].x;
-        });
-    }
+    const int64_t baseA = (int64_t)(lineA0 + ll) * N, baseB = (int64_t)(lineB0 + ll) * N;
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if (okA) r1[baseA + t + PL::rot(m) * TT] = v[m].x;
+        if (okB) r1[baseB + t + PL::rot(m) * TT] = v[m].y;
+    });
 }
 
 // Column stage: b = Re IDFT_c(g_r . DFT_c(r1)) / M on tiles of one pattern
