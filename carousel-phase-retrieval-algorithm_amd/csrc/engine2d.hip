@@ -16,9 +16,10 @@
 // (src/CPRA_MKL.cpp:94-110 over include/detail/MKL_Impl.inl).  All centring
 // shifts are register renamings (fft_core.hpp).
 //
-// Shrinkwrap (north-star addition) reuses the same passes with different
-// epilogues: |x| -> row DFT; column DFT, x Gaussian transfer, column IDFT;
-// row IDFT -> blurred image + per-pattern max; threshold -> mask.
+// Shrinkwrap (beyond the reference) blurs |x| in separable form: one row
+// pass (|x| -> row DFT -> row factor of the Gaussian transfer function -> row
+// IDFT, real part), one column pass (the column factor, with the per-pattern
+// maximum), then the threshold -> mask.
 #include <cmath>
 #include <cstring>
 #include <string>

@@ -76,7 +76,7 @@ int cpra_hip_free(cpra_hip_handle h, void* ptr);
 int cpra_hip_memcpy(cpra_hip_handle h, void* dst, const void* src, uint64_t bytes);
 /* Replaces CudaImpl::Sync (include/CUDA_Impl.cuh:99-103). */
 int cpra_hip_sync(cpra_hip_handle h);
-/* Graph capture of the handle's stream (north-star addition; SURVEY's
+/* Graph capture of the handle's stream (no reference counterpart;
  * graph_capture / graph_launch): every call made on the handle between
  * cpra_hip_graph_begin and cpra_hip_graph_end is recorded, not run, and the
  * instantiated graph replays it with cpra_hip_graph_launch on the handle's
@@ -165,7 +165,7 @@ int cpra_hip_complex2d_to_3d(cpra_hip_handle h, const void* slices, void* vol, c
                              uint64_t m, uint64_t n, uint64_t p, uint64_t l);
 /* Re-projection R: slices[p][u][v] = trilinear sample of the cube volume
  * [n][n][n] at the slice points above (0 outside).  complex_data selects
- * complex<T> or T elements.  No reference counterpart (north-star addition). */
+ * complex<T> or T elements.  No reference counterpart. */
 int cpra_hip_slice_extract(cpra_hip_handle h, const void* vol, void* slices, const void* angles,
                            uint64_t n, uint64_t p, int complex_data);
 

@@ -5,8 +5,8 @@
  * CpraImpl<T> interface: the reference spells its iterations out as
  * sequences of primitive calls in the drivers (src/CPRA_MKL.cpp:37-55,
  * 94-110, 126-140; src/benchmark_mkl.cpp:45-63).  The engines run the same
- * mathematics (or the north-star additions: HIO, shrinkwrap, R-factor) as a
- * few fused HBM passes per iteration.  FP32 (complex64) handles only; the
+ * mathematics (or the additions beyond the reference: HIO, shrinkwrap,
+ * R-factor) as a few fused HBM passes per iteration.  FP32 (complex64) handles only; the
  * pattern size {m, n} of the handle must be in {64,128,192,256,384,512}.
  */
 #ifndef CPRA_HIP_ENGINE_H
@@ -100,7 +100,7 @@ int cpra_hip_rfactor(cpra_hip_handle h, const void* x, const float* amp, uint64_
 int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t* mask,
                                      uint64_t num);
 
-/* Autotuner (north-star addition): times each column-kernel variant
+/* Autotuner (no reference counterpart): times each column-kernel variant
  * (thread mapping: 16-column blocks, per-wave columns or 32-column blocks of
  * 512 threads, x XCD-aware tile order) for `iters` eager HIO iterations on
  * the caller's data, then dense against sparse row passes (rows without a
@@ -139,8 +139,8 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 int cpra_hip_set_kernel_timing(cpra_hip_handle h, int enable);
 int cpra_hip_get_kernel_times(cpra_hip_handle h, double* ms_total, int64_t* counts, int n);
 
-/* ---- amplitude loss and its adjoint (north-star addition; the reference has
- * no loss or gradient) ------------------------------------------------------
+/* ---- amplitude loss and its adjoint (the reference has no loss or
+ * gradient) -----------------------------------------------------------------
  * L_b = sum_k (|F x_b|_k - A_k)^2 with F the centred, unnormalised DFT of the
  * handle's 2D (m x n) or 3D (l x m x n) geometry, A [amp_batch][...] real,
  * broadcast over the batch by b % amp_batch.  The gradient follows the
@@ -168,7 +168,7 @@ int cpra_hip_amp_gd_run(cpra_hip_handle h, void* x, const float* amp, uint64_t a
 int cpra_hip_amp_gd_run_3d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                            int iters, float mu, double* loss_host);
 
-/* ---- mixed-size batch (north-star addition) ---------------------------------
+/* ---- mixed-size batch (no reference counterpart) ----------------------------
  * Difference map (textbook map, p->textbook must be 1; beta, space_mode,
  * dc_skip_zero, iters, use_graph as in cpra_hip_dm_run) over size classes of
  * square patterns.  Class k: count patterns of n x n (n in
@@ -196,7 +196,7 @@ typedef struct {
 int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes, int nclasses,
                           const cpra_hip_dm_params* p, int launch_mode);
 
-/* ---- 3D carousel (north-star addition) --------------------------------------
+/* ---- 3D carousel (no reference counterpart) ---------------------------------
  * Projections of one real object at angles th_p about the z axis, their
  * diffraction amplitudes measured.  One outer iteration: inner_iters HIO
  * iterations on every projection (real-positive inside support2d); F_p =

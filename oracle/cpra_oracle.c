@@ -5,8 +5,8 @@
  * this library.  It restates the reference's MKL backend
  * (include/detail/MKL_Impl.inl) and driver compositions (src/CPRA_MKL.cpp)
  * in plain C; FFTs are an independent FP64 mixed-radix Stockham transform.
- * Each function cites the reference lines it follows; the north-star
- * additions cite the textbook definition they implement.
+ * Each function cites the reference lines it follows; the additions beyond
+ * the reference cite the textbook definition they implement.
  */
 #include "cpra_oracle.h"
 

@@ -7,7 +7,7 @@
  *
  * It is a plain-C restatement of the reference's MKL backend semantics
  * (include/detail/MKL_Impl.inl) and of the reference driver compositions
- * (src/CPRA_MKL.cpp, src/CONV_MKL.cpp), plus the north-star additions that
+ * (src/CPRA_MKL.cpp, src/CONV_MKL.cpp), plus the additions that
  * the reference does not contain (HIO, shrinkwrap, R-factor, slice
  * merge/extraction, amplitude-loss gradient), all in FP64 unless a function
  * name ends in _f32 (those mirror the reference's single-precision pointwise
@@ -157,7 +157,7 @@ int orc_rfactor(const double* x, const double* A, int64_t M, int64_t N, int64_t 
 int orc_converge_error(double* old_, const double* new_, double* err, int64_t m, int64_t n,
                        int64_t l, int64_t batch);
 
-/* ---- carousel slice geometry (north-star gather-trilinear merge) ---------
+/* ---- carousel slice geometry (gather-trilinear merge) ---------------------
  * Volume [L][M][N] (index z, y, x), cube of side n = L = M = N, centre c = n/2.
  * Slice p (angle th_p) is an [M][N] image with rows u (along the rotation
  * axis z) and columns v; its sample (u, v) sits at the volume point
@@ -173,7 +173,7 @@ int orc_slice_scatter(const double* slices, int64_t n, const double* angles, int
 int orc_slice_merge(const double* slices, int64_t n, const double* angles, int64_t P,
                     double* vol, int complex_data);
 
-/* ---- amplitude loss (north star) ------------------------------------------
+/* ---- amplitude loss --------------------------------------------------------
  * L_b = sum_k (|F x_b|_k - A_k)^2 with F = Forward2D (centred, unnormalised).
  * grad = dL/d(re) + i dL/d(im) = 2*M*N*(x - Ptilde_M x) where Ptilde_M is
  * the modulus projection without the A == 0 skip.  For volumes use *_3d. */

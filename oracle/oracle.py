@@ -424,7 +424,7 @@ def amp_grad_3d(x, A):
 
 
 def carousel3d(x, A, support2d, angles, support3d, outer, inner=1, beta=0.9, dc_skip_zero=True):
-    """FP64 restatement of one 3D carousel run (the north-star carousel: the
+    """FP64 restatement of one 3D carousel run (the batched carousel: the
     reference's Jacobi-form sweep, src/CPRA_MKL.cpp:58-124, with the 3D
     consistency step): per outer iteration, `inner` HIO steps on every
     projection (orc_hio_step, real-positive), F = Forward2D(x), V = slice merge
