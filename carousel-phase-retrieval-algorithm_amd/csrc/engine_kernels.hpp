@@ -325,20 +325,26 @@ __global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
     __shared__ V lds[LPB * SPAN];
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
-    // Sparse rows of tile-major patterns: consecutive blocks (which the
-    // dispatcher deals to different XCDs) take the same row block of
-    // consecutive patterns, so every XCD gets its share of the support rows
-    // whatever their position in the pattern.  The grid is whole patterns.
-    // natural_order[0] != 0 (rows judged dense, every row bit set): the
-    // natural order, which was faster there (152 against 188 us per pass at
-    // 512 x 256^2).  Rotating each pattern's row blocks by the pattern index
-    // instead balances the XCDs without reading the flag, but measured slower
-    // than the better of the two orders at every support fraction tried.
+    // Sparse rows of tile-major patterns.  The dispatcher deals consecutive
+    // blocks to the eight XCDs in turn, so in the natural order row block j
+    // of every pattern lands on the same XCD, and the support rows (the
+    // middle rows of every pattern) on a few XCDs only.  With whole groups of
+    // eight patterns per launch, block i goes to pattern 8 g + i % 8 of group
+    // g: each XCD walks through the row blocks of its own patterns in order,
+    // which balances the XCDs whatever the support and keeps each XCD's reads
+    // within one pattern.  Otherwise consecutive blocks take the same row
+    // block of consecutive patterns, unless natural_order[0] != 0 (rows
+    // judged dense, every row bit set), where the natural order measured
+    // faster (152 against 188 us per pass at 512 x 256^2).  The grid is
+    // whole patterns.
     unsigned vb = blockIdx.x;
     if constexpr (SPARSE && TILED) {
-        if (natural_order == nullptr || natural_order[0] == 0u) {
-            constexpr unsigned BPP = N / LPB;
-            const unsigned npat = gridDim.x / BPP;
+        constexpr unsigned BPP = N / LPB;
+        const unsigned npat = gridDim.x / BPP;
+        if (npat % 8 == 0) {
+            const unsigned g = blockIdx.x / (8 * BPP);
+            vb = (g * 8 + (blockIdx.x & 7u)) * BPP + (blockIdx.x >> 3) % BPP;
+        } else if (natural_order == nullptr || natural_order[0] == 0u) {
             vb = (blockIdx.x % npat) * BPP + blockIdx.x / npat;
         }
     }

@@ -32,13 +32,14 @@ def boundary_pixels(oracle, x, data, rp):
 
 
 @pytest.mark.parametrize("n,P,starts", [(64, 2, 2), (256, 3, 1), (128, 1, 4), (192, 2, 1),
-                                        (384, 1, 1), (512, 1, 1)])
+                                        (384, 1, 1), (512, 1, 1), (256, 8, 1)])
 @pytest.mark.parametrize("rp", [1, 0])
 @pytest.mark.parametrize("sparse", [1, 0])
 def test_hio_steps_match_oracle(hipmod, oracle, gpu, synth, n, P, starts, rp, sparse):
     """Each GPU iteration against the FP64 oracle's iteration from the same
     (GPU) state: 1e-5 relative L2 outside decision-boundary pixels.  sparse=1
-    finishes rows without support pixels in the column kernel."""
+    finishes rows without support pixels in the column kernel (with eight
+    patterns, the sparse row pass deals each XCD whole patterns)."""
     data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, starts_per_pattern=starts,
                                config=70 + n)
     # a tight support (the object disc) so that many rows have no support pixel
