@@ -675,13 +675,18 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
 // One HIO iteration on the row-spectral state Y of the handle's batch, for
 // callers that compose their own iteration body (the carousel driver).
 // engine_hio_prepare sizes the workspaces and lists the support rows; it must
-// run (eagerly) before any capture of engine_hio_iteration.
+// run (eagerly) before any capture of engine_hio_iteration.  The carousel's
+// support is fixed for the whole run, so outside a capture the adaptive choice
+// is read back once (one synchronisation) and a dense choice runs the dense
+// kernels, which skip the sparse kernels' row-bit reads and register copies.
 void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support)
 {
     engine_check_geometry(h, "carousel");
     const Geo g = geo_of(h);
     h->ws_z.ensure(sizeof(c2<float>) * (size_t)g.M * g.N * g.B);
     h->hio_sparse = decide_sparse(h, st, support);
+    if (h->hio_sparse && sparse_mode(h) == SPARSE_AUTO && !h->capturing && st == h->stream)
+        h->hio_sparse = rows_sparse_used(h) == 1;
 }
 
 void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
