@@ -431,28 +431,26 @@ __global__ __launch_bounds__(256) void k_merge_fill(const double* __restrict__ t
 
 // Columns near the rotation axis are covered by every angle (at 180 x 128^2:
 // up to 536 list entries against a mean of 5.5).  Columns with more than
-// kMergeHeavy entries get a block of their own (k_merge_heavy); the rest run
-// one wave per column with four entries' loads in flight.
+// kMergeHeavy entries get a block of their own; the rest run one wave per
+// column with four entries' loads in flight.
 constexpr int kMergeHeavy = 32;
+constexpr int kMergeXT = 16;     // light columns per block (one wave each)
+constexpr int kHeavyWaves = 16;  // waves per heavy column
+static_assert(kMergeXT == kHeavyWaves, "both block kinds use 1024 threads and n * 16 elements of LDS");
 
-// Block = 16 consecutive x at one y, one wave per column (a short chain of
-// dependent loads per wave); lanes over z.  Heavy columns are written as 0
-// here and overwritten by k_merge_heavy.
+// Light block: 16 consecutive x at one y, one wave per column (a short chain
+// of dependent loads per wave), lanes over z; the columns are staged in LDS
+// and stored z-major.  Heavy columns are left to their own blocks.
 template <typename E, typename T>
-__global__ __launch_bounds__(1024) void k_merge_list(const E* __restrict__ sT, const int* __restrict__ offsets,
-                                                    const int* __restrict__ lidx, const T* __restrict__ lw,
-                                                    const T* __restrict__ inv, int n, E* __restrict__ vol)
+__device__ __forceinline__ void merge_light_block(unsigned t_id, const E* __restrict__ sT,
+                                                  const int* __restrict__ offsets,
+                                                  const int* __restrict__ lidx, const T* __restrict__ lw,
+                                                  const T* __restrict__ inv, int n, E* __restrict__ vol,
+                                                  E* tile, int* skip)
 {
     using Ops = ElemOps<E>;
-    constexpr int XT = 16;
-    extern __shared__ unsigned char smem[];
-    E* tile = reinterpret_cast<E*>(smem);  // [n][XT]
+    constexpr int XT = kMergeXT;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // XCD-aware tile order (the grid is a multiple of 8): XCD x takes the
-    // contiguous tile range [x*G/8, (x+1)*G/8), a band of y, so voxel columns
-    // that read the same slice columns share that XCD's L2
-    const unsigned G = gridDim.x, b = blockIdx.x;
-    const unsigned t_id = (b & 7u) * (G >> 3) + (b >> 3);
     const int tiles_x = n / XT;
     const int y = (int)t_id / tiles_x, x0 = ((int)t_id % tiles_x) * XT;
     {
@@ -460,10 +458,12 @@ __global__ __launch_bounds__(1024) void k_merge_list(const E* __restrict__ sT, c
         const int col = y * n + x0 + xl;
         int e0 = offsets[col];
         const int e1 = offsets[col + 1];
+        const bool heavy = e1 - e0 > kMergeHeavy;
+        if (lane == 0) skip[xl] = heavy ? 1 : 0;
         E acc[kMergeMaxZPerLane];
 #pragma unroll
         for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
-        if (e1 - e0 <= kMergeHeavy) {
+        if (!heavy) {
             // four entries per round: their loads are independent
             for (; e0 + 4 <= e1; e0 += 4) {
                 const E* cp[4];
@@ -505,27 +505,23 @@ __global__ __launch_bounds__(1024) void k_merge_list(const E* __restrict__ sT, c
     __syncthreads();
     for (int i = threadIdx.x; i < n * XT; i += blockDim.x) {
         const int z = i / XT, xl = i % XT;
-        vol[((int64_t)z * n + y) * n + x0 + xl] = tile[i];
+        if (!skip[xl]) vol[((int64_t)z * n + y) * n + x0 + xl] = tile[i];
     }
 }
 
-// One block of 16 waves per heavy column: wave w sums the entries
-// e0 + w, e0 + w + 16, ... (four in flight), then the 16 partial columns are
-// added in LDS in a fixed order.
-constexpr int kHeavyWaves = 16;
-
+// Heavy block: 16 waves on one column; wave w sums the entries e0 + w,
+// e0 + w + 16, ... (four in flight), then the 16 partial columns are added in
+// LDS in a fixed order.
 template <typename E, typename T>
-__global__ __launch_bounds__(64 * kHeavyWaves) void k_merge_heavy(
-    const E* __restrict__ sT, const int* __restrict__ offsets, const int* __restrict__ lidx,
-    const T* __restrict__ lw, const T* __restrict__ inv, const int* __restrict__ heavy, int n,
-    E* __restrict__ vol)
+__device__ __forceinline__ void merge_heavy_block(int col, const E* __restrict__ sT,
+                                                  const int* __restrict__ offsets,
+                                                  const int* __restrict__ lidx, const T* __restrict__ lw,
+                                                  const T* __restrict__ inv, int n, E* __restrict__ vol,
+                                                  E* part)
 {
     using Ops = ElemOps<E>;
     constexpr int NW = kHeavyWaves;
-    extern __shared__ unsigned char smem[];
-    E* part = reinterpret_cast<E*>(smem);  // [NW][n]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = heavy[blockIdx.x];
     const int e0 = offsets[col], e1 = offsets[col + 1];
     E acc[kMergeMaxZPerLane];
 #pragma unroll
@@ -574,6 +570,33 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void k_merge_heavy(
         for (int w = 1; w < NW; ++w) v = Ops::fma((T)1, part[w * n + z], v);
         vol[((int64_t)z * n + y) * n + x] = Ops::scale(v, s);
     }
+}
+
+// One launch for the whole merge: the first heavy_blocks blocks (nheavy
+// rounded up to a multiple of 8) take the heavy columns, longest chains
+// first, so they overlap the light blocks; the light blocks follow, n/16 * n
+// of them (a multiple of 8).  XCD-aware light order: block b runs on XCD
+// b mod 8, and XCD x takes the contiguous tile range [x*G/8, (x+1)*G/8), a
+// band of y, so voxel columns that read the same slice columns share that
+// XCD's L2.  Every voxel is written by exactly one block.
+template <typename E, typename T>
+__global__ __launch_bounds__(1024) void k_merge_apply(const E* __restrict__ sT, const int* __restrict__ offsets,
+                                                     const int* __restrict__ lidx, const T* __restrict__ lw,
+                                                     const T* __restrict__ inv, const int* __restrict__ heavy,
+                                                     int nheavy, int heavy_blocks, int n, E* __restrict__ vol)
+{
+    extern __shared__ unsigned char smem[];
+    __shared__ int skip[kMergeXT];
+    E* buf = reinterpret_cast<E*>(smem);  // [n][16] light tile or [16][n] heavy partials
+    const int b = (int)blockIdx.x;
+    if (b < heavy_blocks) {
+        if (b < nheavy) merge_heavy_block<E, T>(heavy[b], sT, offsets, lidx, lw, inv, n, vol, buf);
+        return;  // block-uniform
+    }
+    const unsigned lb = (unsigned)(b - heavy_blocks);
+    const unsigned G = gridDim.x - (unsigned)heavy_blocks;
+    const unsigned t_id = (lb & 7u) * (G >> 3) + (lb >> 3);
+    merge_light_block<E, T>(t_id, sT, offsets, lidx, lw, inv, n, vol, buf, skip);
 }
 
 // Builds the gather list of (angles, n, P) in the handle (synchronises once).
@@ -650,21 +673,17 @@ void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int 
     const size_t idx_bytes = ((sizeof(int) * ent + 255) / 256) * 256;
     const size_t w_bytes = ((sizeof(float) * ent + 255) / 256) * 256;
     char* base = static_cast<char*>(h->ws_mlist.ptr);
-    // n / 16 * n tiles: a multiple of 8 for every n that is a multiple of 16
-    hipLaunchKernelGGL((k_merge_list<E, float>), dim3((n / 16) * n), dim3(1024), sizeof(E) * (size_t)n * 16,
-                       h->stream, sT, static_cast<const int*>(h->ws_mcount.ptr),
-                       reinterpret_cast<const int*>(base), reinterpret_cast<const float*>(base + idx_bytes),
-                       reinterpret_cast<const float*>(base + idx_bytes + w_bytes), n, static_cast<E*>(vol));
-    check_launch("k_merge_list");
-    if (h->mplan_heavy > 0) {
-        hipLaunchKernelGGL((k_merge_heavy<E, float>), dim3(h->mplan_heavy), dim3(64 * kHeavyWaves),
-                           sizeof(E) * kHeavyWaves * (size_t)n,
-                           h->stream, sT, static_cast<const int*>(h->ws_mcount.ptr),
-                           reinterpret_cast<const int*>(base), reinterpret_cast<const float*>(base + idx_bytes),
-                           reinterpret_cast<const float*>(base + idx_bytes + w_bytes),
-                           static_cast<const int*>(h->ws_mheavy.ptr), n, static_cast<E*>(vol));
-        check_launch("k_merge_heavy");
-    }
+    // n / 16 * n light tiles: a multiple of 8 for every n that is a multiple of 16
+    const int nheavy = h->mplan_heavy;
+    const int heavy_blocks = (nheavy + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_merge_apply<E, float>), dim3(heavy_blocks + (n / 16) * n), dim3(1024),
+                       sizeof(E) * (size_t)n * kMergeXT, h->stream, sT,
+                       static_cast<const int*>(h->ws_mcount.ptr), reinterpret_cast<const int*>(base),
+                       reinterpret_cast<const float*>(base + idx_bytes),
+                       reinterpret_cast<const float*>(base + idx_bytes + w_bytes),
+                       static_cast<const int*>(h->ws_mheavy.ptr), nheavy, heavy_blocks, n,
+                       static_cast<E*>(vol));
+    check_launch("k_merge_apply");
 }
 
 // complex64 merge / extraction on the handle's stream (the carousel driver)
