@@ -193,8 +193,9 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=5, timing_iter
         "column": (px2 * COL_BYTES_PER_PX, 1),
         "row": (px2 * ROW_BYTES_PER_PX, 1),
         "proj_fft": (2 * px2 * 16, 2),                 # column DFT + column IDFT
-        "merge": (px2 * 8 * 3 + vx * 8, 3),            # transpose r+w, gather r, volume w
-        "volume": (5 * vx * 16 + vx, 5),               # five FFT passes + the support mask
+        "merge": (px2 * 8 + vx * 8, 1),                # gather r (slices already [P][v][u]), volume w
+        # n <= 128: two whole-plane kernels and the z-line kernel; above, five passes
+        "volume": ((3 if n <= 128 else 5) * vx * 16 + vx, 3 if n <= 128 else 5),
         "extract": (px2 * 8 + vx * 8, 2),
     }
     kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}

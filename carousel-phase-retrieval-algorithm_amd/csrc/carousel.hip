@@ -251,12 +251,18 @@ struct Carousel {
     {
         const int64_t W = (int64_t)n * n;
         for (int i = 0; i < inner; ++i) engine_hio_iteration(h, h->stream, Y, amp, amp_batch, sup2d, &hp, kt);
-        // Y -> F = Forward2D(x): the column DFT of the row-spectral state
+        // Y -> F = Forward2D(x): the column DFT of the row-spectral state,
+        // written straight into the merge's layout [P][v][u] where a kernel
+        // for n exists and tuning key 6 allows it (the extraction below
+        // overwrites Y in full, so Y need not hold F)
         if (kt) kt->begin(h, h->stream, KT_PFFT);
-        fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, -1, true, 1.0, false);
+        void* sT = merge_lines_ws(h, n, P);
+        const bool lines = engine_merge_lines(h) && fft_pass_strided_lines_fwd(h, Y, sT, n, (int64_t)P * n, n, n, W);
+        if (!lines) fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, -1, true, 1.0, false);
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_MERGE);
-        merge_apply_c64(h, Y, vol, n, P);
+        if (lines) merge_apply_lines_c64(h, sT, vol, n, P);
+        else merge_apply_c64(h, Y, vol, n, P);
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_VOL);
         volume_constraint(ZL_ROUNDTRIP);

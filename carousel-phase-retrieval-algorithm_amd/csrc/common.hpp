@@ -146,6 +146,12 @@ void fft_pass_contig(cpra_hip_ctx* h, void* data, int n, int64_t lines, int dir,
                      double scale, bool dbl);
 void fft_pass_strided(cpra_hip_ctx* h, void* data, int n, int64_t lines, int64_t es, int64_t G,
                       int64_t GS, int dir, bool shift, double scale, bool dbl);
+// Forward centred column transform of complex64 lines (as fft_pass_strided
+// with dir -1, shift on, no scale) written line-major to out[line * n + u].
+// Returns false, launching nothing, for sizes it has no kernel for (n other
+// than 64, 128, 256, or lines / G not whole tiles).
+bool fft_pass_strided_lines_fwd(cpra_hip_ctx* h, const void* data, void* out, int n, int64_t lines,
+                                int64_t es, int64_t G, int64_t GS);
 bool fft_fast_size(int n);
 // Event pair around one engine kernel when the handle's timing is on; the
 // elapsed time is read back after the run (kernel_timer_collect).
@@ -197,6 +203,7 @@ enum KernelKind {
 bool engine_tiled_state(const cpra_hip_ctx* h);
 // whole-plane x/y transforms in the carousel's volume constraint (tuning plane=)
 bool engine_plane_fft(const cpra_hip_ctx* h);
+bool engine_merge_lines(const cpra_hip_ctx* h);
 void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void* in, void* out);
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out);
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)
@@ -210,6 +217,8 @@ void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* an
 // merge through a gather list built once per geometry (synchronises)
 void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P);
 void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P);
+void* merge_lines_ws(cpra_hip_ctx* h, int n, int P);
+void merge_apply_lines_c64(cpra_hip_ctx* h, const void* sT, void* vol, int n, int P);
 
 // frees the captured engine graphs of a handle (engine2d.hip)
 void cpra_engine_release(cpra_hip_ctx* h);

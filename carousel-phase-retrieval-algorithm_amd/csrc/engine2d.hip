@@ -641,6 +641,15 @@ bool engine_plane_fft(const cpra_hip_ctx* h)
     return it == h->tuning.end() || it->second != 0;
 }
 
+// The carousel's forward column DFT written straight into the merge's layout
+// (n in {64, 128, 256}), tuning key 6 (1 = on, the default; 0 = the DFT in
+// place and a transpose before the merge).
+bool engine_merge_lines(const cpra_hip_ctx* h)
+{
+    auto it = h->tuning.find((uint64_t)6 << 32);
+    return it == h->tuning.end() || it->second != 0;
+}
+
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
 {
     launch_amp_retile(st, n, patterns, in, out);
@@ -1357,7 +1366,7 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
     });
 }
 
-// Applies the col_<M>=, rows_sparse=, chunk=, tiled= and plane= entries of a
+// Applies the col_<M>=, rows_sparse=, chunk=, tiled=, plane= and merge_lines= entries of a
 // spec such as an autotune report; other keys are skipped.
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
 {
@@ -1380,6 +1389,9 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
             } else if (std::sscanf(p, "plane=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)5 << 32] = v ? 1 : 0;
                 p += used;
+            } else if (std::sscanf(p, "merge_lines=%d;%n", &v, &used) == 1 && used > 0) {
+                h->tuning[(uint64_t)6 << 32] = v ? 1 : 0;
+                p += used;
             } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
                 CPRA_ARG(col_variant_valid(v), "cpra_hip_set_tuning: variant %d out of range", v);
                 h->tuning[((uint64_t)1 << 32) | (uint64_t)m] = v;
@@ -1399,9 +1411,11 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
     return eguard(h, [&]() -> int {
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
-        std::snprintf(out, out_len, "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;rows_sparse_used=%d;",
+        std::snprintf(out, out_len,
+                      "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;rows_sparse_used=%d;",
                       (int)h->m, col_variant_code(cv), sparse_mode(h), (long long)iteration_chunk(h),
-                      tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, rows_sparse_used(h));
+                      tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, engine_merge_lines(h) ? 1 : 0,
+                      rows_sparse_used(h));
         return CPRA_HIP_OK;
     });
 }

@@ -658,17 +658,21 @@ void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
     h->mplan_heavy = (int)heavy.size();
 }
 
-// R^T(s)/R^T(1) of complex64 slices through the list of merge_plan_c64.
-void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P)
+// Workspace of the slices in the merge's layout [P][v][u] (line-major
+// columns), valid until the next merge on the handle.
+void* merge_lines_ws(cpra_hip_ctx* h, int n, int P)
+{
+    h->ws_misc.ensure((size_t)P * n * n * sizeof(c2<float>));
+    return h->ws_misc.ptr;
+}
+
+// R^T(s)/R^T(1) through the list of merge_plan_c64, from slices already in the
+// merge's layout sT[p][v][u].
+void merge_apply_lines_c64(cpra_hip_ctx* h, const void* sT_, void* vol, int n, int P)
 {
     CPRA_ARG(h->mplan_n == n && h->mplan_P == P, "carousel merge: no gather list for this geometry");
     using E = c2<float>;
-    const size_t per = (size_t)P * n * n;
-    h->ws_misc.ensure(per * sizeof(E));
-    E* sT = static_cast<E*>(h->ws_misc.ptr);
-    hipLaunchKernelGGL((k_transpose_slices<E>), dim3((n + 31) / 32, (n + 31) / 32, P), dim3(256), 0,
-                       h->stream, static_cast<const E*>(slices), sT, n);
-    check_launch("k_transpose_slices");
+    const E* sT = static_cast<const E*>(sT_);
     const size_t ent = (size_t)(h->mplan_entries > 0 ? h->mplan_entries : 1);
     const size_t idx_bytes = ((sizeof(int) * ent + 255) / 256) * 256;
     const size_t w_bytes = ((sizeof(float) * ent + 255) / 256) * 256;
@@ -684,6 +688,19 @@ void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int 
                        static_cast<const int*>(h->ws_mheavy.ptr), nheavy, heavy_blocks, n,
                        static_cast<E*>(vol));
     check_launch("k_merge_apply");
+}
+
+// R^T(s)/R^T(1) of complex64 slices [P][u][v] through the list of
+// merge_plan_c64 (transposed into the merge's layout first).
+void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P)
+{
+    CPRA_ARG(h->mplan_n == n && h->mplan_P == P, "carousel merge: no gather list for this geometry");
+    using E = c2<float>;
+    E* sT = static_cast<E*>(merge_lines_ws(h, n, P));
+    hipLaunchKernelGGL((k_transpose_slices<E>), dim3((n + 31) / 32, (n + 31) / 32, P), dim3(256), 0,
+                       h->stream, static_cast<const E*>(slices), sT, n);
+    check_launch("k_transpose_slices");
+    merge_apply_lines_c64(h, sT, vol, n, P);
 }
 
 // complex64 merge / extraction on the handle's stream (the carousel driver)

@@ -83,6 +83,25 @@ def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P):
         assert h.get_tuning()["plane"] == "0"
 
 
+@pytest.mark.parametrize("n,P", [(64, 36), (128, 60), (256, 16)])
+def test_carousel_merge_lines_match_transpose(hipmod, gpu, synth, n, P):
+    """The forward column DFT writes the merge's [P][v][u] layout directly
+    (tuning merge_lines=1, the default); merge_lines=0 transforms in place and
+    transposes before the merge.  The same plan, twiddles and summation order
+    run in both, so the results are bit-identical."""
+    d = synth.make_carousel(n, P, config=54)
+    x1, v1, _ = run_gpu(hipmod, d, 2, use_graph=0, tuning="merge_lines=1;")
+    x0, v0, _ = run_gpu(hipmod, d, 2, use_graph=0, tuning="merge_lines=0;")
+    print(f"merge lines vs transpose n={n} P={P}: x {relerr(x1, x0):.2e} vol {relerr(v1, v0):.2e}")
+    assert np.array_equal(x1, x0)
+    assert np.array_equal(v1, v0)
+    assert np.abs(v1).max() > 0
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        assert h.get_tuning()["merge_lines"] == "1"
+        assert h.set_tuning("merge_lines=0;")
+        assert h.get_tuning()["merge_lines"] == "0"
+
+
 def test_carousel_rfactor_falls(hipmod, gpu, synth):
     d = synth.make_carousel(64, 36, config=51)
     # the start has the measured magnitudes exactly (R = 0); the first 3D
