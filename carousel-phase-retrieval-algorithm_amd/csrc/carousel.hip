@@ -216,6 +216,91 @@ static void zline_launch(hipStream_t st, int n, int mode, c2<float>* V, const ui
     check_launch("k_zline_constraint");
 }
 
+// First HIO column pass of an outer iteration after the extraction, which
+// leaves the projections' 2D spectra F in the state buffer.  The unfused
+// sequence is a column IDFT pass (Y = IDFT_c(F)/n) and the column kernel's
+// DFT_c(Y), which gives F back up to rounding.  This kernel instead runs the
+// modulus step on F itself and writes both Y = IDFT_c(F)/n (in place) and
+// Z = IDFT_c(modulus(F)) for the row kernel: one launch fewer, and 28 B/px
+// (read F, A; write Y, Z) against 16 + 20.  Modulus step as in the engine's
+// column kernel: F * A/|F| scaled by 1/n^2, F kept (scaled) where |F| = 0 or,
+// with skip_zero, where A = 0.  Dense row-major state, block = C columns of
+// one projection.
+template <int N>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_car_cols_from_spectrum(
+    c2<float>* YF, c2<float>* __restrict__ Z, const float* __restrict__ A, int64_t amp_batch,
+    float scale, float inv_n, int skip_zero)
+{
+    using V = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, C = SizeCfg<N>::C;
+    constexpr unsigned TPP = N / C;  // column tiles per projection
+    __shared__ V lds[N * C];
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const unsigned b = blockIdx.x / TPP;
+    const int col0 = (int)(blockIdx.x - b * TPP) * C;
+    const size_t pat = (size_t)N * N;
+    V* Yp = YF + b * pat + col0;
+    V* Zp = Z + b * pat + col0;
+    const float* Ap = A + (size_t)(b % (unsigned)amp_batch) * pat + col0;
+    // element m of this lane: row t + rot(m)*T (the centring shift folded in)
+    const unsigned toff = (unsigned)t * N + (unsigned)c;
+    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)N; };
+    V v[E], u[E];
+    float av[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = (Yp + rowbase(m))[toff];
+    });
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        av[m] = (Ap + rowbase(m))[toff];
+    });
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float a = av[m];
+        const V x = v[m];
+        const float m2 = x.x * x.x + x.y * x.y;
+        float sc = scale;
+        if (!(skip_zero && a == 0.f) && m2 > 0.f) sc = scale * a * rsqrtf(m2);
+        u[m] = cscale(x, sc);
+    });
+    LdsCol<V, C, false> acc{lds + c};
+    fft_line<PL, +1>(v, t, g_tw<N>(), acc);
+    fft_line<PL, +1>(u, t, g_tw<N>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        (Yp + rowbase(m))[toff] = cscale(v[m], inv_n);
+        (Zp + rowbase(m))[toff] = u[m];
+    });
+}
+
+template <int N>
+static void cols_from_spectrum_n(hipStream_t st, int P, c2<float>* Y, c2<float>* Z, const float* A,
+                                 int64_t amp_batch, int skip_zero)
+{
+    const float scale = (float)(1.0 / ((double)N * N));
+    const float inv_n = (float)(1.0 / (double)N);
+    hipLaunchKernelGGL((k_car_cols_from_spectrum<N>), dim3((unsigned)(P * (N / SizeCfg<N>::C))),
+                       dim3(SizeCfg<N>::TPB), 0, st, Y, Z, A, amp_batch, scale, inv_n, skip_zero);
+}
+
+static void cols_from_spectrum_launch(hipStream_t st, int n, int P, c2<float>* Y, c2<float>* Z,
+                                      const float* A, int64_t amp_batch, int skip_zero)
+{
+    switch (n) {
+    case 64: cols_from_spectrum_n<64>(st, P, Y, Z, A, amp_batch, skip_zero); break;
+    case 128: cols_from_spectrum_n<128>(st, P, Y, Z, A, amp_batch, skip_zero); break;
+    case 192: cols_from_spectrum_n<192>(st, P, Y, Z, A, amp_batch, skip_zero); break;
+    case 256: cols_from_spectrum_n<256>(st, P, Y, Z, A, amp_batch, skip_zero); break;
+    case 384: cols_from_spectrum_n<384>(st, P, Y, Z, A, amp_batch, skip_zero); break;
+    case 512: cols_from_spectrum_n<512>(st, P, Y, Z, A, amp_batch, skip_zero); break;
+    default: fail(CPRA_HIP_ERR_UNSUPPORTED, "carousel: n=%d has no column kernel", n);
+    }
+    check_launch("k_car_cols_from_spectrum");
+}
+
 struct Carousel {
     cpra_hip_ctx* h;
     int n, P;
@@ -246,11 +331,43 @@ struct Carousel {
         }
     }
 
-    // one outer iteration on h->stream
-    void outer(KernelTimer* kt)
+    // F -> Y = DFT_rows(Backward2D(F)): the column IDFT scaled 1/n
+    void to_row_spectral(KernelTimer* kt)
     {
         const int64_t W = (int64_t)n * n;
-        for (int i = 0; i < inner; ++i) engine_hio_iteration(h, h->stream, Y, amp, amp_batch, sup2d, &hp, kt);
+        if (kt) kt->begin(h, h->stream, KT_PFFT);
+        fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, +1, true, 1.0 / (double)n, false);
+        if (kt) kt->end(h, h->stream);
+    }
+
+    // One HIO iteration whose input is F (left in Y by the extraction): with
+    // dense rows in one launch per pass the column pass starts from F itself;
+    // otherwise the column IDFT pass restores the row-spectral state first.
+    void hio_from_spectrum(KernelTimer* kt)
+    {
+        if (engine_hio_dense_whole(h)) {
+            if (kt) kt->begin(h, h->stream, KT_COL);
+            cols_from_spectrum_launch(h->stream, n, P, Y, static_cast<c2<float>*>(h->ws_z.ptr), amp, amp_batch,
+                                      hp.dc_skip_zero);
+            if (kt) kt->end(h, h->stream);
+            engine_hio_rows(h, h->stream, Y, sup2d, &hp, kt);
+            return;
+        }
+        to_row_spectral(kt);
+        engine_hio_iteration(h, h->stream, Y, amp, amp_batch, sup2d, &hp, kt);
+    }
+
+    // One outer iteration on h->stream.  On exit Y holds F, the 2D spectra of
+    // the re-projected slices; on entry it holds F too when from_spectrum
+    // (every outer iteration but the first), else the row-spectral state.
+    void outer(KernelTimer* kt, bool from_spectrum)
+    {
+        const int64_t W = (int64_t)n * n;
+        if (from_spectrum && inner == 0) to_row_spectral(kt);
+        for (int i = 0; i < inner; ++i) {
+            if (i == 0 && from_spectrum) hio_from_spectrum(kt);
+            else engine_hio_iteration(h, h->stream, Y, amp, amp_batch, sup2d, &hp, kt);
+        }
         // Y -> F = Forward2D(x): the column DFT of the row-spectral state,
         // written straight into the merge's layout [P][v][u] where a kernel
         // for n exists and tuning key 6 allows it (the extraction below
@@ -269,10 +386,6 @@ struct Carousel {
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_EXTRACT);
         extract_c64(h, vol, Y, angles, n, P);
-        if (kt) kt->end(h, h->stream);
-        // F -> Y = DFT_rows(Backward2D(F)): the column IDFT scaled 1/n
-        if (kt) kt->begin(h, h->stream, KT_PFFT);
-        fft_pass_strided(h, Y, n, (int64_t)P * n, n, n, W, +1, true, 1.0 / (double)n, false);
         if (kt) kt->end(h, h->stream);
     }
 };
@@ -350,7 +463,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         KernelTimer* ktp = h->timing ? &kt : nullptr;
         int done = 0;
         if (p->outer_iters > 0) {
-            cr.outer(ktp);  // eager: sizes every workspace before any capture
+            cr.outer(ktp, false);  // eager: sizes every workspace before any capture
             done = 1;
         }
         const int rest = p->outer_iters - done;
@@ -367,7 +480,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             h->stream = gst;
             try {
                 CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
-                cr.outer(nullptr);
+                cr.outer(nullptr, true);
                 CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
             } catch (...) {
                 h->stream = st;
@@ -385,8 +498,9 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             (void)hipEventDestroy(ev1);
             (void)hipStreamDestroy(gst);
         } else {
-            for (int i = 0; i < rest; ++i) cr.outer(ktp);
+            for (int i = 0; i < rest; ++i) cr.outer(ktp, true);
         }
+        if (p->outer_iters > 0) cr.to_row_spectral(ktp);  // F -> the row-spectral state
         // leave: x = IDFT_rows(Y) / n; vol = the constrained real-space object
         fft_pass_contig(h, x, n, (int64_t)cr.P * n, +1, true, 1.0 / (double)n, false);
         if (p->outer_iters > 0) cr.volume_constraint(ZL_INVERSE);

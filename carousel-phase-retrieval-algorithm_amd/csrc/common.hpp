@@ -211,6 +211,9 @@ void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support)
 void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
                           int64_t amp_batch, const uint8_t* support, const cpra_hip_hio_params* p,
                           KernelTimer* kt);
+bool engine_hio_dense_whole(const cpra_hip_ctx* h);
+void engine_hio_rows(cpra_hip_ctx* h, hipStream_t st, void* Y, const uint8_t* support,
+                     const cpra_hip_hio_params* p, KernelTimer* kt);
 // merge3d.hip: complex64 gather merge R^T(s)/R^T(1) and re-projection R
 void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P);
 void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* angles, int n, int P);

@@ -705,6 +705,23 @@ void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float*
     hio_iteration(h, st, Y, amp, amp_batch, support, p, h->hio_sparse, kt);
 }
 
+// Whether the carousel's iterations run dense rows in one launch per pass
+// (then a caller may run its own column pass into ws_z and engine_hio_rows).
+bool engine_hio_dense_whole(const cpra_hip_ctx* h)
+{
+    return !h->hio_sparse && iteration_chunk(h) == (int64_t)h->batch;
+}
+
+// The row pass of one dense HIO iteration: Y updated from Y and Z = ws_z.
+void engine_hio_rows(cpra_hip_ctx* h, hipStream_t st, void* Y, const uint8_t* support,
+                     const cpra_hip_hio_params* p, KernelTimer* kt)
+{
+    const Geo g = geo_of(h);
+    if (kt) kt->begin(h, st, KT_ROW);
+    launch_rows_hio(st, g, h->ws_z.ptr, Y, support, p->beta, p->real_positive, p->error_reduction);
+    if (kt) kt->end(h, st);
+}
+
 template <int N>
 static void dm_entry_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p,
                        const c2<float>* psi, const c2<float>* t1, const c2<float>* t2,
