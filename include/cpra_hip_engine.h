@@ -134,8 +134,9 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
  * stream it runs on; cpra_hip_get_kernel_times returns, per kernel kind
  * (0 = column pass, 1 = row pass, 2 = one shrinkwrap update, 3 = fused
  * gradient-descent pass, 4 = 2D->3D merge, 5 = 3D volume constraint (its
- * five FFT passes, three kernels for n <= 128), 6 = slice extraction, 7 = column pass of the carousel's
- * projections, 8 = one whole mixed-batch iteration of the per-class and
+ * five FFT passes, three kernels for n <= 128), 6 = slice extraction, 7 = column transforms of the
+ * carousel's projections outside its HIO passes (the carousel's kind 0 includes its column pass that
+ * starts from the re-projected spectra), 8 = one whole mixed-batch iteration of the per-class and
  * per-pattern modes; the fused mode records 0 and 1; 9 = the y-line pencil
  * passes of the 3D difference map), the summed milliseconds
  * and the launch count since the last enable (n <= 16 kinds). */
