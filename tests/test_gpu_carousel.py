@@ -64,6 +64,34 @@ def test_carousel_matches_oracle(hipmod, oracle, gpu, synth, n, P, outer, inner,
     assert (vol.real[d["support3d"] == 0] == 0).all()
 
 
+@pytest.mark.parametrize("radius_frac,expect_sparse", [(0.5, "0"), (0.25, "1")])
+def test_carousel_row_mode_follows_support(hipmod, oracle, gpu, synth, radius_frac, expect_sparse):
+    """The adaptive row mode is decided once per run from the 2D supports: the
+    full disc (every row holds support) runs the dense kernels and the fused
+    first column pass; a disc of radius n/4 (half the rows) runs sparse rows
+    and the unfused sequence.  Both match the oracle."""
+    n, P, outer = 64, 24, 2
+    d = synth.make_carousel(n, P, config=55)
+    k = np.arange(n) - n / 2
+    r = radius_frac * n
+    disc = ((k[:, None] ** 2 + k[None, :] ** 2) <= r * r).astype(np.uint8)
+    d["support2d"] = np.ascontiguousarray(np.broadcast_to(disc, (P, n, n)))
+    x = hipmod.to_device(d["x0"])
+    vol = hipmod.zeros((n, n, n), np.complex64)
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        h.carousel3d_run(x, hipmod.to_device(d["amp"]), hipmod.to_device(d["support2d"]),
+                         hipmod.to_device(d["angles"]), hipmod.to_device(d["support3d"]), vol,
+                         outer_iters=outer, inner_iters=1, beta=0.9, use_graph=1)
+        h.Sync()
+        used = h.get_tuning()["rows_sparse_used"]
+    xr, vr = oracle.carousel3d(d["x0"], d["amp"].astype(np.float64), d["support2d"],
+                               d["angles"].astype(np.float64), d["support3d"], outer, 1, 0.9, True)
+    ex, ev = relerr(x.numpy(), xr), relerr(vol.numpy(), vr)
+    print(f"carousel support radius {r}: rows_sparse_used={used} x {ex:.2e} vol {ev:.2e}")
+    assert used == expect_sparse
+    assert ex < 2e-5 * outer and ev < 2e-5 * outer
+
+
 def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth):
     d = synth.make_carousel(64, 36, config=50)
     xg, vg, _ = run_gpu(hipmod, d, 6, use_graph=1)
