@@ -149,7 +149,7 @@ void fft_pass_strided(cpra_hip_ctx* h, void* data, int n, int64_t lines, int64_t
 // Forward centred column transform of complex64 lines (as fft_pass_strided
 // with dir -1, shift on, no scale) written line-major to out[line * n + u].
 // Returns false, launching nothing, for sizes it has no kernel for (n other
-// than 64, 128, 256, or lines / G not whole tiles).
+// than 64, 128, 192, 256, 384, 512, or lines / G not whole tiles).
 bool fft_pass_strided_lines_fwd(cpra_hip_ctx* h, const void* data, void* out, int n, int64_t lines,
                                 int64_t es, int64_t G, int64_t GS);
 bool fft_fast_size(int n);

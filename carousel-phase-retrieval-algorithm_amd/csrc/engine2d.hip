@@ -642,7 +642,7 @@ bool engine_plane_fft(const cpra_hip_ctx* h)
 }
 
 // The carousel's forward column DFT written straight into the merge's layout
-// (n in {64, 128, 256}), tuning key 6 (1 = on, the default; 0 = the DFT in
+// (every carousel size), tuning key 6 (1 = on, the default; 0 = the DFT in
 // place and a transpose before the merge).
 bool engine_merge_lines(const cpra_hip_ctx* h)
 {

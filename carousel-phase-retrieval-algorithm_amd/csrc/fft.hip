@@ -370,14 +370,20 @@ bool fft_pass_strided_lines_fwd(cpra_hip_ctx* h, const void* data, void* out, in
     switch (n) {
     case 64: C = SizeCfg<64>::C; break;
     case 128: C = SizeCfg<128>::C; break;
+    case 192: C = SizeCfg<192>::C; break;
     case 256: C = SizeCfg<256>::C; break;
+    case 384: C = SizeCfg<384>::C; break;
+    case 512: C = SizeCfg<512>::C; break;
     default: return false;
     }
     if (lines <= 0 || G % C != 0 || lines % C != 0 || lines / C > 0x7fffffff) return false;
     switch (n) {
     case 64: launch_strided_lines<64>(h, data, out, lines, es, G, GS); break;
     case 128: launch_strided_lines<128>(h, data, out, lines, es, G, GS); break;
+    case 192: launch_strided_lines<192>(h, data, out, lines, es, G, GS); break;
     case 256: launch_strided_lines<256>(h, data, out, lines, es, G, GS); break;
+    case 384: launch_strided_lines<384>(h, data, out, lines, es, G, GS); break;
+    case 512: launch_strided_lines<512>(h, data, out, lines, es, G, GS); break;
     }
     return true;
 }

@@ -648,6 +648,11 @@ void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
                        lidx, lw, inv);
     check_launch("k_merge_fill");
     CPRA_CHECK_HIP(hipStreamSynchronize(st));  // `off` and `heavy` are released on return
+    // the merge's tile (n * 16 elements: 64 KiB at n = 512) beside its static
+    // flags can exceed the default dynamic-LDS limit
+    CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_merge_apply<c2<float>, float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(sizeof(c2<float>) * (size_t)n * kMergeXT)));
     if (sizeof(c2<float>) * (size_t)n * n <= kExtractPlaneLds)
         CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_extract_plane<c2<float>, float>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,

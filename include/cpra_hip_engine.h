@@ -120,8 +120,8 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * state for square patterns), "plane=<0|1>;" (whole-plane LDS transforms
  * in the carousel's volume constraint for n <= 128; 0 = one pass per axis),
  * and "merge_lines=<0|1>;" (the carousel's forward column DFT written
- * straight into the merge's [P][v][u] layout for n in {64, 128, 256}; 0 = the
- * DFT in place and a transpose before the merge). */
+ * straight into the merge's [P][v][u] layout; 0 = the DFT in place and a
+ * transpose before the merge). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
  * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;rows_sparse_used=<0|1>;" (the last: the row mode

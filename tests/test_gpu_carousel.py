@@ -118,7 +118,7 @@ def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P):
         assert h.get_tuning()["plane"] == "0"
 
 
-@pytest.mark.parametrize("n,P", [(64, 36), (128, 60), (256, 16)])
+@pytest.mark.parametrize("n,P", [(64, 36), (128, 60), (192, 16), (256, 16), (384, 8), (512, 8)])
 def test_carousel_merge_lines_match_transpose(hipmod, gpu, synth, n, P):
     """The forward column DFT writes the merge's [P][v][u] layout directly
     (tuning merge_lines=1, the default); merge_lines=0 transforms in place and
