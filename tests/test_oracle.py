@@ -424,3 +424,246 @@ def test_amp_grad_3d_finite_differences(oracle):
         xm[idx] -= h
         fd = (oracle.amp_loss_3d(xp, A) - oracle.amp_loss_3d(xm, A)) / (2 * h)
         assert abs(fd - g[idx].real) <= 1e-6 * max(1.0, np.abs(g).max())
+
+
+# ------------------------------------------------- schedules and drivers -------
+
+def hio_problem(n=32, B=2, seed=30):
+    """Disc objects with their exact amplitudes and a loose square support,
+    random starts: shrinkwrap then moves the support at every update."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:n, 0:n]
+    A = np.zeros((B, n, n))
+    for b in range(B):
+        r = 5.0 + 2.0 * b
+        obj = rng.uniform(0.2, 1.0, (n, n)) * (((yy - n / 2) ** 2 + (xx - n / 2) ** 2) <= r * r)
+        A[b] = np.abs(cfft2(obj))
+    sup = np.zeros((B, n, n), np.uint8)
+    sup[:, n // 4:3 * n // 4, n // 4:3 * n // 4] = 1
+    x0 = cifft2(A * np.exp(2j * np.pi * rng.random((B, n, n))))
+    return x0, A, sup
+
+
+@pytest.mark.parametrize("iters,sw_every,sigmas", [
+    # the 200-iteration schedule's shape at a small scale: updates at
+    # iterations 2, 4, 6, 8, sigma linear from 3.0 (first update) to 1.5 (last)
+    (8, 2, [3.0, 2.5, 2.0, 1.5]),
+    # iters not a multiple of sw_every: the trailing iteration has no update
+    (9, 2, [3.0, 2.5, 2.0, 1.5]),
+    # one update: sigma0
+    (3, 3, [3.0]),
+])
+def test_hio_shrinkwrap_schedule_composition(oracle, iters, sw_every, sigmas):
+    """orc_hio's schedule equals hio_step iterations with a shrinkwrap after
+    every sw_every-th one, sigma annealed linearly 3.0 -> 1.5 over the updates
+    (one per 20 iterations in the 200-iteration workload: 3.0 at iteration 20,
+    1.5 at iteration 200).  The sigmas and update iterations are written out
+    here, so a wrong update index, cadence or interpolation fails."""
+    x0, A, sup0 = hio_problem()
+    x, s = x0.copy(), sup0.copy()
+    upd = [sw_every * (k + 1) for k in range(len(sigmas))]
+    for it in range(1, iters + 1):
+        x = oracle.hio_step(x, A, s, 0.9, True, True)
+        if it in upd:
+            s = oracle.shrinkwrap(x, sigmas[upd.index(it)], 0.1)
+    xo, so = oracle.hio(x0, A, sup0.copy(), iters, 0.9, True, True, sw_every, 3.0, 1.5, 0.1)
+    assert np.array_equal(so, s)
+    assert np.abs(xo - x).max() == 0.0
+    # the check is sensitive: the same loop with the first update one index
+    # off in sigma (or one iteration late) ends elsewhere
+    if len(sigmas) > 1:
+        x2, s2 = x0.copy(), sup0.copy()
+        for it in range(1, iters + 1):
+            x2 = oracle.hio_step(x2, A, s2, 0.9, True, True)
+            if it in upd:
+                k = upd.index(it)
+                s2 = oracle.shrinkwrap(x2, sigmas[min(k + 1, len(sigmas) - 1)], 0.1)
+        assert not np.array_equal(s2, so) or np.abs(x2 - xo).max() > 1e-9
+    x3, s3 = x0.copy(), sup0.copy()
+    for it in range(1, iters + 1):
+        x3 = oracle.hio_step(x3, A, s3, 0.9, True, True)
+        if it - 1 in upd:
+            s3 = oracle.shrinkwrap(x3, sigmas[upd.index(it - 1)], 0.1)
+    assert not np.array_equal(s3, so) or np.abs(x3 - xo).max() > 1e-9
+
+
+def test_hio_schedule_sigmas_of_the_200_iteration_workload(oracle):
+    """The 200-iteration workload (shrinkwrap every 20, sigma 3.0 -> 1.5):
+    ten updates at sigma = linspace(3.0, 1.5, 10).  Checked on 20 iterations
+    with sw_every = 2 (ten updates as well), against the composition with
+    numpy's linspace."""
+    x0, A, sup0 = hio_problem(n=24, B=1, seed=31)
+    sig = np.linspace(3.0, 1.5, 10)
+    x, s = x0.copy(), sup0.copy()
+    for it in range(1, 21):
+        x = oracle.hio_step(x, A, s, 0.9, True, True)
+        if it % 2 == 0:
+            s = oracle.shrinkwrap(x, float(sig[it // 2 - 1]), 0.1)
+    xo, so = oracle.hio(x0, A, sup0.copy(), 20, 0.9, True, True, 2, 3.0, 1.5, 0.1)
+    assert np.array_equal(so, s)
+    assert np.abs(xo - x).max() == 0.0
+
+
+def exact_problem_complex(M=16, N=16, seed=0, r=4.0):
+    """A complex object inside a disc support, with its exact amplitudes."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:M, 0:N]
+    sup = ((yy - M / 2) ** 2 + (xx - N / 2) ** 2 <= r * r).astype(np.uint8)
+    obj = (rng.uniform(-1.0, 1.0, (M, N)) + 1j * rng.uniform(-1.0, 1.0, (M, N))) * sup
+    return obj, np.abs(cfft2(obj)), sup
+
+
+@pytest.mark.parametrize("beta", [0.8, 0.7, -0.6])
+def test_dm_textbook_support_only_closed_forms(oracle, beta):
+    """space_mode 2 (P_S: zero outside the support, complex values kept).
+    With the support everywhere P_S is the identity and one DM step is P_M psi
+    for any beta; with an empty support P_S = 0 and the step is
+    psi - beta P_M psi (P_M(psi / beta) = sign(beta) P_M psi when A > 0)."""
+    _, A, _ = exact_problem(seed=40)
+    A = A + 0.1  # no zero amplitudes: P_M is scale invariant for positive scales
+    psi = rnd((1, 16, 16), 41)
+    ones = np.ones((1, 16, 16), np.uint8)
+    p1 = oracle.dm_textbook(psi, A[None], beta, 1, mask=ones, space_mode=2)
+    assert np.abs(p1[0] - pm(psi[0], A)).max() < 1e-10
+    zeros = np.zeros((1, 16, 16), np.uint8)
+    p0 = oracle.dm_textbook(psi, A[None], beta, 1, mask=zeros, space_mode=2)
+    expect = psi[0] - beta * np.sign(beta) * pm(np.sign(beta) * psi[0], A) * np.sign(beta)
+    assert np.abs(p0[0] - expect).max() < 1e-10
+
+
+def test_dm_textbook_fixed_points_tell_the_modes_apart(oracle):
+    """An exact complex solution inside the support is a fixed point of
+    space_mode 2 only; an exact real solution with negative values is a fixed
+    point of modes 0 (MKL SpaceConstraint with S = support) and 2, not of 1
+    (positivity)."""
+    objc, Ac, supc = exact_problem_complex(seed=42)
+    p2 = oracle.dm_textbook(objc[None], Ac[None], 0.8, 3, mask=supc[None], space_mode=2)
+    assert np.abs(p2[0] - objc).max() < 1e-10
+    p1 = oracle.dm_textbook(objc[None], Ac[None], 0.8, 3, mask=supc[None], space_mode=1)
+    assert np.abs(p1[0] - objc).max() > 1e-3
+    p0 = oracle.dm_textbook(objc[None], Ac[None], 0.8, 3, S=supc[None].astype(float), space_mode=0)
+    assert np.abs(p0[0] - objc).max() > 1e-3
+    objr = objc.real
+    Ar = np.abs(cfft2(objr))
+    for mode, kw in ((0, dict(S=supc[None].astype(float))), (2, dict(mask=supc[None]))):
+        p = oracle.dm_textbook(objr[None].astype(complex), Ar[None], 0.8, 3, space_mode=mode, **kw)
+        assert np.abs(p[0] - objr).max() < 1e-10, mode
+    p = oracle.dm_textbook(objr[None].astype(complex), Ar[None], 0.8, 3, mask=supc[None],
+                           space_mode=1)
+    assert np.abs(p[0] - objr).max() > 1e-3
+
+
+def test_dm_textbook_mode0_empty_constraint(oracle):
+    """space_mode 0 with S < 0.9 everywhere: SpaceConstraint zeroes every
+    pixel, so one step is psi - beta P_M psi."""
+    _, A, _ = exact_problem(seed=43)
+    A = A + 0.1
+    psi = rnd((1, 16, 16), 44)
+    S = np.full((1, 16, 16), 0.5)
+    p = oracle.dm_textbook(psi, A[None], 0.8, 1, S=S, space_mode=0)
+    assert np.abs(p[0] - (psi[0] - 0.8 * pm(psi[0], A))).max() < 1e-10
+
+
+def test_carousel_driver_fixed_point(oracle):
+    """Every projection at the same exact solution: the warm start averages
+    equal neighbours, the DM iterations and the final projection keep it, and
+    the convergence error is zero (src/CPRA_MKL.cpp:58-140)."""
+    obj, A, sup = exact_problem(seed=50)
+    P, nb = 4, 2
+    AP = np.broadcast_to(A, (P, 16, 16)).copy()
+    SP = np.broadcast_to(sup.astype(float), (P, 16, 16)).copy()
+    init = np.broadcast_to(obj.astype(complex), (P, nb, 16, 16)).copy()
+    psi, err = oracle.carousel_driver(AP, SP, init, 2, 3, 5, 0.9)
+    assert np.abs(psi - obj).max() < 1e-10
+    assert np.abs(err).max() < 1e-10
+
+
+@pytest.mark.parametrize("preit", [0, 3])
+def test_carousel_driver_sweep_by_hand(oracle, preit):
+    """One sweep over P = 3 projections started at c_p * obj (obj an exact
+    real solution, c_p > 0), worked by hand from src/CPRA_MKL.cpp:35-140:
+      - Step A (`preit` DM iterations on projection 0): one iteration maps
+        c * obj to obj and later ones keep it (P_M(c obj) = obj for c > 0),
+        so c_0 becomes 1;
+      - projections are finished one at a time, p = 0, 1, 2: warm start
+        (:63-76), then the DM iterations, so the warm start of p reads p - 1
+        as it left its own iterations in this sweep:
+          p = 0:     (c_1 + c_0) / 2
+          0 < p < 2: (c_{p-1}'' + c_{p+1}) / 2   (the own value has weight 0)
+          p = 2:     (c_1'' + c_2) / 2
+        with c'' = 1 after at least one DM iteration, c'' = c' with none;
+      - with iters = 1 the convergence error of the last iteration is
+        ||F(c' obj) - F obj|| / ||F obj|| = |c' - 1|, and psi ends at obj;
+      - with iters = 0 nothing runs but the warm starts, and the final
+        projection P_S((1 + 1/beta) P_M psi - psi/beta) gives
+        (1 + 1/beta - c'/beta) obj.
+    Each projection has two batch members with their own c."""
+    obj, A, sup = exact_problem(seed=51)
+    beta = 0.9
+    c = np.array([[0.5, 0.6], [1.0, 1.2], [1.5, 1.3]])   # [P][member]
+    P, nb = c.shape
+    AP = np.broadcast_to(A, (P, 16, 16)).copy()
+    SP = np.broadcast_to(sup.astype(float), (P, 16, 16)).copy()
+    init = c[:, :, None, None] * obj[None, None].astype(complex)
+    c0 = np.ones(nb) if preit > 0 else c[0]
+    # iters = 1
+    one = np.ones(nb)
+    cp = np.stack([(c[1] + c0) / 2, (one + c[2]) / 2, (one + c[2]) / 2])
+    psi, err = oracle.carousel_driver(AP, SP, init, 1, 1, preit, beta)
+    assert np.abs(psi - obj).max() < 1e-10
+    assert np.allclose(err, np.abs(cp - 1.0), atol=1e-10)
+    # a Jacobi-style sweep (every warm start from the values before the
+    # sweep) gives other errors for p = 1 and, for member 1, p = 2
+    jac = np.stack([(c[1] + c0) / 2, (c0 + c[2]) / 2, (c[1] + c[2]) / 2])
+    assert np.abs(np.abs(jac - 1.0) - np.abs(cp - 1.0)).max() > 0.05
+    # iters = 0: warm starts only
+    c0p = (c[1] + c0) / 2
+    c1p = (c0p + c[2]) / 2
+    c2p = (c1p + c[2]) / 2
+    psi0, err0 = oracle.carousel_driver(AP, SP, init, 1, 0, preit, beta)
+    coef = 1 + 1 / beta - np.stack([c0p, c1p, c2p]) / beta
+    assert np.allclose(psi0, coef[:, :, None, None] * obj, atol=1e-10)
+    assert np.all(err0 == 0)
+
+
+def test_carousel3d_composition(oracle):
+    """oracle.carousel3d against one outer iteration written out with numpy's
+    n-dimensional FFT: x <- HIO steps; V = merge of Forward2D(x);
+    w = Backward3D(V); v = max(Re w, 0) inside support3d, 0 outside;
+    x = Backward2D(slices of Forward3D(v)).
+    Angles (0, 0): both slices are the grid plane y = n/2 (sin 0 = 0,
+    cos 0 = 1 exactly), so the merge is their mean on that plane and 0
+    elsewhere, all by hand.  Angles (0, pi/2): the merge and extraction come
+    from orc_slice_merge / orc_slice_extract, pinned above.  A random complex
+    spectrum makes Re w take both signs and w complex, so using |w|, keeping
+    Im w, dropping positivity or the support each fail."""
+    n = 8
+    c = n // 2
+    rng = np.random.default_rng(60)
+    Vt = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    F = np.stack([Vt[:, c, :], Vt[:, c + 1, :]])
+    x = cifft2(F)
+    s3 = (rng.random((n, n, n)) < 0.6).astype(np.uint8)
+    A = np.abs(F) + 0.1
+    s2 = np.ones((2, n, n), np.uint8)
+    for ang in (np.array([0.0, 0.0]), np.array([0.0, np.pi / 2])):
+        for inner in (0, 1):
+            xin = x if inner == 0 else oracle.hio_step(x, A, s2, 0.9, True, True)
+            Fin = cfft2(xin)
+            if ang[1] == 0.0:
+                V = np.zeros((n, n, n), complex)
+                V[:, c, :] = 0.5 * (Fin[0] + Fin[1])
+            else:
+                V = oracle.slice_merge(Fin, ang)
+            w = cifft3(V)
+            v = np.where((s3 > 0) & (w.real > 0), w.real, 0.0)
+            Fv = cfft3(v)
+            if ang[1] == 0.0:
+                xe = cifft2(np.stack([Fv[:, c, :], Fv[:, c, :]]))
+            else:
+                xe = cifft2(oracle.slice_extract(Fv, ang))
+            xo, vo = oracle.carousel3d(x, A, s2, ang, s3, 1, inner, 0.9, True)
+            assert np.abs(vo - v).max() < 1e-12 * np.abs(v).max()
+            assert np.abs(xo - xe).max() < 1e-12 * np.abs(xe).max()
+            assert (w.real < 0).any() and (np.abs(w.imag) > 1e-3 * np.abs(w).max()).any()
+            assert ((s3 == 0) & (w.real > 0)).any()
