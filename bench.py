@@ -5,16 +5,20 @@ each from its own seeded disc object (radii U[30, 50] px, Poisson noise at
 1e6 photons), HIO beta = 0.9 with real-positive support, shrinkwrap every 20
 iterations (sigma 3.0 -> 1.5 px, threshold 0.1 x max), the iteration body
 captured in a hipGraph.  One step = one HIO iteration of the whole batch
-(512 pattern-iterations), shrinkwrap updates included.  Inputs are resident
-in HBM before the timed region.
+(512 pattern-iterations), shrinkwrap updates included; the K timed steps
+are iterations 1..K of the schedule from the seeded start, replaying the
+chunk graph that the warmup captured.  Inputs are resident in HBM before
+the timed region.
 
 Prints one JSON line (rank 0) with:
   value      batched HIO iterations per second over the K timed steps;
   roofline   the dominant fused kernel: algorithmic bytes per launch / its
              average duration (HIP events on the stream it runs on, from a
              separate eager run of the same workload), against 8 TB/s;
-  cpu_baseline  the FP64 CPU oracle timed on a bounded sample of the same
-             workload on this host's cores;
+  cpu_baseline  the reference's CPU path (its MKL backend restated in FP32
+             over libmkl_rt, one pass per primitive) on 32 of the patterns,
+             the same schedule, on this host's cores and on 8 threads;
+  cpu_baseline_fp64_oracle  the FP64 CPU oracle on 16 of the patterns;
   fp64_check two of the patterns against the FP64 oracle: one iteration
              from the same state (relative L2), and the R-factor after the
              full 200-iteration schedule next to the oracle's own spread
@@ -31,6 +35,14 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# ROCm's graph packet capture (on by default) replays kernel nodes with a
+# per-node cost that eager launches do not have: the 180 x 128^2 carousel's
+# graph ran 4 % below its eager launches with it, level with them without it,
+# and the 512 x 256^2 HIO graph runs at the eager rate either way.  The
+# runtime reads this once, when HIP initialises, so it is set before the
+# library loads; a caller's own setting wins.
+os.environ.setdefault("DEBUG_CLR_GRAPH_PACKET_CAPTURE", "0")
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "phase-retrieval iters/s (512×256² batch, 128³ carousel); % HBM peak"
@@ -63,34 +75,114 @@ def measured_traffic(kernel: str, tuning: dict):
     return None
 
 
-def cpu_baseline_oracle(data, n_patterns, min_seconds=10.0):
-    """Times the FP64 oracle's HIO (same schedule) on a sample of patterns."""
+def cpu_baseline_oracle(data, n_patterns=16):
+    """The FP64 oracle on the same 200-iteration HIO + shrinkwrap schedule,
+    on `n_patterns` of the patterns, scaled to 512-pattern batched
+    iterations."""
     import numpy as np
 
     import __graft_entry__ as ge
 
     o = ge.load_oracle()
+    from cpra_amd import synth
+
+    sched = dict(synth.HIO_SCHEDULE)
+    iters = sched.pop("iters")
     x = data["x0"][:n_patterns].astype(np.complex128)
     A = data["amp"][:n_patterns].astype(np.float64)
     s = data["support"][:n_patterns]
-    iters = 2
-    while True:
-        t0 = time.perf_counter()
-        o.hio(x, A, s, iters, beta=0.9, real_positive=True, dc_skip_zero=True, sw_every=0)
-        dt = time.perf_counter() - t0
-        if dt >= min_seconds or iters >= 4096:
-            break
-        iters = max(iters * 2, int(iters * min_seconds / max(dt, 1e-3) * 1.1))
+    t0 = time.perf_counter()
+    o.hio(x, A, s, iters, **sched)
+    dt = time.perf_counter() - t0
     pit_per_s = n_patterns * iters / dt
     return {
         "value": pit_per_s / 512.0,
         "unit": "iters/s",
         "cores": min(o.num_threads(), n_patterns),
         "kind": "port",
-        "sample": f"FP64 oracle HIO, {n_patterns} of the 512 patterns x {iters} iterations "
-                  f"({dt:.1f} s), scaled to 512-pattern batched iterations",
+        "sample": f"FP64 oracle, the 200-iteration HIO + shrinkwrap schedule on {n_patterns} of the "
+                  f"512 patterns ({dt:.1f} s), scaled to 512-pattern batched iterations",
         "pattern_iters_per_s": pit_per_s,
     }
+
+
+_MKL_CHILD = r"""
+import json, os, sys
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, os.path.join(sys.argv[1], "oracle"))
+import __graft_entry__ as ge
+ge.load_package()
+from cpra_amd import synth
+import mkl_path as mp
+n_patterns, threads = int(sys.argv[2]), int(sys.argv[3])
+if not mp.available():
+    print(json.dumps({"error": mp.load_error()}))
+    sys.exit(0)
+data = synth.make_2d_batch(n_patterns, 256, 30.0, 50.0, 1e6, starts_per_pattern=1, config=1)
+sched = dict(synth.HIO_SCHEDULE)
+med, rates, runs = mp.time_hio_schedule(data["x0"], data["amp"], data["support"], threads, sched)
+print(json.dumps({"pattern_iters_per_s": med, "reps": rates, "schedules_per_rep": runs,
+                  "library": mp.library_path(), "cpu_model": mp.cpu_model(),
+                  "physical_cores": mp.physical_cores()}))
+"""
+
+
+def cpu_baseline_mkl_path(n_patterns=32):
+    """The reference's CPU path: its MKL backend restated in FP32 over
+    libmkl_rt (oracle/mkl_path.c: MKL DFTI with the reference's descriptor,
+    its shift and constraint loops under OpenMP), running the same
+    200-iteration HIO + shrinkwrap schedule one pass per primitive, on the
+    first `n_patterns` of the 512 patterns (the same seeded data).  Protocol:
+    2 untimed warm-up iterations, then 3 repetitions of at least 10 s and 20
+    iterations each, median; on all of this host's cores available to the
+    run (OMP_NUM_THREADS when set) and on 8 threads, the reference
+    benchmark's setting.  Runs in a child process that never touches the
+    GPU.  Returns None when libmkl_rt cannot be loaded."""
+    import subprocess
+
+    try:
+        import mkl_path as mp  # noqa: F401
+    except ImportError:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import mkl_path as mp
+    cores = mp.physical_cores()
+    env_threads = os.environ.get("OMP_NUM_THREADS", "")
+    all_threads = min(cores, int(env_threads)) if env_threads.isdigit() else cores
+    out = {}
+    for threads in sorted({all_threads, 8}, reverse=True):
+        r = subprocess.run([sys.executable, "-c", _MKL_CHILD, ROOT, str(n_patterns), str(threads)],
+                           capture_output=True, text=True, timeout=900,
+                           env={**os.environ, "HIP_VISIBLE_DEVICES": "", "CUDA_VISIBLE_DEVICES": "",
+                                "OMP_NUM_THREADS": str(threads)})
+        if r.returncode != 0:
+            return {"error": r.stderr[-400:]}
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        if "error" in rec:
+            return None
+        out[threads] = rec
+    # the faster of the two thread counts is the baseline (on a shared host
+    # the run on every core can be the slower one)
+    best = max(out, key=lambda t: out[t]["pattern_iters_per_s"])
+    top = out[best]
+    res = {
+        "value": top["pattern_iters_per_s"] / 512.0,
+        "unit": "iters/s",
+        "cores": best,
+        "kind": "port",
+        "sample": f"the reference's MKL backend in FP32 over {top['library']} (DFTI + OpenMP, one "
+                  f"pass per primitive), the 200-iteration HIO + shrinkwrap schedule on {n_patterns} "
+                  f"of the 512 patterns, median of 3 runs of >= 10 s after 2 warm-up iterations, "
+                  f"{best} threads (the faster of {sorted(out)}); scaled to 512-pattern batched "
+                  "iterations",
+        "cpu_model": top["cpu_model"],
+        "physical_cores_visible": top["physical_cores"],
+        "pattern_iters_per_s": top["pattern_iters_per_s"],
+        "reps_pattern_iters_per_s": top["reps"],
+    }
+    res["by_threads"] = {str(t): {"value": r["pattern_iters_per_s"] / 512.0, "unit": "iters/s",
+                                  "pattern_iters_per_s": r["pattern_iters_per_s"],
+                                  "reps_pattern_iters_per_s": r["reps"]} for t, r in out.items()}
+    return res
 
 
 _MKL_SNIPPET = r"""
@@ -109,10 +201,11 @@ s = torch.from_numpy(data["support"].astype(bool))
 """
 
 
-def cpu_baseline_mkl(n_patterns=16, iters=4):
-    """HIO on the host with torch's CPU FFT (MKL-backed), FP32, for reference.
-    Runs in a child process so that torch's bundled HIP runtime never shares
-    a process with the benchmark's."""
+def cpu_baseline_torch(n_patterns=16, iters=4):
+    """Fallback where libmkl_rt cannot be loaded: HIO on the host with torch's
+    CPU FFT (MKL-backed), FP32, without shrinkwrap.  Runs in a child process
+    so that torch's bundled HIP runtime never shares a process with the
+    benchmark's."""
     import subprocess
 
     code = _MKL_SNIPPET + _MKL_BODY
@@ -144,8 +237,9 @@ print(json.dumps({
     "value": n_patterns * iters / dt / 512.0,
     "unit": "iters/s",
     "cores": torch.get_num_threads(),
-    "kind": "torch-cpu-mkl-fp32",
-    "sample": f"{n_patterns} patterns x {iters} HIO iterations (no shrinkwrap)",
+    "kind": "port",
+    "sample": f"fallback: torch's CPU FFT (MKL-backed), FP32, {n_patterns} patterns x {iters} HIO "
+              "iterations without shrinkwrap",
 }))
 """
 
@@ -701,18 +795,39 @@ def main():
         tune = "" if args.no_autotune else h.autotune(x, amp, sup, iters=10)
     log(f"bench: autotune {tune}")
 
-    # warmup (also captures the 20-iteration graph)
-    h.hio_run(x, amp, sup, iters=args.warmup, use_graph=1, **sched)
+    # warmup: at least one whole shrinkwrap chunk from the seeded start, so
+    # that the chunk's graph (sw_every iterations) is captured here and the
+    # timed run only replays it
+    warm = max(args.warmup, sched["sw_every"])
+    h.hio_run(x, amp, sup, iters=warm, use_graph=1, **sched)
     h.Sync()
-    log("bench: warmup done")
+    captures0 = int(h.get_tuning()["graph_captures"])
+    log(f"bench: warmup done ({warm} iterations, {captures0} graph captures so far)")
 
-    h.Sync()
-    t0 = time.perf_counter()
-    h.hio_run(x, amp, sup, iters=args.steps, use_graph=1, **sched)
-    h.Sync()
-    elapsed = time.perf_counter() - t0
+    def timed_run(k):
+        # the timed steps are iterations 1..k of the schedule: the state goes
+        # back to the seeded start first (outside the timed region), since
+        # the work per iteration follows the support, which the shrinkwrap
+        # updates change (the first update grows it to every row)
+        x.upload(data["x0"])
+        sup.upload(data["support"])
+        h.Sync()
+        t0 = time.perf_counter()
+        h.hio_run(x, amp, sup, iters=k, use_graph=1, **sched)
+        h.Sync()
+        return time.perf_counter() - t0
+
+    elapsed = timed_run(args.steps)
     its = args.steps / elapsed
-    log(f"bench: {args.steps} iterations in {elapsed:.3f} s -> {its:.1f} it/s")
+    captures1 = int(h.get_tuning()["graph_captures"])
+    log(f"bench: {args.steps} iterations in {elapsed:.3f} s -> {its:.1f} it/s "
+        f"({captures1 - captures0} graph captures in the timed run)")
+    full_sched = None
+    if args.steps != 200:
+        # the whole 200-iteration schedule, for reference beside the value
+        el200 = timed_run(200)
+        full_sched = round(200 / el200, 2)
+        log(f"bench: the 200-iteration schedule -> {full_sched} it/s")
 
     # per-kernel durations with HIP events (eager launches, same workload),
     # from the initial state, so that the support (and with it the row mode
@@ -777,7 +892,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched 2D shrinkwrap-HIO, {P} x {n}x{n} complex64, "
                                f"beta 0.9, shrinkwrap every 20 (sigma 3.0->1.5, thr 0.1)",
-                   "patterns": P, "size": n, "graph": True},
+                   "patterns": P, "size": n, "graph": True,
+                   "graph_captures_in_timed_run": captures1 - captures0,
+                   "warmup_iterations_run": warm,
+                   "timed_iterations": f"1..{args.steps} of the schedule, from the seeded start"},
+        "schedule_200_iters_per_s": full_sched if full_sched is not None else round(its, 2),
         "pattern_iters_per_s": round(its * P, 1),
         "hbm_frac_algorithmic_21Bpx": round(iter_algo_frac, 4),
         "roofline": roof,
@@ -834,12 +953,16 @@ def main():
             result["workloads"][name] = fn(hip, synth, ge, fp64=fp64)
 
     if not args.no_cpu:
-        log("bench: CPU baseline (oracle)")
-        result["cpu_baseline"] = cpu_baseline_oracle(data, 16)
-        try:
-            result["cpu_baseline_mkl"] = cpu_baseline_mkl()
-        except Exception as e:  # noqa: BLE001
-            result["cpu_baseline_mkl"] = {"error": str(e)}
+        log("bench: CPU baseline (the reference's MKL path)")
+        mkl = cpu_baseline_mkl_path()
+        if mkl is None or "error" in mkl:
+            log(f"bench: MKL path unavailable ({mkl}); torch CPU FFT instead")
+            result["cpu_baseline"] = {**cpu_baseline_torch(), "mkl_path_error": mkl}
+        else:
+            result["cpu_baseline"] = mkl
+        log(f"bench: cpu baseline {result['cpu_baseline']}")
+        log("bench: CPU baseline (FP64 oracle)")
+        result["cpu_baseline_fp64_oracle"] = cpu_baseline_oracle(data, 16)
     print(json.dumps(result), flush=True)
 
 

@@ -468,13 +468,14 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         }
         const int rest = p->outer_iters - done;
         if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest >= 2) {
-            hipStream_t gst = nullptr;
-            CPRA_CHECK_HIP(hipStreamCreateWithFlags(&gst, hipStreamNonBlocking));
-            hipEvent_t ev0, ev1;
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            hipStream_t gst = graph_stream(h);
+            hipEvent_t ev0 = nullptr, ev1 = nullptr;
+            if (gst != st) {
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            }
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             h->stream = gst;
@@ -488,15 +489,19 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             }
             h->stream = st;
             CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            ++h->graph_captures;
             for (int i = 0; i < rest; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
-            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            if (ev0) {
+                CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            }
             CPRA_CHECK_HIP(hipStreamSynchronize(gst));
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
-            (void)hipEventDestroy(ev0);
-            (void)hipEventDestroy(ev1);
-            (void)hipStreamDestroy(gst);
+            if (ev0) {
+                (void)hipEventDestroy(ev0);
+                (void)hipEventDestroy(ev1);
+            }
         } else {
             for (int i = 0; i < rest; ++i) cr.outer(ktp, true);
         }

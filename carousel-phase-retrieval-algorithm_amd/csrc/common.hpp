@@ -104,6 +104,9 @@ struct cpra_hip_ctx {
     std::map<uint64_t, std::pair<hipGraph_t, hipGraphExec_t>> user_graphs;
     // per-kernel event timing of the engines (eager launches only)
     bool timing = false;
+    // iteration graphs the engines have captured on this handle (reported by
+    // cpra_hip_get_tuning, so that a caller can tell a replay from a capture)
+    int64_t graph_captures = 0;
     double kt_ms[16] = {};
     int64_t kt_n[16] = {};
 };
@@ -122,6 +125,13 @@ inline unsigned grid_1d(uint64_t work, unsigned block, unsigned cap = 1u << 20)
     if (g > cap) g = cap;
     return (unsigned)g;
 }
+
+// Stream on which an engine captures and replays its iteration graph: the
+// handle's own stream when it has one, so that the replays queue behind the
+// run's other work with no cross-stream events (replays on a second stream
+// measured 2 % slower on the carousel); a non-blocking stream of the handle
+// when the handle runs on the null stream, which cannot be captured.
+hipStream_t graph_stream(cpra_hip_ctx* h);
 
 // CPRA_HIP_DEBUG=1 in the environment: name every launch on stderr and
 // synchronise after it, so a failing kernel is identified by name.

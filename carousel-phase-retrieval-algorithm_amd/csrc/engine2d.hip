@@ -794,8 +794,9 @@ static void dm_rows_launch(hipStream_t st, const Geo& g, const cpra_hip_dm_param
     check_launch("k_dm_rows");
 }
 
-static hipStream_t aux_stream(cpra_hip_ctx* h)
+hipStream_t graph_stream(cpra_hip_ctx* h)
 {
+    if (h->stream) return h->stream;
     if (!h->aux_stream) CPRA_CHECK_HIP(hipStreamCreateWithFlags(&h->aux_stream, hipStreamNonBlocking));
     return h->aux_stream;
 }
@@ -891,11 +892,13 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         hipStream_t gst = st;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (use_graph) {
-            gst = aux_stream(h);
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            gst = graph_stream(h);
+            if (gst != st) {
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            }
         }
         // the chunk body as a graph, one per row mode, captured on first use
         // and kept while the key matches
@@ -916,6 +919,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                     hio_iteration(h, gst, Y, A, (int64_t)amp_batch, support, p, sp, nullptr, tl);
                 CPRA_CHECK_HIP(hipStreamEndCapture(gst, &eg->graph));
                 CPRA_CHECK_HIP(hipGraphInstantiate(&eg->exec, eg->graph, nullptr, nullptr, 0));
+                ++h->graph_captures;
                 eg->er = p->error_reduction;
                 eg->col = colv;
                 eg->tiled = tl;
@@ -961,7 +965,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 ++k;
             }
         }
-        if (use_graph) {
+        if (ev0) {
             CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
             CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
             CPRA_CHECK_HIP(hipEventDestroy(ev0));
@@ -1051,12 +1055,14 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
     };
     const int body = p->iters - 1;
     if (p->use_graph && !h->timing && !h->capturing && body >= 2) {
-        hipStream_t gst = aux_stream(h);
-        hipEvent_t ev0, ev1;
-        CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-        CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-        CPRA_CHECK_HIP(hipEventRecord(ev0, st));
-        CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+        hipStream_t gst = graph_stream(h);
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (gst != st) {
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+        }
         hipGraph_t graph;
         hipGraphExec_t exec;
         // one iteration captured, replayed `body` times
@@ -1064,14 +1070,19 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
         one(gst, false);
         CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
         CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        ++h->graph_captures;
         for (int i = 0; i < body; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
-        CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
-        CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+        if (ev0) {
+            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+        }
         CPRA_CHECK_HIP(hipStreamSynchronize(gst));
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
+        if (ev0) {
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+        }
     } else {
         for (int i = 0; i < body; ++i) one(st, false);
     }
@@ -1119,18 +1130,23 @@ int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const floa
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         hipStream_t gst = nullptr;
         if (p->use_graph && !h->timing && !h->capturing && body >= 2) {
-            gst = aux_stream(h);
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            gst = graph_stream(h);
+            if (gst != st) {
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            }
             CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < gk; ++i) one(gst, false);
             CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
             CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            ++h->graph_captures;
             for (; done + gk <= body; done += gk) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
-            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            if (ev0) {
+                CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            }
         }
         for (; done < body; ++done) one(st, false);
         one(st, true);
@@ -1140,8 +1156,10 @@ int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const floa
             CPRA_CHECK_HIP(hipStreamSynchronize(gst));
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
-            (void)hipEventDestroy(ev0);
-            (void)hipEventDestroy(ev1);
+            if (ev0) {
+                (void)hipEventDestroy(ev0);
+                (void)hipEventDestroy(ev1);
+            }
         }
         return CPRA_HIP_OK;
     });
@@ -1429,10 +1447,11 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
         std::snprintf(out, out_len,
-                      "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;rows_sparse_used=%d;",
+                      "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;rows_sparse_used=%d;"
+                      "graph_captures=%lld;",
                       (int)h->m, col_variant_code(cv), sparse_mode(h), (long long)iteration_chunk(h),
                       tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, engine_merge_lines(h) ? 1 : 0,
-                      rows_sparse_used(h));
+                      rows_sparse_used(h), (long long)h->graph_captures);
         return CPRA_HIP_OK;
     });
 }

@@ -328,28 +328,33 @@ int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes,
         const int body = p->iters - 1;
         KernelTimer kt;
         if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && body >= 2) {
-            hipStream_t gst = nullptr;
-            CPRA_CHECK_HIP(hipStreamCreateWithFlags(&gst, hipStreamNonBlocking));
-            hipEvent_t ev0, ev1;
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            hipStream_t gst = graph_stream(h);
+            hipEvent_t ev0 = nullptr, ev1 = nullptr;
+            if (gst != st) {
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+                CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+            }
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
             r.iteration(gst, false, nullptr);
             CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
             CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            ++h->graph_captures;
             for (int i = 0; i < body; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
-            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
-            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            if (ev0) {
+                CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+                CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            }
             CPRA_CHECK_HIP(hipStreamSynchronize(gst));
             (void)hipGraphExecDestroy(exec);
             (void)hipGraphDestroy(graph);
-            (void)hipEventDestroy(ev0);
-            (void)hipEventDestroy(ev1);
-            (void)hipStreamDestroy(gst);
+            if (ev0) {
+                (void)hipEventDestroy(ev0);
+                (void)hipEventDestroy(ev1);
+            }
         } else {
             for (int i = 0; i < body; ++i) r.iteration(st, false, h->timing ? &kt : nullptr);
         }

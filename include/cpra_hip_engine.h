@@ -124,9 +124,12 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * transpose before the merge). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
- * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;rows_sparse_used=<0|1>;" (the last: the row mode
- * of the last iterations run; in the adaptive mode it is read from the
- * device, which synchronises the handle's stream). */
+ * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;rows_sparse_used=<0|1>;
+ * graph_captures=<k>;" (rows_sparse_used: the row mode of the last iterations
+ * run; in the adaptive mode it is read from the device, which synchronises
+ * the handle's stream; graph_captures: how many iteration graphs the engines
+ * have captured on this handle, so that a caller can check that a run
+ * replayed a graph it had already captured). */
 int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len);
 
 /* Per-kernel timing of the engines: when enabled, engine runs launch
