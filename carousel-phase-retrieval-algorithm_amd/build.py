@@ -52,7 +52,8 @@ def _compile(src: str) -> str:
 
 
 ROOT = os.path.dirname(HERE)
-CPP_TESTS = {"test_hipimpl": os.path.join(ROOT, "tests", "cpp", "test_hipimpl.cpp")}
+CPP_TESTS = {"test_hipimpl": os.path.join(ROOT, "tests", "cpp", "test_hipimpl.cpp"),
+             "test_managed_facade": os.path.join(ROOT, "tests", "cpp", "test_managed_facade.cpp")}
 CPP_BIN = os.path.join(ROOT, "tests", "cpp", "bin")
 
 

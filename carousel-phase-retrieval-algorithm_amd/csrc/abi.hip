@@ -765,7 +765,7 @@ int cpra_hip_converge_error(cpra_hip_handle h, void* old_, const void* new_, voi
             hipLaunchKernelGGL(k_conv_err_fin<double>, dim3(grid_1d(batch, 64)), dim3(64), 0, h->stream, part, nblk, batch, (double*)dev_err);
         }
         check_launch("k_conv_err");
-        CPRA_CHECK_HIP(hipMemcpyAsync(host_err, dev_err, es * batch, hipMemcpyDeviceToHost, h->stream));
+        CPRA_CHECK_HIP(hipMemcpyAsync(host_err, dev_err, es * batch, hipMemcpyDefault, h->stream));
         CPRA_CHECK_HIP(hipStreamSynchronize(h->stream));
         return CPRA_HIP_OK;
     });

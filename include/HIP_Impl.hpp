@@ -6,8 +6,11 @@
 // compiles with any C++17 host compiler.  Link with -lcpra_hip.
 //
 // Inside the reference tree define CPRA_HIP_IN_REFERENCE_TREE so the
-// reference's include/CPRA_Impl.hpp and include/Allocator.hpp provide
-// CpraImpl<T>, AllocatorInterface and IMPL_TYPE (with HIP added to the enum).
+// reference's include/CPRA_Impl.hpp provides CpraImpl<T> and IMPL_TYPE (with
+// HIP added to the enum).  NewHIPAllocator is declared here and defined in
+// HIP_Allocator.hpp, which needs the reference's AllocatorInterface: the
+// reference's include/Allocator.hpp (no include guard, so this header does
+// not include it) includes HIP_Allocator.hpp at its end.
 //
 // Semantics follow the MKL backend by default (the parity target):
 // SpaceConstraint zeroes where (double)S < 0.9 and keeps negative real parts;
@@ -18,27 +21,27 @@
 // last_error()); HIP runtime errors throw CPRA::hip_error, like the CUDA
 // backend's CPRA_CUDA_TRY (include/CUDA_Error_Check.hpp:69-82).
 #pragma once
-#ifdef CPRA_HIP_IN_REFERENCE_TREE
-#include "Allocator.hpp"
-#include "CPRA_Impl.hpp"
-#else
-#include "cpra_impl_interface.hpp"
-#endif
 #include <complex>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#ifdef CPRA_HIP_IN_REFERENCE_TREE
+#include "CPRA_Impl.hpp"
+#else
+#include "cpra_impl_interface.hpp"
+#endif
 
+#include "HIP_Error.hpp"
 #include "cpra_hip.h"
 #include "cpra_hip_engine.h"
 
 namespace CPRA {
 
-struct hip_error : public std::runtime_error {
-    int code;
-    hip_error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
+class AllocatorInterface;
+// Defined in HIP_Allocator.hpp: device memory by default, hipMallocManaged
+// memory with gpu_managed = true (the reference's NewAllocator default).
+inline std::unique_ptr<AllocatorInterface> NewHIPAllocator(bool gpu_managed = false);
 
 template <typename T> class HipImpl final : public CpraImpl<T> {
     static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>,
@@ -145,33 +148,8 @@ std::unique_ptr<CpraImpl<T>> NewHIPImpl(uint64_t m, uint64_t n, uint64_t l, uint
     return std::make_unique<HipImpl<T>>(m, n, l, batch_size);
 }
 
-// Device allocator of the HIP backend (the reference defaults the CUDA path
-// to managed memory, include/Allocator.hpp:73-82; on MI355X device memory
-// plus explicit Memcpy is the fast path, managed memory is opt-in).
-class HipAllocator final : public AllocatorInterface {
-public:
-    explicit HipAllocator(bool managed = false) : managed_(managed) {}
-    void* allocate(uint64_t alloc_bytes, int alignment = 64) const override
-    {
-        (void)alignment;  // HIP allocations are at least 256-byte aligned
-        void* p = nullptr;
-        const int rc = cpra_hip_malloc(nullptr, &p, alloc_bytes, managed_ ? 1 : 0);
-        if (rc < 0) throw hip_error(rc, cpra_hip_last_error(nullptr));
-        return p;
-    }
-    void deallocate(void* p) const override
-    {
-        const int rc = cpra_hip_free(nullptr, p);
-        if (rc < 0) throw hip_error(rc, cpra_hip_last_error(nullptr));
-    }
-
-private:
-    bool managed_;
-};
-
-inline std::unique_ptr<AllocatorInterface> NewHIPAllocator(bool gpu_managed = false)
-{
-    return std::make_unique<HipAllocator>(gpu_managed);
-}
-
 }  // namespace CPRA
+
+#ifndef CPRA_HIP_IN_REFERENCE_TREE
+#include "HIP_Allocator.hpp"
+#endif

@@ -143,7 +143,8 @@ int cpra_hip_normalize(cpra_hip_handle h, void* x, double norm, uint64_t num);
 /* ConvergeError (include/CPRA_Impl.hpp:57; CUDA_Impl.inl:145-172):
  * old <- new - old; host_err[b] = ||old_b|| / ||new_b|| over the m*n*l
  * values of batch member b (the reference's CUDA path sums m*n only).
- * host_err is host memory with `batch` entries of T. Synchronises. */
+ * host_err holds `batch` entries of T in host-visible memory (host or
+ * managed). Synchronises. */
 int cpra_hip_converge_error(cpra_hip_handle h, void* old_, const void* new_, void* host_err,
                             uint64_t m, uint64_t n, uint64_t l, uint64_t batch);
 
