@@ -20,6 +20,7 @@
 // pass (|x| -> row DFT -> row factor of the Gaussian transfer function -> row
 // IDFT, real part), one column pass (the column factor, with the per-pattern
 // maximum), then the threshold -> mask.
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -1090,77 +1091,151 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
     kt.collect(h);
 }
 
+}  // extern "C"
+
+// The difference-map iterations on the handle's batch with the iteration
+// state Y = [Y1; Y2] and Z = [Z1; Z2] (each 2 x the batch, complex64) in the
+// buffers given; graphs only when allow_graph.
+static void dm_run_body(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float* amp,
+                        uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p,
+                        c2<float>* Y, c2<float>* Z, bool allow_graph)
+{
+    const Geo g = geo_of(h);
+    const size_t total = (size_t)g.M * g.N * (size_t)g.B;
+    hipStream_t st = h->stream;
+    dm_entry_launch(st, g, p, static_cast<const c2<float>*>(psi), static_cast<const c2<float>*>(t1),
+                    static_cast<const c2<float>*>(t2), support, Y, Y + total);
+    const Geo g2{g.M, g.N, 2 * g.B};
+    const float scale = (float)(1.0 / ((double)g.M * g.N));
+    auto one = [&](hipStream_t s, bool last) {
+        launch_cols(h, s, g2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr);
+        dm_rows_launch(s, g, p, last, Z, Z + total, static_cast<c2<float>*>(psi), support, Y,
+                       Y + total, static_cast<c2<float>*>(t1), static_cast<c2<float>*>(t2));
+    };
+    const int body = p->iters - 1;  // the last iteration writes t1, t2 in pattern space
+    // graph: a body of at most kDmGraphIters iterations, captured once per
+    // call and replayed (a graph of all iterations cost more to build than
+    // its launches saved: about 10 ms for 1000 iterations at 512^2)
+    constexpr int kDmGraphIters = 16;
+    const int gk = body < kDmGraphIters ? body : kDmGraphIters;
+    int done = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t gst = nullptr;
+    if (allow_graph && p->use_graph && !h->timing && !h->capturing && body >= 2) {
+        gst = graph_stream(h);
+        if (gst != st) {
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+            CPRA_CHECK_HIP(hipEventRecord(ev0, st));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
+        }
+        CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < gk; ++i) one(gst, false);
+        CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
+        CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        ++h->graph_captures;
+        for (; done + gk <= body; done += gk) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+        if (ev0) {
+            CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
+            CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+        }
+    }
+    for (; done < body; ++done) one(st, false);
+    one(st, true);
+    if (exec) {
+        // the replays must finish before their graph is destroyed; the
+        // rest is already queued on st behind them
+        CPRA_CHECK_HIP(hipStreamSynchronize(gst));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (ev0) {
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+        }
+    }
+}
+
+static void dm_check_args(cpra_hip_ctx* h, const void* psi, const void* t1, const void* t2,
+                          const float* amp, uint64_t amp_batch, const uint8_t* support,
+                          const cpra_hip_dm_params* p, const char* who)
+{
+    CPRA_ARG(h && psi && amp && support && p, "%s: NULL argument", who);
+    CPRA_ARG(p->textbook || (t1 && t2), "%s: the reference variant needs t1 and t2", who);
+    CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0, "%s: amp_batch must divide batch", who);
+    CPRA_ARG(p->iters >= 0, "%s: iters < 0", who);
+    CPRA_ARG(p->beta != 0.f, "%s: beta == 0", who);
+    CPRA_ARG(p->space_mode >= 0 && p->space_mode <= 2, "%s: bad space_mode", who);
+    engine_check_geometry(h, who);
+}
+
+// ---- the difference map without allocation -------------------------------
+// cpra_hip_dm_run_noalloc keeps [Y1; Y2; Z1; Z2] (32 bytes per pixel of the
+// batch) in this file-scope array instead of the handle's workspaces.  Each
+// GPU the library runs on holds its own copy until the process ends.  In host
+// code `scratch` only names the symbol: its device address on each GPU comes
+// from hipGetSymbolAddress, looked up once per device (dm_scratch_ptr) and
+// used everywhere host code needs the array.
+static __device__ __align__(256) unsigned char scratch[CPRA_HIP_DM_NOALLOC_MAX_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};
+
+static void* dm_scratch_ptr(cpra_hip_ctx* h)
+{
+    int dev = -1;
+    CPRA_CHECK_HIP(hipGetDevice(&dev));
+    CPRA_ARG(dev == h->device, "cpra_hip_dm_run_noalloc: the handle's device (%d) must be current "
+             "(device %d is)", h->device, dev);
+    CPRA_ARG(dev >= 0 && dev < 64, "cpra_hip_dm_run_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            fail(CPRA_HIP_ERR_HIP, "cpra_hip_dm_run_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                 hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+extern "C" {
+
 int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
                     uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p)
 {
     return eguard(h, [&]() -> int {
-        CPRA_ARG(h && psi && amp && support && p, "cpra_hip_dm_run: NULL argument");
-        CPRA_ARG(p->textbook || (t1 && t2), "cpra_hip_dm_run: the reference variant needs t1 and t2");
-        CPRA_ARG(amp_batch >= 1 && h->batch % amp_batch == 0, "cpra_hip_dm_run: amp_batch must divide batch");
-        CPRA_ARG(p->iters >= 0, "cpra_hip_dm_run: iters < 0");
-        CPRA_ARG(p->beta != 0.f, "cpra_hip_dm_run: beta == 0");
-        CPRA_ARG(p->space_mode >= 0 && p->space_mode <= 2, "cpra_hip_dm_run: bad space_mode");
-        engine_check_geometry(h, "cpra_hip_dm_run");
+        dm_check_args(h, psi, t1, t2, amp, amp_batch, support, p, "cpra_hip_dm_run");
         if (p->iters == 0) return CPRA_HIP_OK;
         const Geo g = geo_of(h);
         const size_t total = (size_t)g.M * g.N * (size_t)g.B;
         h->ws_z.ensure(sizeof(c2<float>) * 2 * total);    // [Z1; Z2]
         h->ws_dm.ensure(sizeof(c2<float>) * 2 * total);   // [Y1; Y2]
-        c2<float>* Y = static_cast<c2<float>*>(h->ws_dm.ptr);
-        c2<float>* Z = static_cast<c2<float>*>(h->ws_z.ptr);
-        hipStream_t st = h->stream;
-        dm_entry_launch(st, g, p, static_cast<const c2<float>*>(psi), static_cast<const c2<float>*>(t1),
-                        static_cast<const c2<float>*>(t2), support, Y, Y + total);
-        const Geo g2{g.M, g.N, 2 * g.B};
-        const float scale = (float)(1.0 / ((double)g.M * g.N));
-        auto one = [&](hipStream_t s, bool last) {
-            launch_cols(h, s, g2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr);
-            dm_rows_launch(s, g, p, last, Z, Z + total, static_cast<c2<float>*>(psi), support, Y,
-                           Y + total, static_cast<c2<float>*>(t1), static_cast<c2<float>*>(t2));
-        };
-        const int body = p->iters - 1;  // the last iteration writes t1, t2 in pattern space
-        // graph: a body of at most kDmGraphIters iterations, captured once per
-        // call and replayed (a graph of all iterations cost more to build than
-        // its launches saved: about 10 ms for 1000 iterations at 512^2)
-        constexpr int kDmGraphIters = 16;
-        const int gk = body < kDmGraphIters ? body : kDmGraphIters;
-        int done = 0;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        hipStream_t gst = nullptr;
-        if (p->use_graph && !h->timing && !h->capturing && body >= 2) {
-            gst = graph_stream(h);
-            if (gst != st) {
-                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-                CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
-                CPRA_CHECK_HIP(hipEventRecord(ev0, st));
-                CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
-            }
-            CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < gk; ++i) one(gst, false);
-            CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
-            CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            ++h->graph_captures;
-            for (; done + gk <= body; done += gk) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
-            if (ev0) {
-                CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
-                CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
-            }
-        }
-        for (; done < body; ++done) one(st, false);
-        one(st, true);
-        if (exec) {
-            // the replays must finish before their graph is destroyed; the
-            // rest is already queued on st behind them
-            CPRA_CHECK_HIP(hipStreamSynchronize(gst));
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            if (ev0) {
-                (void)hipEventDestroy(ev0);
-                (void)hipEventDestroy(ev1);
-            }
-        }
+        dm_run_body(h, psi, t1, t2, amp, amp_batch, support, p,
+                    static_cast<c2<float>*>(h->ws_dm.ptr), static_cast<c2<float>*>(h->ws_z.ptr), true);
+        return CPRA_HIP_OK;
+    });
+}
+
+int cpra_hip_dm_run_noalloc(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
+                            uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p)
+{
+    return eguard(h, [&]() -> int {
+        dm_check_args(h, psi, t1, t2, amp, amp_batch, support, p, "cpra_hip_dm_run_noalloc");
+        const Geo g = geo_of(h);
+        const size_t total = (size_t)g.M * g.N * (size_t)g.B;
+        const size_t need = sizeof(c2<float>) * 4 * total;
+        CPRA_ARG(need <= (size_t)CPRA_HIP_DM_NOALLOC_MAX_BYTES,
+                 "cpra_hip_dm_run_noalloc: the batch needs %zu bytes of scratch (32 per pixel), more "
+                 "than the %zu available; use cpra_hip_dm_run", need,
+                 (size_t)CPRA_HIP_DM_NOALLOC_MAX_BYTES);
+        if (p->iters == 0) return CPRA_HIP_OK;
+        c2<float>* base = static_cast<c2<float>*>(dm_scratch_ptr(h));
+        // every buffer is written in full before it is read: dm_entry writes
+        // Y, each column pass Z, each row pass Y
+        dm_run_body(h, psi, t1, t2, amp, amp_batch, support, p, base, base + 2 * total, false);
         return CPRA_HIP_OK;
     });
 }

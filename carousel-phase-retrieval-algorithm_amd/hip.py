@@ -131,6 +131,7 @@ _SIGS = {
     "cpra_hip_shrinkwrap": (C.c_int, [_H, _P, _P, C.c_float, C.c_float, _P]),
     "cpra_hip_dm_default_params": (None, [C.POINTER(DmParams)]),
     "cpra_hip_dm_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
+    "cpra_hip_dm_run_noalloc": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
     "cpra_hip_dm3d_run": (C.c_int, [_H, _P, _P, _P, _P, _U64, _P, C.POINTER(DmParams)]),
     "cpra_hip_graph_begin": (C.c_int, [_H]),
     "cpra_hip_graph_end": (C.c_int, [_H, C.POINTER(C.c_uint64)]),
@@ -431,7 +432,10 @@ class HipImpl:
                                            C.byref(p), rf.ctypes.data if rf is not None else None))
         return rf
 
-    def dm_run(self, psi, amp, support, t1=None, t2=None, amp_batch=None, **kw) -> bool:
+    def dm_run(self, psi, amp, support, t1=None, t2=None, amp_batch=None, noalloc=False,
+               **kw) -> bool:
+        """cpra_hip_dm_run, or with noalloc=True cpra_hip_dm_run_noalloc (the
+        iteration state in the library's static scratch array)."""
         p = DmParams()
         lib().cpra_hip_dm_default_params(C.byref(p))
         for k, v in kw.items():
@@ -439,8 +443,9 @@ class HipImpl:
                 raise AttributeError(k)
             setattr(p, k, v)
         ab = amp_batch if amp_batch is not None else amp.numel() // (self.m * self.n)
-        return self._check(lib().cpra_hip_dm_run(self._h, _addr(psi), _addr(t1), _addr(t2),
-                                                 _addr(amp), ab, _addr(support), C.byref(p)))
+        fn = lib().cpra_hip_dm_run_noalloc if noalloc else lib().cpra_hip_dm_run
+        return self._check(fn(self._h, _addr(psi), _addr(t1), _addr(t2), _addr(amp), ab,
+                              _addr(support), C.byref(p)))
 
     def dm3d_run(self, psi, amp, support, t1=None, t2=None, amp_batch=None, **kw) -> bool:
         """Difference map on volumes [batch][l][m][n] (cpra_hip_dm3d_run)."""

@@ -75,6 +75,23 @@ typedef struct {
 void cpra_hip_dm_default_params(cpra_hip_dm_params* p);
 int cpra_hip_dm_run(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
                     uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p);
+/* cpra_hip_dm_run without any allocation, for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.  Same
+ * arguments and results (bit for bit) as cpra_hip_dm_run; the iteration
+ * state [Y1; Y2; Z1; Z2], 32 bytes per pixel of the batch, lives in a
+ * static device array of CPRA_HIP_DM_NOALLOC_MAX_BYTES that every GPU the
+ * library runs on holds until the process ends, so batches up to
+ * CPRA_HIP_DM_NOALLOC_MAX_BYTES / 32 pixels (e.g. 8 patterns of 512^2 or 32
+ * of 256^2); larger ones are refused with CPRA_HIP_ERR_ARG.  Launches are
+ * eager (p->use_graph is ignored) and nothing is allocated, created or
+ * synchronised, so a caller may capture the call.  The handle's device must
+ * be current.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads, other handles) race and
+ * must be serialised by the caller. */
+#define CPRA_HIP_DM_NOALLOC_MAX_BYTES (64u << 20)
+int cpra_hip_dm_run_noalloc(cpra_hip_handle h, void* psi, void* t1, void* t2, const float* amp,
+                            uint64_t amp_batch, const uint8_t* support, const cpra_hip_dm_params* p);
 /* The same map on volumes [batch][l][m][n] of the handle's 3D geometry (l, m,
  * n in {64,128,192,256,384,512}), with Forward3D/Backward3D in P_M: the
  * iteration body of the reference's conventional reconstruction,
