@@ -135,7 +135,7 @@ def test_loss_and_grad_f32_match_oracle(hipmod, oracle, gpu, n, B, P):
 
 
 @pytest.mark.parametrize("n,dtype,tol", [(64, "float32", 1e-5), (32, "float64", 1e-11),
-                                         (128, "float32", 1e-5)])
+                                         (128, "float32", 1e-5), (256, "float32", 1e-5)])
 def test_loss_and_grad_3d_match_oracle(hipmod, oracle, gpu, n, dtype, tol):
     vol = ellipsoid_volume(n, 5)
     A = np.abs(centred_fftn(vol))
@@ -213,7 +213,7 @@ def test_fused_gd_decreases_loss(hipmod, gpu):
     assert (np.diff(hist, axis=0) < 0).all(), hist
 
 
-@pytest.mark.parametrize("n,steps", [(64, 5), (128, 2)])
+@pytest.mark.parametrize("n,steps", [(64, 5), (128, 2), (256, 3)])
 def test_fused_gd_3d_matches_oracle_loop(hipmod, oracle, gpu, n, steps):
     vol = ellipsoid_volume(n, 30 + n)
     A32 = np.abs(centred_fftn(vol)).astype(np.float32)

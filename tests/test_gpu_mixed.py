@@ -70,7 +70,7 @@ def test_mixed_matches_single_size_engine(hipmod, gpu, synth):
 
 @pytest.mark.parametrize("space_mode", [1, 2])
 def test_mixed_step_matches_oracle(hipmod, oracle, gpu, synth, space_mode):
-    classes = make_classes(synth, [(128, 2), (192, 1), (384, 1)], 3)
+    classes = make_classes(synth, [(128, 2), (192, 1), (256, 2), (384, 1)], 3)
     got = run_mixed(hipmod, classes, hipmod.MIXED_FUSED, 1, 0, space_mode)
     for c, g in zip(classes, got):
         ref = oracle.dm_textbook(c["psi"], c["amp"], 0.8, 1, mask=c["support"], space_mode=space_mode)

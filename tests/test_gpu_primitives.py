@@ -190,7 +190,7 @@ def test_forward_backward_2d_f64(hipmod, oracle, gpu, M, N, B):
 
 
 @pytest.mark.parametrize("L,M,N,B", [(64, 64, 64, 1), (128, 128, 128, 1), (16, 16, 16, 2),
-                                     (32, 64, 128, 1)])
+                                     (32, 64, 128, 1), (256, 256, 256, 1)])
 def test_forward_backward_3d_f32(hipmod, oracle, gpu, L, M, N, B):
     rng = np.random.default_rng(L + M + N)
     x = (rng.standard_normal((B, L, M, N)) + 1j * rng.standard_normal((B, L, M, N))).astype(np.complex64)
@@ -202,6 +202,38 @@ def test_forward_backward_3d_f32(hipmod, oracle, gpu, L, M, N, B):
         assert h.Backward3D(d)
         h.Sync()
     assert relerr(d.numpy(), x.astype(np.complex128)) < 1e-5
+
+
+def test_forward_backward_3d_512_separable(hipmod, gpu):
+    """Forward3D at 512^3 (the reference benchmark's Step D volume) on a sum of
+    two separable volumes a(z) b(y) c(x): its centred 3D DFT is the sum of the
+    outer products of the three centred 1D DFTs, computed with numpy in FP64
+    line by line, so the whole 1 GiB result is checked without an FP64 3D
+    transform.  Backward3D then returns the input."""
+    n = 512
+    rng = np.random.default_rng(512)
+    f = [[rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3)] for _ in range(2)]
+    c1 = lambda v: np.fft.fftshift(np.fft.fft(np.fft.fftshift(v)))  # noqa: E731
+
+    def outer_sum(parts):
+        out = np.zeros((n, n, n), np.complex64)
+        for a, b, c in parts:
+            out += (a.astype(np.complex64)[:, None, None] * b.astype(np.complex64)[None, :, None]
+                    * c.astype(np.complex64)[None, None, :])
+        return out
+
+    x = outer_sum(f)
+    d = hipmod.to_device(x[None])
+    with hipmod.HipImpl(n, n, n, 1) as h:
+        assert h.Forward3D(d)
+        h.Sync()
+        y = d.numpy()[0]
+        expect = outer_sum([[c1(a), c1(b), c1(c)] for a, b, c in f])
+        assert relerr(y, expect) < 1e-5
+        del y, expect
+        assert h.Backward3D(d)
+        h.Sync()
+    assert relerr(d.numpy()[0], x) < 1e-5
 
 
 def test_fft_known_answers_f32(hipmod, gpu):
