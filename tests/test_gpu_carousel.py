@@ -49,7 +49,7 @@ def run_gpu(hipmod, d, outer, inner=1, use_graph=1, want_rfactor=False, tuning=N
                                                     (64, 24, 3, 1, "rows_sparse=1;"),
                                                     (64, 24, 2, 0, None), (128, 180, 1, 1, None),
                                                     (128, 60, 2, 1, None), (192, 8, 1, 1, None),
-                                                    (256, 8, 2, 1, None)])
+                                                    (256, 8, 2, 1, None), (384, 4, 1, 1, None)])
 def test_carousel_matches_oracle(hipmod, oracle, gpu, synth, n, P, outer, inner, tuning):
     d = synth.make_carousel(n, P, config=40 + n)
     x, vol, _ = run_gpu(hipmod, d, outer, inner, use_graph=0, tuning=tuning)
