@@ -301,9 +301,14 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=5, timing_iter
         # K forward column DFTs and one column IDFT at the end of the run
         "proj_fft": (px2 * 16 * (K + 1) / K, (K + 1) / K),
         "merge": (px2 * 8 + vx * 8, 1),                # gather r (slices already [P][v][u]), volume w
-        # n <= 128: two whole-plane kernels and the z-line kernel; above, five passes
-        "volume": ((3 if n <= 128 else 5) * vx * 16 + vx, 3 if n <= 128 else 5),
-        "extract": (px2 * 8 + vx * 8, 2),
+        # n <= 128: the inverse whole-plane kernel and the z-line kernel (the
+        # forward plane kernel runs fused with the re-projection); above,
+        # five passes
+        "volume": ((2 if n <= 128 else 5) * vx * 16 + vx, 2 if n <= 128 else 5),
+        # n <= 128: forward plane transform (read and write the volume) and
+        # the re-projection from LDS (write the slices); above, the
+        # extraction (read the volume, write the slices)
+        "extract": (px2 * 8 + vx * (16 if n <= 128 else 8), 1),
     }
     kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}
     dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
@@ -755,6 +760,16 @@ def main():
 
     assert hip.device_count() > 0, "bench.py needs a GPU"
     fp64 = not args.no_fp64_check
+    if args.calibrate_copy:
+        ncal = 1 << 26
+        a_ = hip.zeros(ncal, np.complex64)
+        b_ = hip.zeros(ncal, np.complex64)
+        with hip.HipImpl(1, 1, 1, 1) as hc:
+            hc.MergeAddData(a_, b_, 1.0, 0.0, ncal)
+            hc.Sync()
+        a_.free()
+        b_.free()
+        log("bench: copy calibration kernel done (2^26 complex64)")
     if args.workload != "hio2d":
         fn = {"carousel3d": bench_carousel, "gd2d": bench_gd2d, "gd3d": bench_gd3d,
               "mixed": bench_mixed, "conv3d": bench_conv3d,
@@ -776,16 +791,6 @@ def main():
     sched = dict(synth.HIO_SCHEDULE)
     sched.pop("iters")
     h = hip.HipImpl(n, n, 1, P)
-
-    if args.calibrate_copy:
-        ncal = 1 << 26
-        a_ = hip.zeros(ncal, np.complex64)
-        b_ = hip.zeros(ncal, np.complex64)
-        h.MergeAddData(a_, b_, 1.0, 0.0, ncal)
-        h.Sync()
-        a_.free()
-        b_.free()
-        log("bench: copy calibration kernel done (2^26 complex64)")
 
     # autotune the kernel variants on this workload (restores x, support)
     if args.tuning:

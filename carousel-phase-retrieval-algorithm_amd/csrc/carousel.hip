@@ -22,8 +22,9 @@
 // not seven: the z-line kernel runs IDFT_z, the constraint and DFT_z on each
 // line in registers, between the inverse and forward (y, x) passes.  The
 // outer body (12 launches for inner = 1; 10 for n <= 128, where one kernel per
-// direction transforms whole z-planes in LDS) is captured once in a hipGraph and
-// replayed; its working set (the projections, the transposed slices and the
+// direction transforms whole z-planes in LDS) is captured in a hipGraph, up to
+// ten outer iterations per graph, which the handle keeps across calls and
+// replays; its working set (the projections, the transposed slices and the
 // volume: 56 MB at 180 x 128^2 and 128^3) stays in the Infinity Cache.
 #include <cmath>
 
@@ -33,7 +34,9 @@
 
 namespace cpra {
 
-enum ZlMode { ZL_ROUNDTRIP = 0, ZL_INVERSE = 1 };
+// ZL_ROUNDTRIP_NO_FORWARD is a volume_constraint mode only (the z-line
+// kernel runs ZL_ROUNDTRIP)
+enum ZlMode { ZL_ROUNDTRIP = 0, ZL_INVERSE = 1, ZL_ROUNDTRIP_NO_FORWARD = 2 };
 
 // z-lines of the volume [L][W] (W = M*N points per plane), after the inverse
 // x and y passes: IDFT_z (scale), the real-positive support projection, and
@@ -108,17 +111,28 @@ static int zline_tile(int n)
 // XFIRST runs x then y, the order of the inverse passes; otherwise y then x,
 // the order of the forward passes.  Each line goes through the same plan,
 // twiddles and centring fold as in the one-axis kernels of fft.hip.
+// angles per group of the fused plane transform and re-projection (the
+// group's (cos, sin) table sits in LDS beside the plane)
+constexpr int kPlaneExtractMaxAngles = 256;
+
+template <int N> struct PlaneGeom {
+    using PL = typename SizeCfg<N>::P;
+    static constexpr int E = PL::E, TT = PL::T, TPB = N * TT, SP = N + N / 16, K = N * N / TPB;
+    static_assert(64 % TT == 0 && TPB % 64 == 0 && N % 8 == 0, "plane layout");
+    static __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+};
+
+// The plane Pp (N x N) transformed along both axes in lds; the result is left
+// in NAT layout.
 template <int N, int DIR, bool XFIRST>
-__global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d(c2<float>* V)
+__device__ __forceinline__ void plane2d_body(const c2<float>* Pp, c2<float>* lds)
 {
     using Vt = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, TPB = N * TT, SP = N + N / 16, K = N * N / TPB;
-    static_assert(64 % TT == 0 && TPB % 64 == 0 && N % 8 == 0, "plane layout");
-    __shared__ Vt lds[N * SP];
+    using G = PlaneGeom<N>;
+    using PL = typename G::PL;
+    constexpr int E = G::E, TT = G::TT, TPB = G::TPB, SP = G::SP, K = G::K;
     const int tid = threadIdx.x;
-    Vt* Pp = V + (size_t)blockIdx.x * (N * N);
-    auto pad = [](int i) { return i + (i >> 4); };
+    auto pad = [](int i) { return G::pad(i); };
 
     Vt g[K];
     if constexpr (XFIRST) {
@@ -164,12 +178,95 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d(c2<float>* V)
     // XFIRST: NAT -> (x) -> TRN -> (y) -> NAT; else TRN -> (y) -> NAT -> (x) -> NAT
     axis(true);
     axis(XFIRST);
+}
 
-    sfor<K>([&](auto kc) {
+template <int N>
+__device__ __forceinline__ void plane2d_store(c2<float>* Pp, const c2<float>* lds)
+{
+    using G = PlaneGeom<N>;
+    sfor<G::K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int i = tid + k * TPB;
-        Pp[i] = lds[(i / N) * SP + pad(i % N)];
+        const int i = threadIdx.x + k * G::TPB;
+        Pp[i] = lds[(i / N) * G::SP + G::pad(i % N)];
     });
+}
+
+template <int N, int DIR, bool XFIRST>
+__global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d(c2<float>* V)
+{
+    __shared__ c2<float> lds[N * PlaneGeom<N>::SP];
+    c2<float>* Pp = V + (size_t)blockIdx.x * (N * N);
+    plane2d_body<N, DIR, XFIRST>(Pp, lds);
+    plane2d_store<N>(Pp, lds);
+}
+
+// The forward (y, x) plane transform fused with the re-projection of the
+// plane: block (u, angle group g of `groups`) transforms plane z = u in LDS,
+// group 0 writes it back to V, and every group samples its angles' slice rows
+// u from the LDS plane (the bilinear weights and FP64 geometry of
+// k_extract_plane, so the same bits), which the separate extraction would
+// read back from V.  Each group transforms the plane itself: the groups put
+// the CUs that one block per plane leaves idle to work on the sampling.
+template <int N>
+__global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d_extract(
+    c2<float>* V, const double* __restrict__ trig, int P, int groups, c2<float>* __restrict__ out)
+{
+    using Vt = c2<float>;
+    using G = PlaneGeom<N>;
+    __shared__ Vt lds[N * G::SP];
+    __shared__ double tl[2 * kPlaneExtractMaxAngles];
+    const int u = (int)blockIdx.x / groups, g = (int)blockIdx.x % groups;
+    const int p0 = (int)((int64_t)g * P / groups), p1 = (int)((int64_t)(g + 1) * P / groups);
+    for (int i = threadIdx.x; i < 2 * (p1 - p0); i += blockDim.x) tl[i] = trig[2 * p0 + i];
+    Vt* Pp = V + (size_t)u * (N * N);
+    plane2d_body<N, -1, false>(Pp, lds);  // ends with a barrier
+    if (g == 0) plane2d_store<N>(Pp, lds);
+    const double c = (double)(N / 2);
+    for (int i = threadIdx.x; i < (p1 - p0) * N; i += blockDim.x) {
+        const int pl = i / N, v = i % N;
+        const int p = p0 + pl;
+        const double pxd = c + ((double)v - c) * tl[2 * pl], pyd = c + ((double)v - c) * tl[2 * pl + 1];
+        const double fxd = floor(pxd), fyd = floor(pyd);
+        const float fx = (float)fxd, fy = (float)fyd;
+        const float ax = (float)(pxd - fxd), ay = (float)(pyd - fyd);
+        Vt acc = mk<Vt>(0.f, 0.f);
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dxx = 0; dxx < 2; ++dxx) {
+                const int xi = (int)fx + dxx, yi = (int)fy + dy;
+                const float w = (dxx ? ax : 1.f - ax) * (dy ? ay : 1.f - ay);
+                if (xi >= 0 && xi < N && yi >= 0 && yi < N && w != 0.f) {
+                    const Vt a = lds[yi * G::SP + G::pad(xi)];
+                    acc = mk<Vt>(acc.x + w * a.x, acc.y + w * a.y);
+                }
+            }
+        out[((int64_t)p * N + u) * N + v] = acc;
+    }
+}
+
+// true when the fused kernel covers (n, P): n <= 128 planes in LDS and at
+// most kPlaneExtractMaxAngles angles per group
+static bool plane2d_extract_launch(hipStream_t st, int n, int P, c2<float>* V, const double* trig,
+                                   c2<float>* out)
+{
+    int groups = 1;
+    while (n * groups < 256 && groups * 2 <= P) groups *= 2;
+    while ((P + groups - 1) / groups > kPlaneExtractMaxAngles) groups *= 2;
+    const dim3 grid((unsigned)(n * groups));
+    switch (n) {
+    case 64:
+        hipLaunchKernelGGL((k_plane2d_extract<64>), grid, dim3(64 * SizeCfg<64>::P::T), 0, st, V, trig, P,
+                           groups, out);
+        break;
+    case 128:
+        hipLaunchKernelGGL((k_plane2d_extract<128>), grid, dim3(128 * SizeCfg<128>::P::T), 0, st, V, trig,
+                           P, groups, out);
+        break;
+    default: return false;
+    }
+    check_launch("k_plane2d_extract");
+    return true;
 }
 
 // true when the plane kernel covers size n (the plane fits in LDS)
@@ -316,16 +413,20 @@ struct Carousel {
 
     // inverse (x, y) passes, fused z-line constraint, forward (y, x) passes
     // (n <= 128: one whole-plane kernel per direction for the x and y passes)
+    // mode ZL_ROUNDTRIP_NO_FORWARD: the round trip without the forward (y, x)
+    // transform, which the caller runs fused with the re-projection
     void volume_constraint(int mode)
     {
         const int64_t W = (int64_t)n * n;
         const bool plane = engine_plane_fft(h);
+        const bool forward = mode == ZL_ROUNDTRIP;
+        if (mode == ZL_ROUNDTRIP_NO_FORWARD) mode = ZL_ROUNDTRIP;
         if (!(plane && plane2d_launch(h->stream, n, +1, vol))) {
             fft_pass_contig(h, vol, n, (int64_t)n * n, +1, true, 1.0, false);            // x
             fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, +1, true, 1.0, false);  // y
         }
         zline_launch(h->stream, n, mode, vol, sup3d);                                // z
-        if (mode == ZL_ROUNDTRIP && !(plane && plane2d_launch(h->stream, n, -1, vol))) {
+        if (forward && !(plane && plane2d_launch(h->stream, n, -1, vol))) {
             fft_pass_strided(h, vol, n, (int64_t)n * n, n, n, W, -1, true, 1.0, false);
             fft_pass_contig(h, vol, n, (int64_t)n * n, -1, true, 1.0, false);
         }
@@ -381,11 +482,18 @@ struct Carousel {
         if (lines) merge_apply_lines_c64(h, sT, vol, n, P);
         else merge_apply_c64(h, Y, vol, n, P);
         if (kt) kt->end(h, h->stream);
+        // the forward plane transform and the re-projection in one kernel
+        // where the planes fit in LDS and the gather list's trig table of
+        // these angles is there
+        const bool fused = engine_plane_fft(h) && engine_plane_extract(h) && n <= 128 &&
+                           h->mplan_n == n && h->mplan_P == P && h->ws_trig.ptr;
         if (kt) kt->begin(h, h->stream, KT_VOL);
-        volume_constraint(ZL_ROUNDTRIP);
+        volume_constraint(fused ? ZL_ROUNDTRIP_NO_FORWARD : ZL_ROUNDTRIP);
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_EXTRACT);
-        extract_c64(h, vol, Y, angles, n, P);
+        if (!(fused && plane2d_extract_launch(h->stream, n, P, vol, static_cast<const double*>(h->ws_trig.ptr),
+                                              Y)))
+            extract_c64(h, vol, Y, angles, n, P);
         if (kt) kt->end(h, h->stream);
     }
 };
@@ -465,6 +573,9 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         if (p->outer_iters > 0) {
             cr.outer(ktp, false);  // eager: sizes every workspace before any capture
             done = 1;
+            // two more eager iterations queue GPU work that covers the host's
+            // capture and graph update below
+            for (; done < 3 && p->outer_iters - done >= 12; ++done) cr.outer(ktp, true);
         }
         const int rest = p->outer_iters - done;
         if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest >= 2) {
@@ -478,27 +589,45 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             }
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
+            // a body of up to kBody outer iterations: each graph launch costs
+            // about 4.5 us of idle GPU at its boundary, a third of an outer
+            // iteration's kernel time at 180 x 128^2, so one launch covers
+            // several; the remainder runs eagerly
+            constexpr int kBody = 10;
+            const int body = rest < kBody ? rest : kBody;
             h->stream = gst;
             try {
                 CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
-                cr.outer(nullptr, true);
+                for (int i = 0; i < body; ++i) cr.outer(nullptr, true);
                 CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
             } catch (...) {
                 h->stream = st;
                 throw;
             }
             h->stream = st;
-            CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            ++h->graph_captures;
-            for (int i = 0; i < rest; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+            KeptGraph& kg = h->kept[KG_CAROUSEL];
+            try {
+                exec = kept_graph_exec(h, kg, graph);
+            } catch (...) {
+                (void)hipGraphDestroy(graph);
+                throw;
+            }
+            (void)hipGraphDestroy(graph);
+            graph = nullptr;
+            for (int i = 0; i + body <= rest; i += body) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+            kept_graph_launched(kg, gst);
+            h->stream = gst;
+            try {
+                for (int i = rest - rest % body; i < rest; ++i) cr.outer(nullptr, true);
+            } catch (...) {
+                h->stream = st;
+                throw;
+            }
+            h->stream = st;
             if (ev0) {
                 CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
                 CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
-            }
-            CPRA_CHECK_HIP(hipStreamSynchronize(gst));
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            if (ev0) {
+                CPRA_CHECK_HIP(hipEventSynchronize(ev1));
                 (void)hipEventDestroy(ev0);
                 (void)hipEventDestroy(ev1);
             }

@@ -70,6 +70,26 @@ struct Buffer {
 
 struct EngineGraph;  // engine2d.hip
 
+// An engine's iteration graph kept by its handle across calls.  Each call
+// captures its body again (host-side recording) and re-points the kept
+// executable at it with hipGraphExecUpdate; only a changed topology (another
+// launch sequence) is instantiated anew.  `done` marks the last replay
+// queued, which an update waits for, so that no replay in flight sees its
+// arguments change.
+struct KeptGraph {
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    void release()
+    {
+        if (done) (void)hipEventSynchronize(done);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (done) (void)hipEventDestroy(done);
+        exec = nullptr;
+        done = nullptr;
+    }
+};
+enum { KG_CAROUSEL = 0, KG_MIXED, KG_DM, KG_DM3D, KG_COUNT };
+
 }  // namespace cpra
 
 struct cpra_hip_ctx {
@@ -93,6 +113,8 @@ struct cpra_hip_ctx {
     std::map<uint64_t, int> tuning;
     // captured HIO chunk bodies: [0] dense rows, [1] sparse rows
     cpra::EngineGraph* graph[2] = {nullptr, nullptr};
+    // the other engines' iteration graphs, kept across calls (KG_*)
+    cpra::KeptGraph kept[cpra::KG_COUNT];
     // sparse rows for engine_hio_iteration (decided by engine_hio_prepare)
     bool hio_sparse = false;
     // the support rows have been listed (the adaptive mode's flag is valid)
@@ -125,6 +147,12 @@ inline unsigned grid_1d(uint64_t work, unsigned block, unsigned cap = 1u << 20)
     if (g > cap) g = cap;
     return (unsigned)g;
 }
+
+// The executable of `kg` updated to (or instantiated from) the captured
+// graph g, which the caller still owns and destroys.  engine2d.hip.
+hipGraphExec_t kept_graph_exec(cpra_hip_ctx* h, KeptGraph& kg, hipGraph_t g);
+// Records kg.done on st after the replays the caller has queued.
+void kept_graph_launched(KeptGraph& kg, hipStream_t st);
 
 // Stream on which an engine captures and replays its iteration graph: the
 // handle's own stream when it has one, so that the replays queue behind the
@@ -214,6 +242,7 @@ bool engine_tiled_state(const cpra_hip_ctx* h);
 // whole-plane x/y transforms in the carousel's volume constraint (tuning plane=)
 bool engine_plane_fft(const cpra_hip_ctx* h);
 bool engine_merge_lines(const cpra_hip_ctx* h);
+bool engine_plane_extract(const cpra_hip_ctx* h);
 void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void* in, void* out);
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out);
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)

@@ -651,6 +651,14 @@ bool engine_merge_lines(const cpra_hip_ctx* h)
     return it == h->tuning.end() || it->second != 0;
 }
 
+// Tuning key 7: the carousel's forward plane transform and re-projection in
+// one kernel (1, the default) or two.
+bool engine_plane_extract(const cpra_hip_ctx* h)
+{
+    auto it = h->tuning.find((uint64_t)7 << 32);
+    return it == h->tuning.end() || it->second != 0;
+}
+
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
 {
     launch_amp_retile(st, n, patterns, in, out);
@@ -802,8 +810,32 @@ hipStream_t graph_stream(cpra_hip_ctx* h)
     return h->aux_stream;
 }
 
+hipGraphExec_t kept_graph_exec(cpra_hip_ctx* h, KeptGraph& kg, hipGraph_t g)
+{
+    if (kg.exec) {
+        if (kg.done) CPRA_CHECK_HIP(hipEventSynchronize(kg.done));
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(kg.exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess)
+            return kg.exec;
+        (void)hipGetLastError();  // a refused update leaves the old executable: rebuild
+        (void)hipGraphExecDestroy(kg.exec);
+        kg.exec = nullptr;
+    }
+    CPRA_CHECK_HIP(hipGraphInstantiate(&kg.exec, g, nullptr, nullptr, 0));
+    ++h->graph_captures;
+    return kg.exec;
+}
+
+void kept_graph_launched(KeptGraph& kg, hipStream_t st)
+{
+    if (!kg.done) CPRA_CHECK_HIP(hipEventCreateWithFlags(&kg.done, hipEventDisableTiming));
+    CPRA_CHECK_HIP(hipEventRecord(kg.done, st));
+}
+
 void cpra_engine_release(cpra_hip_ctx* h)
 {
+    for (KeptGraph& kg : h->kept) kg.release();
     for (EngineGraph*& eg : h->graph) {
         if (eg) {
             eg->release();
@@ -969,6 +1001,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         if (ev0) {
             CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
             CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            CPRA_CHECK_HIP(hipEventSynchronize(ev1));
             CPRA_CHECK_HIP(hipEventDestroy(ev0));
             CPRA_CHECK_HIP(hipEventDestroy(ev1));
         }
@@ -1064,23 +1097,26 @@ static void dm3d_run(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const float
             CPRA_CHECK_HIP(hipEventRecord(ev0, st));
             CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
         }
-        hipGraph_t graph;
-        hipGraphExec_t exec;
+        hipGraph_t graph = nullptr;
         // one iteration captured, replayed `body` times
         CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
         one(gst, false);
         CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
-        CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        ++h->graph_captures;
+        KeptGraph& kg = h->kept[KG_DM3D];
+        hipGraphExec_t exec = nullptr;
+        try {
+            exec = kept_graph_exec(h, kg, graph);
+        } catch (...) {
+            (void)hipGraphDestroy(graph);
+            throw;
+        }
+        (void)hipGraphDestroy(graph);
         for (int i = 0; i < body; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+        kept_graph_launched(kg, gst);
         if (ev0) {
             CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
             CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
-        }
-        CPRA_CHECK_HIP(hipStreamSynchronize(gst));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        if (ev0) {
+            CPRA_CHECK_HIP(hipEventSynchronize(ev1));
             (void)hipEventDestroy(ev0);
             (void)hipEventDestroy(ev1);
         }
@@ -1119,42 +1155,40 @@ static void dm_run_body(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const fl
     constexpr int kDmGraphIters = 16;
     const int gk = body < kDmGraphIters ? body : kDmGraphIters;
     int done = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipStream_t gst = nullptr;
     if (allow_graph && p->use_graph && !h->timing && !h->capturing && body >= 2) {
-        gst = graph_stream(h);
+        hipStream_t gst = graph_stream(h);
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (gst != st) {
             CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
             CPRA_CHECK_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
             CPRA_CHECK_HIP(hipEventRecord(ev0, st));
             CPRA_CHECK_HIP(hipStreamWaitEvent(gst, ev0, 0));
         }
+        hipGraph_t graph = nullptr;
         CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < gk; ++i) one(gst, false);
         CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
-        CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        ++h->graph_captures;
+        KeptGraph& kg = h->kept[KG_DM];
+        hipGraphExec_t exec = nullptr;
+        try {
+            exec = kept_graph_exec(h, kg, graph);
+        } catch (...) {
+            (void)hipGraphDestroy(graph);
+            throw;
+        }
+        (void)hipGraphDestroy(graph);
         for (; done + gk <= body; done += gk) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+        kept_graph_launched(kg, gst);
         if (ev0) {
             CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
             CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
+            CPRA_CHECK_HIP(hipEventSynchronize(ev1));
+            CPRA_CHECK_HIP(hipEventDestroy(ev0));
+            CPRA_CHECK_HIP(hipEventDestroy(ev1));
         }
     }
     for (; done < body; ++done) one(st, false);
     one(st, true);
-    if (exec) {
-        // the replays must finish before their graph is destroyed; the
-        // rest is already queued on st behind them
-        CPRA_CHECK_HIP(hipStreamSynchronize(gst));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        if (ev0) {
-            (void)hipEventDestroy(ev0);
-            (void)hipEventDestroy(ev1);
-        }
-    }
 }
 
 static void dm_check_args(cpra_hip_ctx* h, const void* psi, const void* t1, const void* t2,
@@ -1502,6 +1536,9 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
             } else if (std::sscanf(p, "merge_lines=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)6 << 32] = v ? 1 : 0;
                 p += used;
+            } else if (std::sscanf(p, "plane_extract=%d;%n", &v, &used) == 1 && used > 0) {
+                h->tuning[(uint64_t)7 << 32] = v ? 1 : 0;
+                p += used;
             } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
                 CPRA_ARG(col_variant_valid(v), "cpra_hip_set_tuning: variant %d out of range", v);
                 h->tuning[((uint64_t)1 << 32) | (uint64_t)m] = v;
@@ -1522,11 +1559,11 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
         std::snprintf(out, out_len,
-                      "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;rows_sparse_used=%d;"
-                      "graph_captures=%lld;",
+                      "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;plane_extract=%d;"
+                      "rows_sparse_used=%d;graph_captures=%lld;",
                       (int)h->m, col_variant_code(cv), sparse_mode(h), (long long)iteration_chunk(h),
                       tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, engine_merge_lines(h) ? 1 : 0,
-                      rows_sparse_used(h), (long long)h->graph_captures);
+                      engine_plane_extract(h) ? 1 : 0, rows_sparse_used(h), (long long)h->graph_captures);
         return CPRA_HIP_OK;
     });
 }

@@ -212,6 +212,8 @@ __global__ __launch_bounds__(256) void k_slice_extract(const E* __restrict__ vol
 // 4 KiB table of its angles' (cos, sin) (at most 256 angles per group); the
 // CU has 160 KiB
 constexpr size_t kExtractPlaneLds = 128 * 1024;
+// the largest carousel size (the merge's LDS tile is n x kMergeXT elements)
+constexpr int kMergeMaxN = 64 * kMergeMaxZPerLane;
 constexpr size_t kExtractTrigLds = 4096;
 
 // R with the plane in LDS (n*n elements fit): block (u, angle group) loads
@@ -648,15 +650,18 @@ void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
                        lidx, lw, inv);
     check_launch("k_merge_fill");
     CPRA_CHECK_HIP(hipStreamSynchronize(st));  // `off` and `heavy` are released on return
-    // the merge's tile (n * 16 elements: 64 KiB at n = 512) beside its static
-    // flags can exceed the default dynamic-LDS limit
+    // The merge's tile (n * 16 elements: 64 KiB at n = 512) beside its static
+    // flags, and the extraction's trig table and plane (132 KiB at n = 128),
+    // can exceed the default dynamic-LDS limit.  The attribute is a property
+    // of the kernel for the whole process, so it is set to the largest size
+    // the code launches, not to this handle's n: a handle of another size
+    // must not lower it under one that is live.
     CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_merge_apply<c2<float>, float>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(sizeof(c2<float>) * (size_t)n * kMergeXT)));
-    if (sizeof(c2<float>) * (size_t)n * n <= kExtractPlaneLds)
-        CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_extract_plane<c2<float>, float>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(kExtractTrigLds + sizeof(c2<float>) * (size_t)n * n)));
+                                       (int)(sizeof(c2<float>) * (size_t)kMergeMaxN * kMergeXT)));
+    CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_extract_plane<c2<float>, float>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(kExtractTrigLds + kExtractPlaneLds)));
     h->mplan_n = n;
     h->mplan_P = P;
     h->mplan_entries = run;

@@ -341,17 +341,21 @@ int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes,
             CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
             r.iteration(gst, false, nullptr);
             CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
-            CPRA_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            ++h->graph_captures;
+            KeptGraph& kg = h->kept[KG_MIXED];
+            try {
+                exec = kept_graph_exec(h, kg, graph);
+            } catch (...) {
+                (void)hipGraphDestroy(graph);
+                throw;
+            }
+            (void)hipGraphDestroy(graph);
+            graph = nullptr;
             for (int i = 0; i < body; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+            kept_graph_launched(kg, gst);
             if (ev0) {
                 CPRA_CHECK_HIP(hipEventRecord(ev1, gst));
                 CPRA_CHECK_HIP(hipStreamWaitEvent(st, ev1, 0));
-            }
-            CPRA_CHECK_HIP(hipStreamSynchronize(gst));
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            if (ev0) {
+                CPRA_CHECK_HIP(hipEventSynchronize(ev1));
                 (void)hipEventDestroy(ev0);
                 (void)hipEventDestroy(ev1);
             }

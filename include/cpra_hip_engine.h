@@ -136,12 +136,14 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * per launch of the iteration kernels, 0 = whole batch; tiled: tile-major
  * state for square patterns), "plane=<0|1>;" (whole-plane LDS transforms
  * in the carousel's volume constraint for n <= 128; 0 = one pass per axis),
- * and "merge_lines=<0|1>;" (the carousel's forward column DFT written
+ * "merge_lines=<0|1>;" (the carousel's forward column DFT written
  * straight into the merge's [P][v][u] layout; 0 = the DFT in place and a
- * transpose before the merge). */
+ * transpose before the merge) and "plane_extract=<0|1>;" (n <= 128: the
+ * volume's forward plane transform and the re-projection in one kernel; 0 =
+ * two). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
- * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;rows_sparse_used=<0|1>;
+ * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_extract=<0|1>;rows_sparse_used=<0|1>;
  * graph_captures=<k>;" (rows_sparse_used: the row mode of the last iterations
  * run; in the adaptive mode it is read from the device, which synchronises
  * the handle's stream; graph_captures: how many iteration graphs the engines

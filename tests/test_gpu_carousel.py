@@ -173,3 +173,21 @@ def test_carousel_unsupported(hipmod, gpu):
     with hipmod.HipImpl(n, n, 1, 4) as h:
         with pytest.raises(hipmod.CpraHipError):
             h.carousel3d_run(z, a, s2, ang, s3, z, outer_iters=1)   # vol aliases x
+
+
+@pytest.mark.parametrize("n,P", [(64, 36), (128, 180), (128, 7)])
+def test_carousel_fused_plane_extract_matches_two_kernels(hipmod, gpu, synth, n, P):
+    """For n <= 128 the volume's forward plane transform and the
+    re-projection run as one kernel (tuning plane_extract=1, the default),
+    each angle group transforming its plane itself; plane_extract=0 runs the
+    plane kernel and the plane-staged extraction.  The same transforms and
+    the same sampling arithmetic: identical bits, the volume included."""
+    d = synth.make_carousel(n, P, config=57)
+    x1, v1, _ = run_gpu(hipmod, d, 3, use_graph=1, tuning="plane_extract=1;")
+    x0, v0, _ = run_gpu(hipmod, d, 3, use_graph=1, tuning="plane_extract=0;")
+    assert np.array_equal(x1, x0)
+    assert np.array_equal(v1, v0)
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        assert h.get_tuning()["plane_extract"] == "1"
+        assert h.set_tuning("plane_extract=0;")
+        assert h.get_tuning()["plane_extract"] == "0"
