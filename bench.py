@@ -253,7 +253,7 @@ def kernel_entry(ms, cnt, iters, bytes_per_iter, launches_per_iter=1):
             "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
-def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=5, timing_iters=5, fp64=True):
+def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_iters=5, fp64=True):
     """3D carousel: 180 projections of 128^2 (12 random ellipsoids in 128^3,
     Poisson noise at 1e7 photons), 50 outer iterations of [1 HIO iteration on
     every projection, merge, 3D constraint, re-projection]; graph-captured
@@ -270,21 +270,31 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=5, timing_iter
                                           dev["support3d"], vol, outer_iters=k, inner_iters=1,
                                           beta=0.9, use_graph=g)
     run(warmup)
-    h.Sync()
-    t0 = time.perf_counter()
-    run(outer)
-    h.Sync()
-    el = time.perf_counter() - t0
-    its = outer / el
-    log(f"bench[carousel]: {outer} outer iterations in {el * 1e3:.2f} ms -> {its:.1f} it/s")
-    # the same run with eager launches (use_graph = 0), reported beside the value
     run(warmup, 0)
     h.Sync()
-    t0 = time.perf_counter()
-    run(outer, 0)
-    h.Sync()
-    its_eager = outer / (time.perf_counter() - t0)
-    log(f"bench[carousel]: eager launches -> {its_eager:.1f} it/s")
+
+    def timed(g):
+        # each run from the same start (uploaded outside the timed region)
+        dev["x0"].upload(d["x0"])
+        h.Sync()
+        t0 = time.perf_counter()
+        run(outer, g)
+        h.Sync()
+        return time.perf_counter() - t0
+
+    # graph replays (the value) and eager launches (beside it), alternating,
+    # three runs each; the median of each
+    runs = {1: [], 0: []}
+    for _ in range(3):
+        for g in (0, 1):
+            runs[g].append(timed(g))
+    el = sorted(runs[1])[1]
+    its = outer / el
+    its_eager = outer / sorted(runs[0])[1]
+    log(f"bench[carousel]: {outer} outer iterations in {el * 1e3:.2f} ms -> {its:.1f} it/s "
+        f"(graph runs {[round(outer / t, 1) for t in runs[1]]})")
+    log(f"bench[carousel]: eager launches -> {its_eager:.1f} it/s "
+        f"(runs {[round(outer / t, 1) for t in runs[0]]})")
     h.set_kernel_timing(True)
     run(timing_iters, 0)
     h.Sync()
@@ -809,7 +819,7 @@ def main():
     captures0 = int(h.get_tuning()["graph_captures"])
     log(f"bench: warmup done ({warm} iterations, {captures0} graph captures so far)")
 
-    def timed_run(k):
+    def timed_run(k, g=1):
         # the timed steps are iterations 1..k of the schedule: the state goes
         # back to the seeded start first (outside the timed region), since
         # the work per iteration follows the support, which the shrinkwrap
@@ -818,7 +828,7 @@ def main():
         sup.upload(data["support"])
         h.Sync()
         t0 = time.perf_counter()
-        h.hio_run(x, amp, sup, iters=k, use_graph=1, **sched)
+        h.hio_run(x, amp, sup, iters=k, use_graph=g, **sched)
         h.Sync()
         return time.perf_counter() - t0
 
@@ -827,6 +837,9 @@ def main():
     captures1 = int(h.get_tuning()["graph_captures"])
     log(f"bench: {args.steps} iterations in {elapsed:.3f} s -> {its:.1f} it/s "
         f"({captures1 - captures0} graph captures in the timed run)")
+    # the same steps with eager launches, beside the value
+    its_eager = args.steps / timed_run(args.steps, 0)
+    log(f"bench: eager launches -> {its_eager:.1f} it/s")
     full_sched = None
     if args.steps != 200:
         # the whole 200-iteration schedule, for reference beside the value
@@ -902,6 +915,7 @@ def main():
                    "warmup_iterations_run": warm,
                    "timed_iterations": f"1..{args.steps} of the schedule, from the seeded start"},
         "schedule_200_iters_per_s": full_sched if full_sched is not None else round(its, 2),
+        "value_eager_launches": round(its_eager, 2),
         "pattern_iters_per_s": round(its * P, 1),
         "hbm_frac_algorithmic_21Bpx": round(iter_algo_frac, 4),
         "roofline": roof,

@@ -27,6 +27,8 @@
 // replays; its working set (the projections, the transposed slices and the
 // volume: 56 MB at 180 x 128^2 and 128^3) stays in the Infinity Cache.
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "common.hpp"
 #include "fft_core.hpp"
@@ -578,7 +580,14 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             for (; done < 3 && p->outer_iters - done >= 12; ++done) cr.outer(ktp, true);
         }
         const int rest = p->outer_iters - done;
-        if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest >= 2) {
+        // a body of kBody outer iterations: each graph launch costs about
+        // 4.5 us of idle GPU at its boundary, a third of an outer iteration's
+        // kernel time at 180 x 128^2, so one launch covers several.  The body
+        // has the same length in every call, so the graph kept by the handle
+        // is re-pointed, not rebuilt; shorter runs and the remainder run
+        // eagerly
+        constexpr int kBody = 10;
+        if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest >= kBody) {
             hipStream_t gst = graph_stream(h);
             hipEvent_t ev0 = nullptr, ev1 = nullptr;
             if (gst != st) {
@@ -589,31 +598,50 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             }
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
-            // a body of up to kBody outer iterations: each graph launch costs
-            // about 4.5 us of idle GPU at its boundary, a third of an outer
-            // iteration's kernel time at 180 x 128^2, so one launch covers
-            // several; the remainder runs eagerly
-            constexpr int kBody = 10;
-            const int body = rest < kBody ? rest : kBody;
-            h->stream = gst;
-            try {
-                CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
-                for (int i = 0; i < body; ++i) cr.outer(nullptr, true);
-                CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
-            } catch (...) {
-                h->stream = st;
-                throw;
-            }
-            h->stream = st;
+            const int body = kBody;
+            // what the body's launches depend on: when a call repeats the
+            // kept graph's key, the graph replays as it is, with no capture
+            // and no update (their host time, about half a millisecond for
+            // the 100 launches of the body at 180 x 128^2, would otherwise
+            // leave the GPU idle)
+            std::vector<uint64_t> key;
+            engine_state_key(h, key);
+            for (const void* ptr : {(const void*)cr.Y, (const void*)amp, (const void*)support2d,
+                                    (const void*)angles, (const void*)support3d, (const void*)vol, (const void*)gst})
+                key.push_back((uint64_t)(uintptr_t)ptr);
+            uint32_t beta_bits;
+            std::memcpy(&beta_bits, &cr.hp.beta, sizeof(beta_bits));
+            for (uint64_t v : {(uint64_t)n, (uint64_t)cr.P, (uint64_t)amp_batch, (uint64_t)beta_bits,
+                               (uint64_t)cr.hp.dc_skip_zero, (uint64_t)cr.inner, (uint64_t)body})
+                key.push_back(v);
             KeptGraph& kg = h->kept[KG_CAROUSEL];
-            try {
-                exec = kept_graph_exec(h, kg, graph);
-            } catch (...) {
+            if (kg.exec && kg.key == key) {
+                exec = kg.exec;
+                // no replay may still be in flight when the next call changes
+                // the body's inputs: queue behind the last one
+                if (kg.done) CPRA_CHECK_HIP(hipStreamWaitEvent(gst, kg.done, 0));
+            } else {
+                h->stream = gst;
+                try {
+                    CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
+                    for (int i = 0; i < body; ++i) cr.outer(nullptr, true);
+                    CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
+                } catch (...) {
+                    h->stream = st;
+                    throw;
+                }
+                h->stream = st;
+                kg.key.clear();
+                try {
+                    exec = kept_graph_exec(h, kg, graph);
+                } catch (...) {
+                    (void)hipGraphDestroy(graph);
+                    throw;
+                }
                 (void)hipGraphDestroy(graph);
-                throw;
+                graph = nullptr;
+                kg.key = std::move(key);
             }
-            (void)hipGraphDestroy(graph);
-            graph = nullptr;
             for (int i = 0; i + body <= rest; i += body) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
             kept_graph_launched(kg, gst);
             h->stream = gst;

@@ -76,9 +76,13 @@ struct EngineGraph;  // engine2d.hip
 // launch sequence) is instantiated anew.  `done` marks the last replay
 // queued, which an update waits for, so that no replay in flight sees its
 // arguments change.
+// `key`, when an engine keeps one, lists everything its captured body
+// depends on (buffers, parameters, workspaces, tuning); a call with the same
+// key replays the kept graph without capturing again.
 struct KeptGraph {
     hipGraphExec_t exec = nullptr;
     hipEvent_t done = nullptr;
+    std::vector<uint64_t> key;
     void release()
     {
         if (done) (void)hipEventSynchronize(done);
@@ -86,6 +90,7 @@ struct KeptGraph {
         if (done) (void)hipEventDestroy(done);
         exec = nullptr;
         done = nullptr;
+        key.clear();
     }
 };
 enum { KG_CAROUSEL = 0, KG_MIXED, KG_DM, KG_DM3D, KG_COUNT };
@@ -153,6 +158,10 @@ inline unsigned grid_1d(uint64_t work, unsigned block, unsigned cap = 1u << 20)
 hipGraphExec_t kept_graph_exec(cpra_hip_ctx* h, KeptGraph& kg, hipGraph_t g);
 // Records kg.done on st after the replays the caller has queued.
 void kept_graph_launched(KeptGraph& kg, hipStream_t st);
+// Appends to `key` the handle state a captured engine body can depend on:
+// every workspace address, the tuning choices, the merge's gather-list
+// geometry and the HIO row mode.
+void engine_state_key(const cpra_hip_ctx* h, std::vector<uint64_t>& key);
 
 // Stream on which an engine captures and replays its iteration graph: the
 // handle's own stream when it has one, so that the replays queue behind the

@@ -827,6 +827,24 @@ hipGraphExec_t kept_graph_exec(cpra_hip_ctx* h, KeptGraph& kg, hipGraph_t g)
     return kg.exec;
 }
 
+void engine_state_key(const cpra_hip_ctx* h, std::vector<uint64_t>& key)
+{
+    for (const Buffer* b : {&h->ws_z, &h->ws_blur, &h->ws_keys, &h->ws_part, &h->ws_part2, &h->ws_misc,
+                            &h->ws_backup, &h->ws_rows, &h->ws_dm, &h->ws_trig, &h->ws_mcount,
+                            &h->ws_mlist, &h->ws_mheavy, &h->ws_state, &h->ws_amp_t})
+        key.push_back((uint64_t)(uintptr_t)b->ptr);
+    for (const auto& kv : h->tuning) {
+        key.push_back(kv.first);
+        key.push_back((uint64_t)(int64_t)kv.second);
+    }
+    key.push_back((uint64_t)h->mplan_n);
+    key.push_back((uint64_t)h->mplan_P);
+    key.push_back((uint64_t)h->mplan_heavy);
+    key.push_back((uint64_t)h->mplan_entries);
+    key.push_back(h->hio_sparse ? 1u : 0u);
+    key.push_back(h->rows_listed ? 1u : 0u);
+}
+
 void kept_graph_launched(KeptGraph& kg, hipStream_t st)
 {
     if (!kg.done) CPRA_CHECK_HIP(hipEventCreateWithFlags(&kg.done, hipEventDisableTiming));

@@ -93,10 +93,13 @@ def test_carousel_row_mode_follows_support(hipmod, oracle, gpu, synth, radius_fr
     assert ex < 2e-5 * outer and ev < 2e-5 * outer
 
 
-def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth):
+@pytest.mark.parametrize("outer", [6, 25])
+def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth, outer):
+    # 25 outer iterations: three eager, two replays of the ten-iteration
+    # body, two eager; 6: too short for a body, all eager
     d = synth.make_carousel(64, 36, config=50)
-    xg, vg, _ = run_gpu(hipmod, d, 6, use_graph=1)
-    xe, ve, _ = run_gpu(hipmod, d, 6, use_graph=0)
+    xg, vg, _ = run_gpu(hipmod, d, outer, use_graph=1)
+    xe, ve, _ = run_gpu(hipmod, d, outer, use_graph=0)
     assert np.array_equal(xg, xe)
     assert np.array_equal(vg, ve)
 

@@ -27,23 +27,35 @@ def test_carousel_graph_reused_across_calls(hipmod, gpu, synth):
     betas = (0.9, 0.75)
     keep, got = [], []
     with hipmod.HipImpl(n, n, 1, P) as h:
+        # 15 outer iterations: three eager, one replay of the ten-iteration
+        # body, two eager
         for k, (d, b) in enumerate(zip(ds, betas)):
             dev = {key: hipmod.to_device(d[key]) for key in ("x0", "amp", "support2d", "angles", "support3d")}
             vol = hipmod.zeros((n, n, n), np.complex64)
             keep.append((dev, vol))
             h.carousel3d_run(dev["x0"], dev["amp"], dev["support2d"], dev["angles"],
-                             dev["support3d"], vol, outer_iters=4, inner_iters=1, beta=b, use_graph=1)
+                             dev["support3d"], vol, outer_iters=15, inner_iters=1, beta=b, use_graph=1)
             h.Sync()
             got.append((dev["x0"].numpy(), vol.numpy()))
             if k == 0:
                 c0 = captures(h)
+        assert captures(h) == c0
+        # the same buffers and parameters again (fresh start data): the kept
+        # graph replays as it is, without a capture
+        dev, vol = keep[1]
+        dev["x0"].upload(ds[1]["x0"])
+        h.carousel3d_run(dev["x0"], dev["amp"], dev["support2d"], dev["angles"],
+                         dev["support3d"], vol, outer_iters=15, inner_iters=1, beta=betas[1], use_graph=1)
+        h.Sync()
+        assert np.array_equal(dev["x0"].numpy(), got[1][0])
+        assert np.array_equal(vol.numpy(), got[1][1])
         assert captures(h) == c0
     for d, b, (x, v) in zip(ds, betas, got):
         dev = {key: hipmod.to_device(d[key]) for key in ("x0", "amp", "support2d", "angles", "support3d")}
         vol = hipmod.zeros((n, n, n), np.complex64)
         with hipmod.HipImpl(n, n, 1, P) as h:
             h.carousel3d_run(dev["x0"], dev["amp"], dev["support2d"], dev["angles"],
-                             dev["support3d"], vol, outer_iters=4, inner_iters=1, beta=b, use_graph=0)
+                             dev["support3d"], vol, outer_iters=15, inner_iters=1, beta=b, use_graph=0)
             h.Sync()
         assert np.array_equal(x, dev["x0"].numpy())
         assert np.array_equal(v, vol.numpy())
