@@ -802,7 +802,12 @@ def main():
     sched.pop("iters")
     h = hip.HipImpl(n, n, 1, P)
 
-    # autotune the kernel variants on this workload (restores x, support)
+    # autotune the kernel variants on this workload (restores x, support);
+    # the pick goes to the autotuner's JSON cache (timed afresh every run)
+    import tempfile
+
+    tune_cache = os.path.join(tempfile.gettempdir(), "cpra_hip_tuning.json")
+    hip.set_tuning_cache(tune_cache, refresh=True)
     if args.tuning:
         h.set_tuning(args.tuning)
         tune = f"fixed: {args.tuning}"
@@ -920,6 +925,7 @@ def main():
         "hbm_frac_algorithmic_21Bpx": round(iter_algo_frac, 4),
         "roofline": roof,
         "autotune": tune,
+        "autotune_cache": tune_cache,
     }
 
     if not args.no_fp64_check:

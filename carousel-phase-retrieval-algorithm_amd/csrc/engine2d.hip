@@ -20,6 +20,7 @@
 // pass (|x| -> row DFT -> row factor of the Gaussian transfer function -> row
 // IDFT, real part), one column pass (the column factor, with the per-pattern
 // maximum), then the threshold -> mask.
+#include <map>
 #include <mutex>
 #include <cmath>
 #include <cstring>
@@ -323,23 +324,23 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
     check_launch("k_eng_cols");
 }
 
-template <int N>
-static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                              const uint8_t* mask, float beta, int rp, int er,
-                              const unsigned* rowbits, const unsigned* order, bool tiled)
+template <int N, int PV>
+static void launch_rows_hio_np(hipStream_t st, const Geo& g, const void* Z, void* Y,
+                               const uint8_t* mask, float beta, int rp, int er,
+                               const unsigned* rowbits, const unsigned* order, bool tiled)
 {
     const int64_t lines = g.B * g.M;
-    constexpr int LPB = HioRowCfg<N>::LPB;
+    constexpr int LPB = HioRowPlan<N, PV>::LPB;
     const unsigned grid = (unsigned)((lines + LPB - 1) / LPB);
     auto* z = static_cast<const c2<float>*>(Z);
     auto* y = static_cast<c2<float>*>(Y);
     const float invN = 1.0f / (float)N;
-    const dim3 blk(HioRowCfg<N>::TPB);
+    const dim3 blk(HioRowPlan<N, PV>::TPB);
     const bool exact = lines % LPB == 0;
     if (tiled) {
         if (g.M != N || N % LPB != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major rows: square patterns only");
 #define CPRA_ROWST(MODE, SP) \
-    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, true, SP, true>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits, order)
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, true, SP, true, PV>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits, order)
         if (er) {
             if (rowbits) CPRA_ROWST(ROW_ER, true); else CPRA_ROWST(ROW_ER, false);
         } else {
@@ -349,7 +350,7 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
         return;
     }
 #define CPRA_ROWS(MODE, EX, SP) \
-    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX, SP>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits, order)
+    hipLaunchKernelGGL((k_eng_rows_hio<N, MODE, EX, SP, false, PV>), dim3(grid), blk, 0, st, z, y, mask, lines, beta, invN, rp, rowbits, order)
 #define CPRA_ROWS2(MODE, SP) \
     do { if (exact) CPRA_ROWS(MODE, true, SP); else CPRA_ROWS(MODE, false, SP); } while (0)
     if (er) {
@@ -361,18 +362,45 @@ static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void*
 #undef CPRA_ROWS
 }
 
+template <int N>
+static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
+                              const uint8_t* mask, float beta, int rp, int er,
+                              const unsigned* rowbits, const unsigned* order, bool tiled, int pv)
+{
+    if constexpr (HasAltRowPlan<N>::value) {
+        if (pv == 1) {
+            launch_rows_hio_np<N, 1>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled);
+            return;
+        }
+    }
+    launch_rows_hio_np<N, 0>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled);
+}
+
+// Tuning key 8: the HIO row kernel's radix-plan variant (HioRowPlan) for
+// pattern width n, 0 where n has no alternative plan.
+static int row_plan(const cpra_hip_ctx* h, int n)
+{
+    auto it = h->tuning.find(((uint64_t)8 << 32) | (uint64_t)n);
+    return it == h->tuning.end() ? 0 : it->second;
+}
+
+static bool has_alt_row_plan(int n)
+{
+    return (n == 256 && HasAltRowPlan<256>::value) || (n == 512 && HasAltRowPlan<512>::value);
+}
+
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
                             const uint8_t* mask, float beta, int rp, int er,
                             const unsigned* rowbits = nullptr, const unsigned* order = nullptr,
-                            bool tiled = false)
+                            bool tiled = false, int pv = 0)
 {
     switch (g.N) {
-    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
-    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
-    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
-    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
-    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
-    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled); break;
+    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
+    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
+    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
+    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
+    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
+    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
     }
     check_launch("k_eng_rows_hio");
 }
@@ -685,7 +713,7 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
         if (kt) kt->end(h, st);
         if (kt) kt->begin(h, st, KT_ROW);
         launch_rows_hio(st, gc, h->ws_z.ptr, xc, mc, p->beta, p->real_positive, p->error_reduction,
-                        bc, order, tiled);
+                        bc, order, tiled, row_plan(h, g.N));
         if (kt) kt->end(h, st);
     }
 }
@@ -727,7 +755,8 @@ void engine_hio_rows(cpra_hip_ctx* h, hipStream_t st, void* Y, const uint8_t* su
 {
     const Geo g = geo_of(h);
     if (kt) kt->begin(h, st, KT_ROW);
-    launch_rows_hio(st, g, h->ws_z.ptr, Y, support, p->beta, p->real_positive, p->error_reduction);
+    launch_rows_hio(st, g, h->ws_z.ptr, Y, support, p->beta, p->real_positive, p->error_reduction,
+                    nullptr, nullptr, false, row_plan(h, g.N));
     if (kt) kt->end(h, st);
 }
 
@@ -849,6 +878,91 @@ void kept_graph_launched(KeptGraph& kg, hipStream_t st)
 {
     if (!kg.done) CPRA_CHECK_HIP(hipEventCreateWithFlags(&kg.done, hipEventDisableTiming));
     CPRA_CHECK_HIP(hipEventRecord(kg.done, st));
+}
+
+// ---- the autotuner's persistent cache ------------------------------------
+// A flat JSON object {"<key>": "<tuning spec>", ...} in the file set with
+// cpra_hip_set_tuning_cache, one entry per (device architecture, workload,
+// pattern size, batch, amplitude batch).  Read on every lookup and rewritten
+// whole (through a temporary file and a rename) on every store, under a
+// process-wide lock; keys and specs hold no quotes or backslashes.
+static std::mutex g_tcache_mu;
+static std::string g_tcache_path;
+static bool g_tcache_refresh = false;
+
+static std::string tuning_cache_key(const cpra_hip_ctx* h, uint64_t amp_batch)
+{
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == '"' || c == '\\') c = '_';
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "%s/hio2d/%llux%llu/b%llu/a%llu", arch.c_str(), (unsigned long long)h->m,
+                  (unsigned long long)h->n, (unsigned long long)h->batch, (unsigned long long)amp_batch);
+    return buf;
+}
+
+static std::map<std::string, std::string> tuning_cache_read(const std::string& path)
+{
+    std::map<std::string, std::string> out;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return out;
+    std::string text;
+    char chunk[4096];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) text.append(chunk, got);
+    std::fclose(f);
+    // "key" : "value" pairs; anything else is skipped
+    size_t i = 0;
+    auto next_string = [&](std::string& s) -> bool {
+        const size_t a = text.find('"', i);
+        if (a == std::string::npos) return false;
+        const size_t b = text.find('"', a + 1);
+        if (b == std::string::npos) return false;
+        s = text.substr(a + 1, b - a - 1);
+        i = b + 1;
+        return true;
+    };
+    std::string k, v;
+    while (next_string(k)) {
+        const size_t colon = text.find(':', i);
+        if (colon == std::string::npos) break;
+        i = colon + 1;
+        if (!next_string(v)) break;
+        out[k] = v;
+    }
+    return out;
+}
+
+static bool tuning_cache_lookup(const cpra_hip_ctx* h, uint64_t amp_batch, std::string& spec)
+{
+    std::lock_guard<std::mutex> lock(g_tcache_mu);
+    if (g_tcache_path.empty() || g_tcache_refresh) return false;
+    const auto m = tuning_cache_read(g_tcache_path);
+    const auto it = m.find(tuning_cache_key(h, amp_batch));
+    if (it == m.end()) return false;
+    spec = it->second;
+    return true;
+}
+
+static void tuning_cache_store(const cpra_hip_ctx* h, uint64_t amp_batch, const std::string& spec)
+{
+    std::lock_guard<std::mutex> lock(g_tcache_mu);
+    if (g_tcache_path.empty()) return;
+    auto m = tuning_cache_read(g_tcache_path);
+    m[tuning_cache_key(h, amp_batch)] = spec;
+    const std::string tmp = g_tcache_path + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) fail(CPRA_HIP_ERR_ARG, "tuning cache: cannot write %s", tmp.c_str());
+    std::fprintf(f, "{\n");
+    size_t k = 0;
+    for (const auto& kv : m)
+        std::fprintf(f, "  \"%s\": \"%s\"%s\n", kv.first.c_str(), kv.second.c_str(), ++k < m.size() ? "," : "");
+    std::fprintf(f, "}\n");
+    const bool ok = std::fclose(f) == 0;
+    if (!ok || std::rename(tmp.c_str(), g_tcache_path.c_str()) != 0)
+        fail(CPRA_HIP_ERR_ARG, "tuning cache: cannot replace %s", g_tcache_path.c_str());
 }
 
 void cpra_engine_release(cpra_hip_ctx* h)
@@ -1387,6 +1501,16 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         CPRA_ARG(iters >= 1 && iters <= 1000, "cpra_hip_autotune_2d: iters out of range");
         CPRA_ARG(!h->capturing, "cpra_hip_autotune_2d: the handle's stream is being captured");
         engine_check_geometry(h, "cpra_hip_autotune_2d");
+        {
+            // a pick of an earlier run for this device, geometry and batch
+            std::string cached;
+            if (tuning_cache_lookup(h, amp_batch, cached)) {
+                const int rc = cpra_hip_set_tuning(h, cached.c_str());
+                if (rc < 0) return rc;
+                if (report && report_len) std::snprintf(report, report_len, "%scached=1;", cached.c_str());
+                return CPRA_HIP_OK;
+            }
+        }
         const Geo g = geo_of(h);
         const int64_t total = (int64_t)g.M * g.N * g.B;
         h->ws_backup.ensure(sizeof(c2<float>) * total + (size_t)total);
@@ -1483,6 +1607,22 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         h->tuning[tkey] = best_lay;
         const int best_v = best_vl[best_lay];
         h->tuning[key] = best_v;
+        // the row kernel's radix plan, where the size has an alternative
+        // (whole iterations, dense rows)
+        int best_rp = 0;
+        if (has_alt_row_plan(g.N)) {
+            const uint64_t rkey = ((uint64_t)8 << 32) | (uint64_t)g.N;
+            double rp_us[2];
+            for (int v = 0; v < 2; ++v) {
+                h->tuning[rkey] = v;
+                rp_us[v] = run_us(iters);
+            }
+            best_rp = rp_us[1] < rp_us[0] ? 1 : 0;
+            h->tuning[rkey] = best_rp;
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), "iter_rowplan0_us=%.2f;iter_rowplan1_us=%.2f;", rp_us[0], rp_us[1]);
+            rep += buf;
+        }
         // rows: dense and sparse on the current support for the report; the
         // choice is the adaptive mode, which picks per shrinkwrap update
         double it_us[3] = {0, 0, 0};
@@ -1516,11 +1656,15 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         h->timing = saved_timing;
         CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
-        char head[128];
-        std::snprintf(head, sizeof(head), "col_%d=%d;rows_sparse=%d;chunk=%d;tiled=%d;col_map=%s;col_xcd=%d;",
-                      g.M, best_v, best_sp, best_chunk, best_lay,
+        char head[192];
+        std::snprintf(head, sizeof(head), "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%d;tiled=%d;",
+                      g.M, best_v, g.N, best_rp, best_sp, best_chunk, best_lay);
+        const std::string spec = head;
+        char desc[96];
+        std::snprintf(desc, sizeof(desc), "col_map=%s;col_xcd=%d;",
                       (best_v & 4) ? "wide" : ((best_v & 1) ? "wave" : "block"), (best_v >> 1) & 1);
-        rep = head + rep;
+        rep = spec + desc + rep;
+        tuning_cache_store(h, amp_batch, spec);
         if (report && report_len) {
             std::snprintf(report, report_len, "%s", rep.c_str());
         }
@@ -1557,6 +1701,11 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
             } else if (std::sscanf(p, "plane_extract=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)7 << 32] = v ? 1 : 0;
                 p += used;
+            } else if (std::sscanf(p, "row_plan_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
+                CPRA_ARG(v == 0 || (v == 1 && has_alt_row_plan(m)),
+                         "cpra_hip_set_tuning: row_plan_%d=%d: no such plan", m, v);
+                h->tuning[((uint64_t)8 << 32) | (uint64_t)m] = v;
+                p += used;
             } else if (std::sscanf(p, "col_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
                 CPRA_ARG(col_variant_valid(v), "cpra_hip_set_tuning: variant %d out of range", v);
                 h->tuning[((uint64_t)1 << 32) | (uint64_t)m] = v;
@@ -1571,17 +1720,26 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
     });
 }
 
+int cpra_hip_set_tuning_cache(const char* path, int refresh)
+{
+    std::lock_guard<std::mutex> lock(g_tcache_mu);
+    g_tcache_path = path ? path : "";
+    g_tcache_refresh = refresh != 0;
+    return CPRA_HIP_OK;
+}
+
 int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
 {
     return eguard(h, [&]() -> int {
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
         std::snprintf(out, out_len,
-                      "col_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;plane_extract=%d;"
-                      "rows_sparse_used=%d;graph_captures=%lld;",
-                      (int)h->m, col_variant_code(cv), sparse_mode(h), (long long)iteration_chunk(h),
-                      tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0, engine_merge_lines(h) ? 1 : 0,
-                      engine_plane_extract(h) ? 1 : 0, rows_sparse_used(h), (long long)h->graph_captures);
+                      "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;"
+                      "plane_extract=%d;rows_sparse_used=%d;graph_captures=%lld;",
+                      (int)h->m, col_variant_code(cv), (int)h->n, row_plan(h, (int)h->n), sparse_mode(h),
+                      (long long)iteration_chunk(h), tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0,
+                      engine_merge_lines(h) ? 1 : 0, engine_plane_extract(h) ? 1 : 0, rows_sparse_used(h),
+                      (long long)h->graph_captures);
         return CPRA_HIP_OK;
     });
 }

@@ -317,15 +317,16 @@ enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
 // finished returns at once (the row exchange never leaves the wave).
 // TILED: Z and Y in the tile-major layout of N x N patterns (Tiled<N>), the
 // mask row-major; a block's LPB lines then lie in one pattern (LPB | N).
-template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false>
-__global__ __launch_bounds__(HioRowCfg<N>::TPB) void k_eng_rows_hio(
+// PV: the radix-plan variant (HioRowPlan).
+template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false, int PV = 0>
+__global__ __launch_bounds__((HioRowPlan<N, PV>::TPB)) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
     int64_t nlines, float beta, float invN, int real_positive,
     const unsigned* __restrict__ rowbits, const unsigned* __restrict__ natural_order = nullptr)
 {
     using V = c2<float>;
-    using PL = typename HioRowCfg<N>::P;
-    constexpr int E = PL::E, TT = PL::T, LPB = HioRowCfg<N>::LPB;
+    using PL = typename HioRowPlan<N, PV>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = HioRowPlan<N, PV>::LPB;
     using TL = Tiled<N, TT>;
     constexpr int SPAN = LdsRow<V, N>::span();
     __shared__ V lds[LPB * SPAN];

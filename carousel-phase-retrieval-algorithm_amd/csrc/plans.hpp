@@ -95,4 +95,29 @@ template <int N> struct HioRowCfg {
     static constexpr int TPB = 256, LPB = TPB / P::T;
 };
 
+// Radix-plan variants of the HIO row kernel that the autotuner times against
+// each other (tuning key row_plan): variant 0 is HioRowCfg; variant 1, where
+// a size has one, keeps 8 values per thread instead of 16 (fewer VGPRs, one
+// more wave per SIMD, two more LDS exchanges per transform), the DM row
+// kernels' plans.  Which one wins depends on the batch and the support, so
+// it is measured rather than fixed.
+template <int N, int PV> struct HioRowPlan : HioRowCfg<N> {};
+template <int N> struct HasAltRowPlan {
+    static constexpr bool value = false;
+};
+template <> struct HioRowPlan<256, 1> {
+    using P = Plan<256, 8, 8, 8, 4>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+template <> struct HasAltRowPlan<256> {
+    static constexpr bool value = true;
+};
+template <> struct HioRowPlan<512, 1> {
+    using P = Plan<512, 8, 8, 8, 8>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+template <> struct HasAltRowPlan<512> {
+    static constexpr bool value = true;
+};
+
 }  // namespace cpra

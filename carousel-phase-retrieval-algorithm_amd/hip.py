@@ -146,6 +146,7 @@ _SIGS = {
     "cpra_hip_autotune_2d": (C.c_int, [_H, _P, _P, _U64, _P, C.c_int, C.c_char_p, _U64]),
     "cpra_hip_set_tuning": (C.c_int, [_H, C.c_char_p]),
     "cpra_hip_get_tuning": (C.c_int, [_H, C.c_char_p, _U64]),
+    "cpra_hip_set_tuning_cache": (C.c_int, [C.c_char_p, C.c_int]),
     "cpra_hip_get_kernel_times": (C.c_int, [_H, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                             C.c_int]),
     "cpra_hip_amp_loss": (C.c_int, [_H, _P, _P, _U64, _P]),
@@ -520,8 +521,8 @@ class HipImpl:
         return self._check(lib().cpra_hip_set_tuning(self._h, spec.encode()))
 
     def get_tuning(self) -> dict:
-        buf = C.create_string_buffer(256)
-        self._check(lib().cpra_hip_get_tuning(self._h, buf, 256))
+        buf = C.create_string_buffer(512)
+        self._check(lib().cpra_hip_get_tuning(self._h, buf, 512))
         return dict(kv.split("=") for kv in buf.value.decode().strip(";").split(";"))
 
     def set_kernel_timing(self, enable: bool) -> bool:
@@ -575,6 +576,13 @@ class HipImpl:
         self._check(fn(self._h, _addr(x), _addr(amp), self._amp_batch(amp, amp_batch, volume),
                        int(iters), float(mu), out.ctypes.data if want_loss else None))
         return out
+
+
+def set_tuning_cache(path, refresh: bool = False) -> None:
+    """The autotuner's persistent JSON cache for this process (None: off)."""
+    rc = lib().cpra_hip_set_tuning_cache(path.encode() if path else None, int(refresh))
+    if rc < 0:
+        raise CpraHipError(rc, lib().cpra_hip_last_error(None).decode())
 
 
 def device_count() -> int:

@@ -120,12 +120,17 @@ int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t*
 /* Autotuner (no reference counterpart): times each column-kernel variant
  * (thread mapping: 16-column blocks, per-wave columns or 32-column blocks of
  * 512 threads, x XCD-aware tile order) for `iters` eager HIO iterations on
- * the caller's data, then dense against sparse row passes (rows without a
- * support pixel finished in the column kernel) and the patterns per launch
- * (whole batch or Infinity-Cache-sized chunks) on whole iterations, keeps
- * the fastest for this handle, restores x and the support, and writes
- * "col_<m>=<v>;rows_sparse=<0|1>;chunk=<c>;...;col_v<k>_us=..;iter_*_us=..;" into
- * report (may be NULL). */
+ * the caller's data, then on whole iterations the state layout, the row
+ * kernel's radix plan where the size has two (256, 512: 16 or 8 values per
+ * thread), dense against sparse row passes (rows without a support pixel
+ * finished in the column kernel) and the patterns per launch (whole batch or
+ * Infinity-Cache-sized chunks); keeps the fastest for this handle, restores x
+ * and the support, and writes "col_<m>=<v>;row_plan_<n>=<v>;rows_sparse=<0|1>;
+ * chunk=<c>;tiled=<0|1>;...;col_v<k>_us=..;iter_*_us=..;" into report (may
+ * be NULL).  With a tuning cache set (cpra_hip_set_tuning_cache), a pick
+ * stored for the same device architecture, pattern size, batch and amplitude
+ * batch is applied instead of timing ("...;cached=1;" in the report), and a
+ * new pick is stored. */
 int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                          uint8_t* support, int iters, char* report, uint64_t report_len);
 /* Re-applies the "col_<m>=<v>;", "rows_sparse=<0|1|2>;", "chunk=<patterns>;"
@@ -140,10 +145,18 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * straight into the merge's [P][v][u] layout; 0 = the DFT in place and a
  * transpose before the merge) and "plane_extract=<0|1>;" (n <= 128: the
  * volume's forward plane transform and the re-projection in one kernel; 0 =
- * two). */
+ * two) and "row_plan_<n>=<0|1>;" (the HIO row kernel's radix plan for
+ * width n; 1 exists for n = 256 and 512). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
+/* The autotuner's persistent cache for the whole process: a JSON object
+ * mapping "<gcn arch>/hio2d/<m>x<n>/b<batch>/a<amp_batch>" to the tuning
+ * spec cpra_hip_set_tuning takes.  path NULL or "" turns it off (the
+ * default); refresh != 0 times every workload again and overwrites its
+ * entry. */
+int cpra_hip_set_tuning_cache(const char* path, int refresh);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
- * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_extract=<0|1>;rows_sparse_used=<0|1>;
+ * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_extract=<0|1>;row_plan_<n>=<0|1>;
+ * rows_sparse_used=<0|1>;
  * graph_captures=<k>;" (rows_sparse_used: the row mode of the last iterations
  * run; in the adaptive mode it is read from the device, which synchronises
  * the handle's stream; graph_captures: how many iteration graphs the engines

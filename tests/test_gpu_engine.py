@@ -405,3 +405,73 @@ def test_tiled_state_rejected_for_non_square(hipmod, gpu, synth):
     with hipmod.HipImpl(64, 128, 1, 2) as h:
         assert h.set_tuning("tiled=1;")
         assert h.get_tuning()["tiled"] == "0"
+
+
+@pytest.mark.parametrize("n,tiled", [(256, 1), (256, 0), (512, 1)])
+@pytest.mark.parametrize("sparse", [1, 0])
+def test_row_plan_variants_match_oracle(hipmod, oracle, gpu, synth, n, tiled, sparse):
+    """The HIO row kernel's second radix plan (row_plan_<n>=1: 8 values per
+    thread, the plan the autotuner times against the default): each step
+    against the FP64 oracle's step from the same state, like the default
+    plan."""
+    P = 2
+    data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, config=90 + n)
+    if sparse:
+        data["support"] = (data["obj"] > 0).astype(np.uint8)
+    x = hipmod.to_device(data["x0"])
+    sup = hipmod.to_device(data["support"])
+    amp = hipmod.to_device(data["amp"])
+    prev = data["x0"]
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        assert h.set_tuning(f"row_plan_{n}=1;tiled={tiled};rows_sparse={sparse};")
+        assert h.get_tuning()[f"row_plan_{n}"] == "1"
+        for step in range(2):
+            h.hio_run(x, amp, sup, iters=1, beta=0.9, real_positive=1, dc_skip_zero=1,
+                      sw_every=0, use_graph=0)
+            h.Sync()
+            got = x.numpy()
+            ref = oracle.hio_step(prev, data["amp"], data["support"], 0.9, True, True)
+            keep = ~boundary_pixels(oracle, prev, data, 1)
+            assert relerr(got[keep], ref[keep]) < 1e-5, (n, step)
+            prev = got
+        with pytest.raises(hipmod.CpraHipError):
+            h.set_tuning("row_plan_192=1;")      # 192 has one plan
+
+
+def test_autotune_row_plan_and_tuning_cache(hipmod, gpu, synth, tmp_path):
+    """The autotuner times the row kernel's two radix plans at 256 and stores
+    its pick in the JSON cache; the next autotune of the same workload on a
+    new handle applies the stored pick without timing; refresh times again."""
+    import json
+
+    n, P = 256, 8
+    data = synth.make_2d_batch(P, n, 30, 50, 1e6, config=131)
+    x = hipmod.to_device(data["x0"])
+    sup = hipmod.to_device(data["support"])
+    amp = hipmod.to_device(data["amp"])
+    path = str(tmp_path / "tuning.json")
+    try:
+        hipmod.set_tuning_cache(path)
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            rep = h.autotune(x, amp, sup, iters=2)
+            fields = dict(kv.split("=") for kv in rep.strip(";").split(";"))
+            assert float(fields["iter_rowplan0_us"]) > 0 and float(fields["iter_rowplan1_us"]) > 0
+            assert fields[f"row_plan_{n}"] in ("0", "1")
+            assert "cached" not in fields
+            first = h.get_tuning()
+        stored = json.load(open(path))
+        (key, spec), = stored.items()
+        assert key.endswith(f"/hio2d/{n}x{n}/b{P}/a{P}") and f"row_plan_{n}=" in spec
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            rep2 = h.autotune(x, amp, sup, iters=2)
+            assert rep2.endswith("cached=1;") and rep2.startswith(spec)
+            second = h.get_tuning()
+        for k in (f"col_{n}", f"row_plan_{n}", "rows_sparse", "chunk", "tiled"):
+            assert first[k] == second[k], k
+        hipmod.set_tuning_cache(path, refresh=True)
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            rep3 = h.autotune(x, amp, sup, iters=2)
+            assert "cached" not in rep3 and "iter_rowplan0_us" in rep3
+        assert np.array_equal(x.numpy(), data["x0"])
+    finally:
+        hipmod.set_tuning_cache(None)
