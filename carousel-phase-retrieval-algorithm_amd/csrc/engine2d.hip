@@ -442,6 +442,20 @@ void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void*
 static void launch_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
 {
     const int64_t total = patterns * n * n;
+    if ((reinterpret_cast<uintptr_t>(in) % 16) == 0 && (reinterpret_cast<uintptr_t>(out) % 16) == 0 &&
+        patterns * (n / kRetileRows) < (1ll << 31)) {
+        const dim3 g4((unsigned)(patterns * (n / kRetileRows)));
+        switch (n) {
+        case 64: hipLaunchKernelGGL(k_amp_retile4<64>, g4, dim3(256), 0, st, in, out); break;
+        case 128: hipLaunchKernelGGL(k_amp_retile4<128>, g4, dim3(256), 0, st, in, out); break;
+        case 192: hipLaunchKernelGGL(k_amp_retile4<192>, g4, dim3(256), 0, st, in, out); break;
+        case 256: hipLaunchKernelGGL(k_amp_retile4<256>, g4, dim3(256), 0, st, in, out); break;
+        case 384: hipLaunchKernelGGL(k_amp_retile4<384>, g4, dim3(256), 0, st, in, out); break;
+        case 512: hipLaunchKernelGGL(k_amp_retile4<512>, g4, dim3(256), 0, st, in, out); break;
+        }
+        check_launch("k_amp_retile4");
+        return;
+    }
     const dim3 grid((unsigned)((total + 255) / 256));
     switch (n) {
     case 64: hipLaunchKernelGGL(k_amp_retile<64>, grid, dim3(256), 0, st, in, out, total); break;

@@ -488,6 +488,28 @@ __global__ __launch_bounds__(256) void k_amp_retile(const float* __restrict__ in
     out[i] = in[b * N * N + rr * N + tile * C + cc];
 }
 
+// The same copy, four columns per thread in row order: block = kRetileRows
+// rows of one pattern, read as contiguous float4 rows and written as float4
+// runs of a tile row (every tile width C is a multiple of 4).  Needs 16-byte
+// aligned amplitudes; the index arithmetic is 32-bit within a pattern, with
+// the divisors compile-time constants.
+constexpr int kRetileRows = 16;
+template <int N>
+__global__ __launch_bounds__(256) void k_amp_retile4(const float* __restrict__ in,
+                                                     float* __restrict__ out)
+{
+    constexpr int C = Tiled<N>::C, Q = N / 4, PER = kRetileRows * Q;
+    static_assert(C % 4 == 0 && N % kRetileRows == 0, "float4 tile rows");
+    const int64_t b = blockIdx.x / (N / kRetileRows);
+    const int r0 = (int)(blockIdx.x % (N / kRetileRows)) * kRetileRows;
+    const float4* src = reinterpret_cast<const float4*>(in + b * N * N) + r0 * Q;
+    float* dst = out + b * N * N;
+    for (int q = threadIdx.x; q < PER; q += 256) {
+        const int r = r0 + q / Q, c = (q % Q) * 4;
+        *reinterpret_cast<float4*>(dst + (c / C) * N * C + r * C + c % C) = src[q];
+    }
+}
+
 // Shrinkwrap blur, separable form.  The centred Gaussian transfer function
 // factors, G(kr, kc) = g_r(kr) g_c(kc) with g(k) = exp(-2 pi^2 sigma^2
 // ((k - n/2)/n)^2), so b = Backward2D(G . Forward2D(|x|)) is a blur of every
