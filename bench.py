@@ -253,7 +253,7 @@ def kernel_entry(ms, cnt, iters, bytes_per_iter, launches_per_iter=1):
             "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
-def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_iters=5, fp64=True):
+def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_iters=10, fp64=True):
     """3D carousel: 180 projections of 128^2 (12 random ellipsoids in 128^3,
     Poisson noise at 1e7 photons), 50 outer iterations of [1 HIO iteration on
     every projection, merge, 3D constraint, re-projection]; graph-captured
@@ -312,17 +312,19 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_ite
         "proj_fft": (px2 * 16 * (K + 1) / K, (K + 1) / K),
         "merge": (px2 * 8 + vx * 8, 1),                # gather r (slices already [P][v][u]), volume w
         # n <= 128: the inverse whole-plane kernel and the z-line kernel (the
-        # forward plane kernel runs fused with the re-projection); above,
-        # five passes
-        "volume": ((2 if n <= 128 else 5) * vx * 16 + vx, 2 if n <= 128 else 5),
-        # n <= 128: forward plane transform (read and write the volume) and
-        # the re-projection from LDS (write the slices); above, the
-        # extraction (read the volume, write the slices)
-        "extract": (px2 * 8 + vx * (16 if n <= 128 else 8), 1),
+        # forward plane kernel runs fused with the re-projection, except in
+        # the run's last iteration, which writes the volume); above, five
+        # passes
+        # (the 3D support is read as bits, packed once per run)
+        "volume": ((2 * vx * 16 + vx / 8 + vx * 16 / K, 2 + 1 / K) if n <= 128 else (5 * vx * 16 + vx / 8, 5)),
+        # read the volume, write the slices: n <= 128 the forward plane
+        # transform in LDS and the re-projection from there (the last
+        # iteration: the extraction from the volume); above, the extraction
+        "extract": (px2 * 8 + vx * 8, 1),
     }
     kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}
     dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
-    algo = px2 * HIO_ALGO_BYTES_PER_PX + (px2 * 8 + vx * 8) * 2 + vx * 17
+    algo = px2 * HIO_ALGO_BYTES_PER_PX + (px2 * 8 + vx * 8) * 2 + vx * 16 + vx // 8
     rec = {
         "metric": "carousel outer iters/s (180 x 128^2 projections, 128^3 volume)",
         "value": round(its, 2), "unit": "iters/s", "steps": outer, "warmup": warmup,

@@ -418,6 +418,7 @@ int cpra_hip_destroy(cpra_hip_handle h)
     h->ws_mcount.release();
     h->ws_mlist.release();
     h->ws_mheavy.release();
+    h->ws_zsup.release();
     if (h->aux_stream) (void)hipStreamDestroy(h->aux_stream);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;

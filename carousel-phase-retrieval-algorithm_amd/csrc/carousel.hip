@@ -40,22 +40,48 @@ namespace cpra {
 // kernel runs ZL_ROUNDTRIP)
 enum ZlMode { ZL_ROUNDTRIP = 0, ZL_INVERSE = 1, ZL_ROUNDTRIP_NO_FORWARD = 2 };
 
+// The 3D support [L][W] packed for the z-line kernel, once per run: for the
+// kernel's block b of C consecutive lines and plane z, C bits (bit c: line
+// b*C + c holds support at z) in C/8 bytes at (b*L + z)*C/8.  The kernel's
+// byte-per-voxel reads of the mask took whole 128-byte lines for C bytes of
+// each (fetch 1.34 times the kernel's algorithmic bytes at 128^3); a block's
+// packed support is L*C/8 contiguous bytes.
+template <int C>
+__global__ __launch_bounds__(256) void k_pack_zline_support(const uint8_t* __restrict__ S, int64_t W, int L,
+                                                            uint8_t* __restrict__ bits)
+{
+    const int64_t total = W / 8 * L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i % (C / 8), bz = i / (C / 8);
+        const int64_t z = bz % L, b = bz / L;
+        const uint8_t* src = S + z * W + b * C + j * 8;
+        unsigned v = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v |= (src[k] != 0 ? 1u : 0u) << k;
+        bits[i] = (uint8_t)v;
+    }
+}
+
 // z-lines of the volume [L][W] (W = M*N points per plane), after the inverse
 // x and y passes: IDFT_z (scale), the real-positive support projection, and
-// (ZL_ROUNDTRIP) DFT_z back.  Block = C consecutive lines.
+// (ZL_ROUNDTRIP) DFT_z back.  Block = C consecutive lines; Sbits: the
+// support packed by k_pack_zline_support<C>.
 template <int L, int MODE>
-__global__ __launch_bounds__(SizeCfg<L>::TPB) void k_zline_constraint(c2<float>* V, const uint8_t* __restrict__ S,
+__global__ __launch_bounds__(SizeCfg<L>::TPB) void k_zline_constraint(c2<float>* V,
+                                                                     const uint8_t* __restrict__ Sbits,
                                                                      int64_t W, float scale)
 {
     using Vt = c2<float>;
     using PL = typename SizeCfg<L>::P;
     constexpr int E = PL::E, TT = PL::T, C = SizeCfg<L>::C;
+    static_assert(C % 8 == 0, "whole bytes of packed support per plane");
     __shared__ Vt lds[L * C];
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
     const int64_t col0 = (int64_t)blockIdx.x * C;
     Vt* Vp = V + col0;
-    const uint8_t* Sp = S + col0;
+    const uint8_t* Sb = Sbits + (size_t)blockIdx.x * (L * C / 8) + (unsigned)t * (C / 8) + (c >> 3);
     const unsigned toff = (unsigned)t * (unsigned)W + (unsigned)c;  // < T*W <= 2^24
     // the volume is in natural order along z; rot() places the centring shift
     auto zbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)W; };
@@ -64,7 +90,7 @@ __global__ __launch_bounds__(SizeCfg<L>::TPB) void k_zline_constraint(c2<float>*
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         v[m] = (Vp + zbase(m))[toff];
-        sm[m] = (Sp + zbase(m))[toff];
+        sm[m] = (uint8_t)((Sb[PL::rot(m) * TT * (C / 8)] >> (c & 7)) & 1);
     });
     LdsCol<Vt, C, false> acc{lds + c};
     fft_line<PL, +1>(v, t, g_tw<L>(), acc);
@@ -203,15 +229,20 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d(c2<float>* V)
 }
 
 // The forward (y, x) plane transform fused with the re-projection of the
-// plane: block (u, angle group g of `groups`) transforms plane z = u in LDS,
-// group 0 writes it back to V, and every group samples its angles' slice rows
-// u from the LDS plane (the bilinear weights and FP64 geometry of
-// k_extract_plane, so the same bits), which the separate extraction would
-// read back from V.  Each group transforms the plane itself: the groups put
-// the CUs that one block per plane leaves idle to work on the sampling.
+// plane: block (u, angle group g of `groups`) transforms plane z = u in LDS
+// and samples its angles' slice rows u from there (the bilinear weights and
+// FP64 geometry of k_extract_plane, so the same bits), which the separate
+// extraction would read back from V.  Each group transforms the plane itself:
+// the groups put the CUs that one block per plane leaves idle to work on the
+// sampling.  V is only read: the next outer iteration's merge overwrites it,
+// and the last outer iteration of a run, whose V is the run's result, runs
+// the forward plane kernel and the separate extraction instead.  (A write of
+// the plane by one group would race with another group of the same plane
+// that has not loaded it yet.)
 template <int N>
 __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d_extract(
-    c2<float>* V, const double* __restrict__ trig, int P, int groups, c2<float>* __restrict__ out)
+    const c2<float>* __restrict__ V, const double* __restrict__ trig, int P, int groups,
+    c2<float>* __restrict__ out)
 {
     using Vt = c2<float>;
     using G = PlaneGeom<N>;
@@ -220,9 +251,7 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d_extract(
     const int u = (int)blockIdx.x / groups, g = (int)blockIdx.x % groups;
     const int p0 = (int)((int64_t)g * P / groups), p1 = (int)((int64_t)(g + 1) * P / groups);
     for (int i = threadIdx.x; i < 2 * (p1 - p0); i += blockDim.x) tl[i] = trig[2 * p0 + i];
-    Vt* Pp = V + (size_t)u * (N * N);
-    plane2d_body<N, -1, false>(Pp, lds);  // ends with a barrier
-    if (g == 0) plane2d_store<N>(Pp, lds);
+    plane2d_body<N, -1, false>(V + (size_t)u * (N * N), lds);  // ends with a barrier
     const double c = (double)(N / 2);
     for (int i = threadIdx.x; i < (p1 - p0) * N; i += blockDim.x) {
         const int pl = i / N, v = i % N;
@@ -249,7 +278,7 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane2d_extract(
 
 // true when the fused kernel covers (n, P): n <= 128 planes in LDS and at
 // most kPlaneExtractMaxAngles angles per group
-static bool plane2d_extract_launch(hipStream_t st, int n, int P, c2<float>* V, const double* trig,
+static bool plane2d_extract_launch(hipStream_t st, int n, int P, const c2<float>* V, const double* trig,
                                    c2<float>* out)
 {
     int groups = 1;
@@ -300,6 +329,35 @@ static void zline_n(hipStream_t st, int mode, c2<float>* V, const uint8_t* S, in
         hipLaunchKernelGGL((k_zline_constraint<L, ZL_INVERSE>), dim3(grid), dim3(SizeCfg<L>::TPB), 0, st, V, S, W, scale);
 }
 
+template <int L>
+static void pack_zline_support_n(hipStream_t st, const uint8_t* S, int64_t W, uint8_t* bits)
+{
+    const int64_t total = W / 8 * L;
+    hipLaunchKernelGGL((k_pack_zline_support<SizeCfg<L>::C>), dim3((unsigned)grid_1d(total, 256, 4096)), dim3(256),
+                       0, st, S, W, L, bits);
+}
+
+// The 3D support S [n][n][n] packed into h->ws_zsup for zline_launch
+// (n^3 / 8 bytes); returns the packed bits.
+static const uint8_t* pack_zline_support(cpra_hip_ctx* h, hipStream_t st, int n, const uint8_t* S)
+{
+    const int64_t W = (int64_t)n * n;
+    h->ws_zsup.ensure((size_t)(W / 8 * n));
+    uint8_t* bits = static_cast<uint8_t*>(h->ws_zsup.ptr);
+    switch (n) {
+    case 64: pack_zline_support_n<64>(st, S, W, bits); break;
+    case 128: pack_zline_support_n<128>(st, S, W, bits); break;
+    case 192: pack_zline_support_n<192>(st, S, W, bits); break;
+    case 256: pack_zline_support_n<256>(st, S, W, bits); break;
+    case 384: pack_zline_support_n<384>(st, S, W, bits); break;
+    case 512: pack_zline_support_n<512>(st, S, W, bits); break;
+    default: fail(CPRA_HIP_ERR_UNSUPPORTED, "carousel: n=%d has no z-line kernel", n);
+    }
+    check_launch("k_pack_zline_support");
+    return bits;
+}
+
+// S: the support packed by pack_zline_support
 static void zline_launch(hipStream_t st, int n, int mode, c2<float>* V, const uint8_t* S)
 {
     const int64_t W = (int64_t)n * n;
@@ -408,8 +466,8 @@ struct Carousel {
     int64_t amp_batch;
     const uint8_t* sup2d;
     const float* angles;
-    const uint8_t* sup3d;
-    c2<float>* vol;  // [n][n][n]
+    const uint8_t* sup3d;  // packed by pack_zline_support
+    c2<float>* vol;        // [n][n][n]
     cpra_hip_hio_params hp;
     int inner;
 
@@ -463,7 +521,10 @@ struct Carousel {
     // One outer iteration on h->stream.  On exit Y holds F, the 2D spectra of
     // the re-projected slices; on entry it holds F too when from_spectrum
     // (every outer iteration but the first), else the row-spectral state.
-    void outer(KernelTimer* kt, bool from_spectrum)
+    // last: the run's last outer iteration, which leaves the constrained 3D
+    // spectrum in vol (the others may leave vol stale: the next merge
+    // overwrites it).
+    void outer(KernelTimer* kt, bool from_spectrum, bool last)
     {
         const int64_t W = (int64_t)n * n;
         if (from_spectrum && inner == 0) to_row_spectral(kt);
@@ -486,8 +547,9 @@ struct Carousel {
         if (kt) kt->end(h, h->stream);
         // the forward plane transform and the re-projection in one kernel
         // where the planes fit in LDS and the gather list's trig table of
-        // these angles is there
-        const bool fused = engine_plane_fft(h) && engine_plane_extract(h) && n <= 128 &&
+        // these angles is there; not in the last iteration, which must leave
+        // the transformed volume in vol
+        const bool fused = !last && engine_plane_fft(h) && engine_plane_extract(h) && n <= 128 &&
                            h->mplan_n == n && h->mplan_P == P && h->ws_trig.ptr;
         if (kt) kt->begin(h, h->stream, KT_VOL);
         volume_constraint(fused ? ZL_ROUNDTRIP_NO_FORWARD : ZL_ROUNDTRIP);
@@ -555,7 +617,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         cr.amp_batch = (int64_t)amp_batch;
         cr.sup2d = support2d;
         cr.angles = angles;
-        cr.sup3d = support3d;
+        cr.sup3d = nullptr;  // packed below
         cr.vol = static_cast<c2<float>*>(vol);
         cpra_hip_hio_default_params(&cr.hp);
         cr.hp.beta = p->beta;
@@ -567,27 +629,29 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         hipStream_t st = h->stream;
         engine_hio_prepare(h, st, support2d);
         if (p->outer_iters > 0) merge_plan_c64(h, angles, n, cr.P);  // the footprint, once per run
+        cr.sup3d = pack_zline_support(h, st, n, support3d);
         // enter the row-spectral state of the projections
         fft_pass_contig(h, x, n, (int64_t)cr.P * n, -1, true, 1.0, false);
         KernelTimer kt;
         KernelTimer* ktp = h->timing ? &kt : nullptr;
+        const int outer_iters = p->outer_iters;
         int done = 0;
-        if (p->outer_iters > 0) {
-            cr.outer(ktp, false);  // eager: sizes every workspace before any capture
+        if (outer_iters > 0) {
+            cr.outer(ktp, false, outer_iters == 1);  // eager: sizes every workspace before any capture
             done = 1;
             // two more eager iterations queue GPU work that covers the host's
             // capture and graph update below
-            for (; done < 3 && p->outer_iters - done >= 12; ++done) cr.outer(ktp, true);
+            for (; done < 3 && outer_iters - done >= 12; ++done) cr.outer(ktp, true, false);
         }
-        const int rest = p->outer_iters - done;
+        const int rest = outer_iters - done;
         // a body of kBody outer iterations: each graph launch costs about
         // 4.5 us of idle GPU at its boundary, a third of an outer iteration's
         // kernel time at 180 x 128^2, so one launch covers several.  The body
         // has the same length in every call, so the graph kept by the handle
-        // is re-pointed, not rebuilt; shorter runs and the remainder run
-        // eagerly
+        // is re-pointed, not rebuilt; shorter runs and the remainder, which
+        // holds at least the last iteration, run eagerly
         constexpr int kBody = 10;
-        if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest >= kBody) {
+        if (p->use_graph && !h->timing && !h->capturing && !debug_launches() && rest > kBody) {
             hipStream_t gst = graph_stream(h);
             hipEvent_t ev0 = nullptr, ev1 = nullptr;
             if (gst != st) {
@@ -624,7 +688,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
                 h->stream = gst;
                 try {
                     CPRA_CHECK_HIP(hipStreamBeginCapture(gst, hipStreamCaptureModeThreadLocal));
-                    for (int i = 0; i < body; ++i) cr.outer(nullptr, true);
+                    for (int i = 0; i < body; ++i) cr.outer(nullptr, true, false);
                     CPRA_CHECK_HIP(hipStreamEndCapture(gst, &graph));
                 } catch (...) {
                     h->stream = st;
@@ -642,11 +706,12 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
                 graph = nullptr;
                 kg.key = std::move(key);
             }
-            for (int i = 0; i + body <= rest; i += body) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
+            const int replays = (rest - 1) / body;  // the last iteration stays outside
+            for (int i = 0; i < replays; ++i) CPRA_CHECK_HIP(hipGraphLaunch(exec, gst));
             kept_graph_launched(kg, gst);
             h->stream = gst;
             try {
-                for (int i = rest - rest % body; i < rest; ++i) cr.outer(nullptr, true);
+                for (int i = replays * body; i < rest; ++i) cr.outer(nullptr, true, i == rest - 1);
             } catch (...) {
                 h->stream = st;
                 throw;
@@ -660,7 +725,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
                 (void)hipEventDestroy(ev1);
             }
         } else {
-            for (int i = 0; i < rest; ++i) cr.outer(ktp, true);
+            for (int i = 0; i < rest; ++i) cr.outer(ktp, true, i == rest - 1);
         }
         if (p->outer_iters > 0) cr.to_row_spectral(ktp);  // F -> the row-spectral state
         // leave: x = IDFT_rows(Y) / n; vol = the constrained real-space object

@@ -109,6 +109,7 @@ struct cpra_hip_ctx {
     std::map<uint64_t, void*> twiddles;
     // engine workspaces
     cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm, ws_trig, ws_mcount, ws_mlist, ws_mheavy;
+    cpra::Buffer ws_zsup;  // the carousel's 3D support packed for the z-line kernel
     // tile-major HIO state and amplitudes (engine2d.hip)
     cpra::Buffer ws_state, ws_amp_t;
     // gather list of the carousel merge (merge3d.hip)

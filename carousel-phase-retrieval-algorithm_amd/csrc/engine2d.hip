@@ -874,7 +874,7 @@ void engine_state_key(const cpra_hip_ctx* h, std::vector<uint64_t>& key)
 {
     for (const Buffer* b : {&h->ws_z, &h->ws_blur, &h->ws_keys, &h->ws_part, &h->ws_part2, &h->ws_misc,
                             &h->ws_backup, &h->ws_rows, &h->ws_dm, &h->ws_trig, &h->ws_mcount,
-                            &h->ws_mlist, &h->ws_mheavy, &h->ws_state, &h->ws_amp_t})
+                            &h->ws_mlist, &h->ws_mheavy, &h->ws_state, &h->ws_amp_t, &h->ws_zsup})
         key.push_back((uint64_t)(uintptr_t)b->ptr);
     for (const auto& kv : h->tuning) {
         key.push_back(kv.first);

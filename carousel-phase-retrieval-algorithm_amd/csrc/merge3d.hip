@@ -20,6 +20,7 @@
 // to [p][v][z] so the reads coalesce).  A block owns XT consecutive columns at
 // one y and stages its [n][XT] output tile in LDS so the volume rows are
 // written contiguously.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -434,16 +435,76 @@ __global__ __launch_bounds__(256) void k_merge_fill(const double* __restrict__ t
 // Columns near the rotation axis are covered by every angle (at 180 x 128^2:
 // up to 536 list entries against a mean of 5.5).  Columns with more than
 // kMergeHeavy entries get a block of their own; the rest run one wave per
-// column with four entries' loads in flight.
+// column.
 constexpr int kMergeHeavy = 32;
 constexpr int kMergeXT = 16;     // light columns per block (one wave each)
 constexpr int kHeavyWaves = 16;  // waves per heavy column
 static_assert(kMergeXT == kHeavyWaves, "both block kinds use 1024 threads and n * 16 elements of LDS");
 
-// Light block: 16 consecutive x at one y, one wave per column (a short chain
-// of dependent loads per wave), lanes over z; the columns are staged in LDS
-// and stored z-major.  Heavy columns are left to their own blocks.
-template <typename E, typename T>
+__device__ __forceinline__ float lane_bcast(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// acc[k] (lanes over z = lane + 64 k, KZ values per lane) += the cnt list
+// entries e0, e0 + stride, ..., in list order.  Lane l loads entry l of each
+// run of 64 (one vector load of indices and one of weights) and readlane
+// hands them out, so the column loads of a round of R entries leave back to
+// back: a list of c entries costs one index load and c / R column rounds,
+// where loading the indices round by round put a dependent scalar load in
+// front of every round.  The last round loads and adds only its live
+// entries (a wave-uniform condition).  R: entries per round (R * KZ column
+// values in flight per lane).
+template <int KZ, int R, typename E, typename T>
+__device__ __forceinline__ void merge_sum(const E* __restrict__ sT, const int* __restrict__ lidx,
+                                          const T* __restrict__ lw, int e0, int cnt, int stride, int n,
+                                          int lane, E (&acc)[KZ])
+{
+    using Ops = ElemOps<E>;
+    for (int b0 = 0; b0 < cnt; b0 += 64) {
+        const int m = min(64, cnt - b0);
+        int li = 0;
+        T lwv = (T)0;
+        if (lane < m) {
+            li = lidx[e0 + (b0 + lane) * stride];
+            lwv = lw[e0 + (b0 + lane) * stride];
+        }
+        for (int j = 0; j < m; j += R) {
+            const int live = m - j;
+            const E* cp[R];
+            T w[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int jr = min(j + r, m - 1);
+                cp[r] = sT + (int64_t)__builtin_amdgcn_readlane(li, jr) * n;
+                w[r] = lane_bcast(lwv, jr);
+            }
+#pragma unroll
+            for (int k = 0; k < KZ; ++k) {
+                const int z = lane + 64 * k;
+                if (z < n) {
+                    E v[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (r < live) v[r] = cp[r][z];
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (r < live) acc[k] = Ops::fma(w[r], v[r], acc[k]);
+                }
+            }
+        }
+    }
+}
+
+// Entries per round of merge_sum.  Measured at 180 x 128^2 (KZ = 2), light
+// and heavy columns alike: 8 per round 19.6-20.8 us per merge, 4 per round
+// 20.4-20.7, 16 per round 27.5.
+template <int KZ> constexpr int kMergeRound = KZ <= 2 ? 8 : 4;
+
+// Light block: 16 consecutive x at one y, one wave per column, lanes over z;
+// the columns are staged in LDS and stored z-major.  Heavy columns are left
+// to their own blocks.
+template <int KZ, typename E, typename T>
 __device__ __forceinline__ void merge_light_block(unsigned t_id, const E* __restrict__ sT,
                                                   const int* __restrict__ offsets,
                                                   const int* __restrict__ lidx, const T* __restrict__ lw,
@@ -458,48 +519,17 @@ __device__ __forceinline__ void merge_light_block(unsigned t_id, const E* __rest
     {
         const int xl = wave;
         const int col = y * n + x0 + xl;
-        int e0 = offsets[col];
+        const int e0 = offsets[col];
         const int e1 = offsets[col + 1];
         const bool heavy = e1 - e0 > kMergeHeavy;
         if (lane == 0) skip[xl] = heavy ? 1 : 0;
-        E acc[kMergeMaxZPerLane];
+        E acc[KZ];
 #pragma unroll
-        for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
-        if (!heavy) {
-            // four entries per round: their loads are independent
-            for (; e0 + 4 <= e1; e0 += 4) {
-                const E* cp[4];
-                T w[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    cp[j] = sT + (int64_t)lidx[e0 + j] * n;
-                    w[j] = lw[e0 + j];
-                }
-#pragma unroll
-                for (int k = 0; k < kMergeMaxZPerLane; ++k) {
-                    const int z = lane + 64 * k;
-                    if (z < n) {
-                        E v[4];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) v[j] = cp[j][z];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) acc[k] = Ops::fma(w[j], v[j], acc[k]);
-                    }
-                }
-            }
-            for (; e0 < e1; ++e0) {
-                const E* cp = sT + (int64_t)lidx[e0] * n;
-                const T w = lw[e0];
-#pragma unroll
-                for (int k = 0; k < kMergeMaxZPerLane; ++k) {
-                    const int z = lane + 64 * k;
-                    if (z < n) acc[k] = Ops::fma(w, cp[z], acc[k]);
-                }
-            }
-        }
+        for (int k = 0; k < KZ; ++k) acc[k] = Ops::zero();
+        if (!heavy) merge_sum<KZ, kMergeRound<KZ>>(sT, lidx, lw, e0, e1 - e0, 1, n, lane, acc);
         const T s = inv[col];
 #pragma unroll
-        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+        for (int k = 0; k < KZ; ++k) {
             const int z = lane + 64 * k;
             if (z < n) tile[z * XT + xl] = Ops::scale(acc[k], s);
         }
@@ -512,9 +542,9 @@ __device__ __forceinline__ void merge_light_block(unsigned t_id, const E* __rest
 }
 
 // Heavy block: 16 waves on one column; wave w sums the entries e0 + w,
-// e0 + w + 16, ... (four in flight), then the 16 partial columns are added in
-// LDS in a fixed order.
-template <typename E, typename T>
+// e0 + w + 16, ..., then the 16 partial columns are added in LDS in a fixed
+// order.
+template <int KZ, typename E, typename T>
 __device__ __forceinline__ void merge_heavy_block(int col, const E* __restrict__ sT,
                                                   const int* __restrict__ offsets,
                                                   const int* __restrict__ lidx, const T* __restrict__ lw,
@@ -525,41 +555,13 @@ __device__ __forceinline__ void merge_heavy_block(int col, const E* __restrict__
     constexpr int NW = kHeavyWaves;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int e0 = offsets[col], e1 = offsets[col + 1];
-    E acc[kMergeMaxZPerLane];
+    E acc[KZ];
 #pragma unroll
-    for (int k = 0; k < kMergeMaxZPerLane; ++k) acc[k] = Ops::zero();
-    int e = e0 + wave;
-    for (; e + 3 * NW < e1; e += 4 * NW) {
-        const E* cp[4];
-        T w[4];
+    for (int k = 0; k < KZ; ++k) acc[k] = Ops::zero();
+    const int cnt = e1 - e0 > wave ? (e1 - e0 - wave + NW - 1) / NW : 0;
+    merge_sum<KZ, kMergeRound<KZ>>(sT, lidx, lw, e0 + wave, cnt, NW, n, lane, acc);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            cp[j] = sT + (int64_t)lidx[e + j * NW] * n;
-            w[j] = lw[e + j * NW];
-        }
-#pragma unroll
-        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
-            const int z = lane + 64 * k;
-            if (z < n) {
-                E v[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) v[j] = cp[j][z];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[k] = Ops::fma(w[j], v[j], acc[k]);
-            }
-        }
-    }
-    for (; e < e1; e += NW) {
-        const E* c0 = sT + (int64_t)lidx[e] * n;
-        const T w0 = lw[e];
-#pragma unroll
-        for (int k = 0; k < kMergeMaxZPerLane; ++k) {
-            const int z = lane + 64 * k;
-            if (z < n) acc[k] = Ops::fma(w0, c0[z], acc[k]);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < kMergeMaxZPerLane; ++k) {
+    for (int k = 0; k < KZ; ++k) {
         const int z = lane + 64 * k;
         if (z < n) part[wave * n + z] = acc[k];
     }
@@ -575,30 +577,58 @@ __device__ __forceinline__ void merge_heavy_block(int col, const E* __restrict__
 }
 
 // One launch for the whole merge: the first heavy_blocks blocks (nheavy
-// rounded up to a multiple of 8) take the heavy columns, longest chains
+// rounded up to a multiple of 8) take the heavy columns, longest lists
 // first, so they overlap the light blocks; the light blocks follow, n/16 * n
 // of them (a multiple of 8).  XCD-aware light order: block b runs on XCD
-// b mod 8, and XCD x takes the contiguous tile range [x*G/8, (x+1)*G/8), a
-// band of y, so voxel columns that read the same slice columns share that
-// XCD's L2.  Every voxel is written by exactly one block.
-template <typename E, typename T>
+// b mod 8, and XCD x takes the rows of y in bands of `band` rows, the bands
+// dealt to the XCDs in turn (rows [(8k + x) band, (8k + x + 1) band)), so
+// voxel columns that read the same slice columns mostly share that XCD's L2.
+// Every slice passes through the rotation axis, so the list lengths fall off
+// as 1/r away from it: with one contiguous band of n/8 rows per XCD (band =
+// n/8), the XCDs holding the middle rows had 1.5 times the mean work at
+// 180 x 128^2; with 4-row bands every XCD is within 5 % of the mean.  Every
+// voxel is written by exactly one block.  KZ: z values per lane, at least
+// n / 64.
+template <typename E, typename T, int KZ>
 __global__ __launch_bounds__(1024) void k_merge_apply(const E* __restrict__ sT, const int* __restrict__ offsets,
                                                      const int* __restrict__ lidx, const T* __restrict__ lw,
                                                      const T* __restrict__ inv, const int* __restrict__ heavy,
-                                                     int nheavy, int heavy_blocks, int n, E* __restrict__ vol)
+                                                     int nheavy, int heavy_blocks, int n, int band,
+                                                     E* __restrict__ vol)
 {
     extern __shared__ unsigned char smem[];
     __shared__ int skip[kMergeXT];
     E* buf = reinterpret_cast<E*>(smem);  // [n][16] light tile or [16][n] heavy partials
     const int b = (int)blockIdx.x;
     if (b < heavy_blocks) {
-        if (b < nheavy) merge_heavy_block<E, T>(heavy[b], sT, offsets, lidx, lw, inv, n, vol, buf);
+        if (b < nheavy) merge_heavy_block<KZ, E, T>(heavy[b], sT, offsets, lidx, lw, inv, n, vol, buf);
         return;  // block-uniform
     }
     const unsigned lb = (unsigned)(b - heavy_blocks);
-    const unsigned G = gridDim.x - (unsigned)heavy_blocks;
-    const unsigned t_id = (lb & 7u) * (G >> 3) + (lb >> 3);
-    merge_light_block<E, T>(t_id, sT, offsets, lidx, lw, inv, n, vol, buf, skip);
+    const unsigned x = lb & 7u, i = lb >> 3;  // XCD, and its i-th tile
+    const unsigned tx = (unsigned)n / kMergeXT;   // tiles per row
+    const unsigned bt = (unsigned)band * tx;      // tiles per band
+    const unsigned row = ((i / bt) * 8u + x) * (unsigned)band + (i % bt) / tx;
+    merge_light_block<KZ, E, T>(row * tx + i % tx, sT, offsets, lidx, lw, inv, n, vol, buf, skip);
+}
+
+// z values per lane of k_merge_apply for size n (a power of two >= n / 64)
+static int merge_kz(int n)
+{
+    const int need = (n + 63) / 64;
+    return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
+}
+
+template <int KZ>
+static void merge_apply_launch(hipStream_t st, const c2<float>* sT, const int* offsets, const int* lidx,
+                               const float* lw, const float* inv, const int* heavy, int nheavy, int heavy_blocks,
+                               int n, void* vol)
+{
+    // 4-row bands where n / 8 is a multiple of 4, else one band per XCD
+    const int band = (n / 8) % 4 == 0 ? 4 : n / 8;
+    hipLaunchKernelGGL((k_merge_apply<c2<float>, float, KZ>), dim3(heavy_blocks + (n / 16) * n), dim3(1024),
+                       sizeof(c2<float>) * (size_t)n * kMergeXT, st, sT, offsets, lidx, lw, inv, heavy, nheavy,
+                       heavy_blocks, n, band, static_cast<c2<float>*>(vol));
 }
 
 // Builds the gather list of (angles, n, P) in the handle (synchronises once).
@@ -633,6 +663,9 @@ void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
     std::vector<int> heavy;
     for (int i = 0; i < cols; ++i)
         if (off[i + 1] - off[i] > kMergeHeavy) heavy.push_back(i);
+    // longest lists first: the blocks that finish last start first
+    std::stable_sort(heavy.begin(), heavy.end(),
+                     [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
     h->ws_mheavy.ensure(sizeof(int) * (heavy.empty() ? 1 : heavy.size()));
     if (!heavy.empty())
         CPRA_CHECK_HIP(hipMemcpyAsync(h->ws_mheavy.ptr, heavy.data(), sizeof(int) * heavy.size(),
@@ -656,9 +689,10 @@ void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P)
     // of the kernel for the whole process, so it is set to the largest size
     // the code launches, not to this handle's n: a handle of another size
     // must not lower it under one that is live.
-    CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_merge_apply<c2<float>, float>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(sizeof(c2<float>) * (size_t)kMergeMaxN * kMergeXT)));
+    for (const void* f : {(const void*)k_merge_apply<c2<float>, float, 1>, (const void*)k_merge_apply<c2<float>, float, 2>,
+                          (const void*)k_merge_apply<c2<float>, float, 4>, (const void*)k_merge_apply<c2<float>, float, 8>})
+        CPRA_CHECK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(sizeof(c2<float>) * (size_t)kMergeMaxN * kMergeXT)));
     CPRA_CHECK_HIP(hipFuncSetAttribute((const void*)k_extract_plane<c2<float>, float>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)(kExtractTrigLds + kExtractPlaneLds)));
@@ -690,13 +724,17 @@ void merge_apply_lines_c64(cpra_hip_ctx* h, const void* sT_, void* vol, int n, i
     // n / 16 * n light tiles: a multiple of 8 for every n that is a multiple of 16
     const int nheavy = h->mplan_heavy;
     const int heavy_blocks = (nheavy + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_merge_apply<E, float>), dim3(heavy_blocks + (n / 16) * n), dim3(1024),
-                       sizeof(E) * (size_t)n * kMergeXT, h->stream, sT,
-                       static_cast<const int*>(h->ws_mcount.ptr), reinterpret_cast<const int*>(base),
-                       reinterpret_cast<const float*>(base + idx_bytes),
-                       reinterpret_cast<const float*>(base + idx_bytes + w_bytes),
-                       static_cast<const int*>(h->ws_mheavy.ptr), nheavy, heavy_blocks, n,
-                       static_cast<E*>(vol));
+    const int* offsets = static_cast<const int*>(h->ws_mcount.ptr);
+    const int* lidx = reinterpret_cast<const int*>(base);
+    const float* lw = reinterpret_cast<const float*>(base + idx_bytes);
+    const float* inv = reinterpret_cast<const float*>(base + idx_bytes + w_bytes);
+    const int* heavy = static_cast<const int*>(h->ws_mheavy.ptr);
+    switch (merge_kz(n)) {
+    case 1: merge_apply_launch<1>(h->stream, sT, offsets, lidx, lw, inv, heavy, nheavy, heavy_blocks, n, vol); break;
+    case 2: merge_apply_launch<2>(h->stream, sT, offsets, lidx, lw, inv, heavy, nheavy, heavy_blocks, n, vol); break;
+    case 4: merge_apply_launch<4>(h->stream, sT, offsets, lidx, lw, inv, heavy, nheavy, heavy_blocks, n, vol); break;
+    default: merge_apply_launch<8>(h->stream, sT, offsets, lidx, lw, inv, heavy, nheavy, heavy_blocks, n, vol); break;
+    }
     check_launch("k_merge_apply");
 }
 
