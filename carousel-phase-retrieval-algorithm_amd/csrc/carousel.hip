@@ -666,8 +666,8 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
             // what the body's launches depend on: when a call repeats the
             // kept graph's key, the graph replays as it is, with no capture
             // and no update (their host time, about half a millisecond for
-            // the 100 launches of the body at 180 x 128^2, would otherwise
-            // leave the GPU idle)
+            // a body of 100 launches at 180 x 128^2, would otherwise leave
+            // the GPU idle)
             std::vector<uint64_t> key;
             engine_state_key(h, key);
             for (const void* ptr : {(const void*)cr.Y, (const void*)amp, (const void*)support2d,

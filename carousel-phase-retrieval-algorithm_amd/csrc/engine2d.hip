@@ -159,7 +159,7 @@ struct EngineGraph {
     uint64_t amp_batch = 0;
     int iters = 0;
     float beta = 0.f;
-    int rp = 0, skip = 0, er = 0, col = -1;
+    int rp = 0, skip = 0, er = 0, col = -1, pv = -1;
     bool tiled = false;
     void release()
     {
@@ -1064,6 +1064,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         }
         bool sparse = decide_sparse(h, st, support);
         const int colv = col_variant_code(col_variant(h, g.M));
+        const int pv = row_plan(h, g.N);  // the row kernels' radix plan, part of the graph's key
         const int chunk = p->sw_every > 0 ? p->sw_every : (p->iters > 0 ? p->iters : 1);
         int done = 0;
         bool use_graph = p->use_graph && !h->timing && !h->capturing && p->iters >= chunk && chunk > 0;
@@ -1089,7 +1090,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                               eg->amp_batch == amp_batch && eg->iters == chunk &&
                               eg->beta == p->beta && eg->rp == p->real_positive &&
                               eg->skip == p->dc_skip_zero && eg->er == p->error_reduction &&
-                              eg->col == colv && eg->tiled == tl;
+                              eg->col == colv && eg->pv == pv && eg->tiled == tl;
             if (!same) {
                 if (!eg) eg = new EngineGraph();
                 eg->release();
@@ -1101,6 +1102,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
                 ++h->graph_captures;
                 eg->er = p->error_reduction;
                 eg->col = colv;
+                eg->pv = pv;
                 eg->tiled = tl;
                 eg->x = Y;
                 eg->amp = A;

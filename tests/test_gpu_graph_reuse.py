@@ -157,3 +157,37 @@ def test_dm3d_graph_reused_across_calls(hipmod, gpu):
             h.Sync()
         for a, b in zip(g, (psi, t1, t2)):
             assert np.array_equal(a, b.numpy())
+
+
+def test_hio_chunk_graph_follows_the_row_plan(hipmod, gpu, synth):
+    """The HIO chunk graph kept by a handle is keyed on the row kernels'
+    radix plan as well: after a change of plan on the same buffers, the run
+    must give the new plan's bits (those of an eager run on a fresh handle
+    with that plan), not replay the old plan's graph."""
+    n, B = 256, 2
+    d = synth.make_2d_batch(B, n, 0.12 * n, 0.18 * n, 1e6, config=31)
+    sched = dict(synth.HIO_SCHEDULE)
+    sched.pop("iters")
+    sched["sw_every"] = 5
+    x = hipmod.to_device(d["x0"])
+    amp = hipmod.to_device(d["amp"])
+    sup = hipmod.to_device(d["support"])
+    got = {}
+    with hipmod.HipImpl(n, n, 1, B) as h:
+        for pv in (0, 1, 0):
+            h.set_tuning(f"row_plan_{n}={pv};")
+            x.upload(d["x0"])
+            sup.upload(d["support"])
+            h.hio_run(x, amp, sup, iters=10, use_graph=1, **sched)
+            h.Sync()
+            got.setdefault(pv, []).append(x.numpy())
+    assert np.array_equal(got[0][0], got[0][1])
+    for pv in (0, 1):
+        xe = hipmod.to_device(d["x0"])
+        se = hipmod.to_device(d["support"])
+        with hipmod.HipImpl(n, n, 1, B) as h:
+            h.set_tuning(f"row_plan_{n}={pv};")
+            h.hio_run(xe, amp, se, iters=10, use_graph=0, **sched)
+            h.Sync()
+        assert np.array_equal(got[pv][0], xe.numpy()), pv
+    assert not np.array_equal(got[0][0], got[1][0])
