@@ -774,8 +774,11 @@ def bench_mixed(hip, synth, ge, sizes=(128, 192, 256, 384), per_class=256, iters
     kt = h.kernel_times()
     h.set_kernel_timing(False)
     px = per_class * sum(n * n for n in sizes)
-    kern = {"k_mixed_cols": kernel_entry(kt["column"][0], kt["column"][1], timing_iters, px * 40),
-            "k_mixed_dm_rows": kernel_entry(kt["row"][0], kt["row"][1], timing_iters, px * 49)}
+    # two-pass textbook map: the column pass reads Y1, Y2 (16) and A (4) and
+    # writes V, W (16); the row pass reads V, W (16) and the support (1) and
+    # writes Y1, Y2 (16)
+    kern = {"k_mixed_cols": kernel_entry(kt["column"][0], kt["column"][1], timing_iters, px * 36),
+            "k_mixed_dm_rows": kernel_entry(kt["row"][0], kt["row"][1], timing_iters, px * 33)}
     dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
     rec = {
         "metric": "mixed-size DM iters/s (256 each of 128^2, 192^2, 256^2, 384^2)",

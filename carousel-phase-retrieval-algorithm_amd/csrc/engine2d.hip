@@ -829,12 +829,66 @@ static void dm_rows_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p,
     case 512: { constexpr int NN = 512; CALL; } break;   \
     }
 
+// The two-pass textbook map on 2D patterns (k_dmt_cols / k_dmt_rows):
+// column pass [Y1; Y2] -> [V; W] over the batch's B patterns, one block per
+// column tile of a pattern for both halves.
+template <int M>
+static void dmt_cols_n(hipStream_t st, const Geo& g, const c2<float>* Y, c2<float>* VW, const float* amp,
+                       int64_t amp_batch, float scale, int skip, float beta)
+{
+    using G = DmtColCfg<M>;
+    const unsigned grid = (unsigned)(g.B * g.N / G::C);
+    const size_t lds = sizeof(c2<float>) * (size_t)M * G::C;
+    hipLaunchKernelGGL((k_dmt_cols<M>), dim3(grid), dim3(G::TPB), lds, st, Y, VW, amp, amp_batch,
+                       g.N, scale, skip, beta, (int64_t)g.M * g.N * g.B);
+}
+
+// column tile width of k_dmt_cols for M-point columns
+static int dmt_col_tile(int M)
+{
+    int c = 0;
+    CPRA_SIZE_SWITCH(M, (c = DmtColCfg<NN>::C));
+    return c;
+}
+
+template <int N>
+static void dmt_rows_n(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p, bool last,
+                       const c2<float>* VW, c2<float>* psi, const uint8_t* mask, c2<float>* Y)
+{
+    const int64_t lines = g.B * g.M;
+    const int64_t half = (int64_t)g.M * g.N * g.B;
+    constexpr int LPB = DmtRowCfg<N>::LPB;
+    const dim3 grid((unsigned)((lines + LPB - 1) / LPB)), blk(DmtRowCfg<N>::TPB);
+#define CPRA_DMT(SP, LA) hipLaunchKernelGGL((k_dmt_rows<N, SP, LA>), grid, blk, 0, st, VW, psi, mask, Y, lines, p->beta, half)
+#define CPRA_DMT2(SP) do { if (last) CPRA_DMT(SP, true); else CPRA_DMT(SP, false); } while (0)
+    if (p->space_mode == 0) CPRA_DMT2(SPACE_MKL);
+    else if (p->space_mode == 1) CPRA_DMT2(SPACE_POS);
+    else CPRA_DMT2(SPACE_CPLX);
+#undef CPRA_DMT2
+#undef CPRA_DMT
+}
+
+
 static void dm_entry_launch(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p,
                             const c2<float>* psi, const c2<float>* t1, const c2<float>* t2,
                             const uint8_t* mask, c2<float>* Y1, c2<float>* Y2)
 {
     CPRA_SIZE_SWITCH(g.N, (dm_entry_n<NN>(st, g, p, psi, t1, t2, mask, Y1, Y2)));
     check_launch("k_dm_entry");
+}
+
+static void dmt_cols_launch(hipStream_t st, const Geo& g, const c2<float>* Y, c2<float>* VW,
+                            const float* amp, int64_t amp_batch, float scale, int skip, float beta)
+{
+    CPRA_SIZE_SWITCH(g.M, (dmt_cols_n<NN>(st, g, Y, VW, amp, amp_batch, scale, skip, beta)));
+    check_launch("k_dmt_cols");
+}
+
+static void dmt_rows_launch(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p, bool last,
+                            const c2<float>* VW, c2<float>* psi, const uint8_t* mask, c2<float>* Y)
+{
+    CPRA_SIZE_SWITCH(g.N, (dmt_rows_n<NN>(st, g, p, last, VW, psi, mask, Y)));
+    check_launch("k_dmt_rows");
 }
 
 static void dm_rows_launch(hipStream_t st, const Geo& g, const cpra_hip_dm_params* p, bool last,
@@ -1291,7 +1345,14 @@ static void dm_run_body(cpra_hip_ctx* h, void* psi, void* t1, void* t2, const fl
                     static_cast<const c2<float>*>(t2), support, Y, Y + total);
     const Geo g2{g.M, g.N, 2 * g.B};
     const float scale = (float)(1.0 / ((double)g.M * g.N));
+    const bool two_pass = p->textbook && g.N % dmt_col_tile(g.M) == 0;
     auto one = [&](hipStream_t s, bool last) {
+        if (two_pass) {
+            // two-pass form of the textbook map: [Y1; Y2] -> [V; W] in Z
+            dmt_cols_launch(s, g, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, p->beta);
+            dmt_rows_launch(s, g, p, last, Z, static_cast<c2<float>*>(psi), support, Y);
+            return;
+        }
         launch_cols(h, s, g2, COL_MOD, Y, Z, amp, (int64_t)amp_batch, scale, p->dc_skip_zero, nullptr);
         dm_rows_launch(s, g, p, last, Z, Z + total, static_cast<c2<float>*>(psi), support, Y,
                        Y + total, static_cast<c2<float>*>(t1), static_cast<c2<float>*>(t2));

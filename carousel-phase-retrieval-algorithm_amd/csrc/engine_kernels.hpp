@@ -897,3 +897,183 @@ __global__ __launch_bounds__(DmRowCfg<N>::TPB) void k_dm_rows(
 }
 
 }  // namespace cpra
+
+namespace cpra {
+
+// ---------------------------------------------------------------------------
+// Elser's textbook difference map on 2D patterns in two passes of 36 and 33
+// bytes per pixel (against 40 + 49 for the body above, which also serves
+// the reference variant and the 3D map).
+//
+// With Y1 = DFT_r(psi) and Y2 = DFT_r(f_S psi), f_S psi = psi/beta +
+// (1 - 1/beta) P_S psi, the column pass turns Y into Z with IDFT_r(Z) = P_M(.)
+// in pattern space (Z1 for psi, Z2 for f_S psi).  Everything else the update
+// needs from psi enters linearly, and IDFT_r(Y1) = N psi, so
+//   IDFT_r(V) = (1 + 1/beta) P_M psi - psi/beta,   V = (1 + 1/beta) Z1 - Y1 / (beta N)
+//   IDFT_r(W) = psi - beta P_M(f_S psi),           W = Y1 / N - beta Z2
+//   psi'      = IDFT_r(W) + beta P_S(IDFT_r(V)).
+// A column block holds Y1, Z1 and Z2 of the same columns, so it writes V and
+// W instead of Z1 and Z2 (one read of the amplitudes for both), and the row
+// pass reads V, W and the support only: psi is neither read nor written
+// until the last iteration.  The values equal those of dm_rows_block's
+// textbook map up to FP32 rounding (the linear combinations are formed in
+// the row-spectral domain).
+// ---------------------------------------------------------------------------
+
+// Column pass: tile `tile` of the B-pattern batch (DmtColCfg<M>: C columns,
+// 512 threads); Y2 = Y1 + half, W = V + half.  lds: M * C elements.
+template <int M>
+__device__ __forceinline__ void dmt_cols_tile(
+    const unsigned tile, c2<float>* lds, const c2<float>* __restrict__ Y1, c2<float>* __restrict__ Vout,
+    const float* __restrict__ A, int64_t amp_batch, int N, float scale, int skip_zero, float beta,
+    size_t half)
+{
+    using V = c2<float>;
+    using PL = typename DmtColCfg<M>::P;
+    constexpr int E = PL::E, TT = PL::T, C = DmtColCfg<M>::C;
+    static_assert(C * TT == DmtColCfg<M>::TPB, "column tile");
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const unsigned tiles_per_pattern = (unsigned)N / C;
+    const unsigned b = tile / tiles_per_pattern;
+    const int col0 = (int)(tile - b * tiles_per_pattern) * C;
+    const size_t pat = (size_t)M * (size_t)N;
+    const size_t blk0 = b * pat + col0;
+    // element m of this lane: row t + rot(m)*T, a wave-uniform row base plus
+    // one per-lane offset shared by all m
+    const unsigned toff = (unsigned)t * (unsigned)N + (unsigned)c;
+    auto rowbase = [&](int m) { return (size_t)(PL::rot(m) * TT) * (size_t)N; };
+    const float* Ap = A + (size_t)(b % (unsigned)amp_batch) * pat + col0;
+    const float invN = 1.f / (float)N;
+    const float ib = 1.f / beta, pb = 1.f + 1.f / beta;
+    V v[E], y[E];
+    float av[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        y[m] = (Y1 + blk0 + rowbase(m))[toff];
+        v[m] = y[m];
+    });
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        av[m] = (Ap + rowbase(m))[toff];
+    });
+    LdsCol<V, C, false> acc{lds + c};
+    // DFT_c, modulus projection with A (scale 1/(MN)), IDFT_c
+    auto project = [&]() {
+        fft_line<PL, -1>(v, t, g_tw<M>(), acc);
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const float a = av[m];
+            const V x = v[m];
+            const float m2 = x.x * x.x + x.y * x.y;
+            float s = scale;
+            if (!(skip_zero && a == 0.f) && m2 > 0.f) s = scale * a * rsqrtf(m2);
+            v[m] = cscale(x, s);
+        });
+        fft_line<PL, +1>(v, t, g_tw<M>(), acc);
+    };
+    project();  // v = Z1
+    const float yv = -ib * invN;
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        (Vout + blk0 + rowbase(m))[toff] = mk<V>(fmaf(pb, v[m].x, yv * y[m].x), fmaf(pb, v[m].y, yv * y[m].y));
+    });
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = (Y1 + half + blk0 + rowbase(m))[toff];
+    });
+    project();  // v = Z2
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        (Vout + half + blk0 + rowbase(m))[toff] =
+            mk<V>(fmaf(-beta, v[m].x, invN * y[m].x), fmaf(-beta, v[m].y, invN * y[m].y));
+    });
+}
+
+template <int M>
+__global__ __launch_bounds__(DmtColCfg<M>::TPB) void k_dmt_cols(
+    const c2<float>* __restrict__ Y, c2<float>* __restrict__ VW, const float* __restrict__ A,
+    int64_t amp_batch, int N, float scale, int skip_zero, float beta, int64_t half)
+{
+    extern __shared__ unsigned char smem[];
+    dmt_cols_tile<M>(blockIdx.x, reinterpret_cast<c2<float>*>(smem), Y, VW, A, amp_batch, N, scale,
+                     skip_zero, beta, (size_t)half);
+}
+
+// Row pass: the LPB lines of block `vb` (DmtRowCfg<N>); W = V + half, Y2 = Y1 + half.
+//   t1'  = P_S(IDFT_r(V))
+//   psi' = IDFT_r(W) + beta t1'
+// then Y1 = DFT_r(psi'), Y2 = DFT_r(psi'/beta + (1 - 1/beta) P_S psi') for
+// the next iteration, or (LAST) psi' in pattern space.
+template <int N, int SPACE, bool LAST>
+__device__ __forceinline__ void dmt_rows_block(
+    const unsigned vb, c2<float>* lds, const c2<float>* __restrict__ VW, c2<float>* __restrict__ psi,
+    const uint8_t* __restrict__ mask, c2<float>* __restrict__ Y, int64_t nlines, float beta, size_t half)
+{
+    using V = c2<float>;
+    using PL = typename DmtRowCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, LPB = DmtRowCfg<N>::LPB;
+    constexpr int SPAN = LdsRow<V, N>::span();
+    static_assert(E <= 32, "support bits in one word");
+    const int tid = threadIdx.x;
+    const int t = tid % TT, ll = tid / TT;
+    const int64_t line = (int64_t)vb * LPB + ll;
+    const bool valid = line < nlines;
+    const size_t base = (size_t)(valid ? line : 0) * N + t;
+    LdsRow<V, N, true> acc{lds + ll * SPAN};
+    const V* tw = g_tw<N>();
+    const float ib = 1.f / beta, cb = 1.f - 1.f / beta;
+    V a[E], b[E];
+    unsigned inbits = 0u;  // bit m: element m lies in the support
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        a[m] = valid ? VW[base + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(a, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const bool inm = valid && mask[base + PL::rot(m) * TT] != 0;
+        inbits |= (inm ? 1u : 0u) << m;
+        a[m] = ps_apply<SPACE>(a[m], inm);  // t1'
+    });
+    // keep the scheduler from hoisting the W loads above the first transform
+    __builtin_amdgcn_sched_barrier(0);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        b[m] = valid ? VW[half + base + PL::rot(m) * TT] : mk<V>(0, 0);
+    });
+    fft_line<PL, +1>(b, t, tw, acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const V pn = mk<V>(fmaf(beta, a[m].x, b[m].x), fmaf(beta, a[m].y, b[m].y));  // psi'
+        if constexpr (LAST) {
+            if (valid) psi[base + PL::rot(m) * TT] = pn;
+        } else {
+            const V q = ps_apply<SPACE>(pn, ((inbits >> m) & 1u) != 0u);
+            a[m] = pn;
+            b[m] = mk<V>(fmaf(cb, q.x, pn.x * ib), fmaf(cb, q.y, pn.y * ib));
+        }
+    });
+    if constexpr (!LAST) {
+        fft_line<PL, -1>(a, t, tw, acc);
+        fft_line<PL, -1>(b, t, tw, acc);
+        if (valid) {
+            sfor<E>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                Y[base + PL::rot(m) * TT] = a[m];
+                Y[half + base + PL::rot(m) * TT] = b[m];
+            });
+        }
+    }
+}
+
+template <int N, int SPACE, bool LAST>
+__global__ __launch_bounds__(DmtRowCfg<N>::TPB) void k_dmt_rows(
+    const c2<float>* __restrict__ VW, c2<float>* __restrict__ psi, const uint8_t* __restrict__ mask,
+    c2<float>* __restrict__ Y, int64_t nlines, float beta, int64_t half)
+{
+    __shared__ c2<float> lds[DmtRowCfg<N>::LPB * LdsRow<c2<float>, N>::span()];
+    dmt_rows_block<N, SPACE, LAST>(blockIdx.x, lds, VW, psi, mask, Y, nlines, beta, (size_t)half);
+}
+
+}  // namespace cpra

@@ -11,8 +11,10 @@
 // baseline).  All three modes compute the same values.
 //
 // The map is Elser's difference map (textbook, beta), the iteration body of
-// cpra_hip_dm_run with textbook = 1: [Y1; Y2] per class in the row-spectral
-// domain, column pass [Y1; Y2] -> [Z1; Z2], row pass -> psi, Y1, Y2.
+// cpra_hip_dm_run with textbook = 1 in its two-pass form (dmt_cols_tile,
+// dmt_rows_block): [Y1; Y2] per class in the row-spectral domain, column
+// pass [Y1; Y2] -> [V; W] (one block per column tile of a pattern for both
+// halves), row pass -> Y1, Y2 (psi after the last iteration).
 #include <vector>
 
 #include "common.hpp"
@@ -24,7 +26,7 @@ constexpr int kMaxClasses = CPRA_HIP_MAX_CLASSES;
 
 struct MixedClass {
     c2<float>* Y;  // [Y1; Y2]: 2 * count patterns
-    c2<float>* Z;  // [Z1; Z2]
+    c2<float>* Z;  // [V; W]
     c2<float>* psi;
     const float* A;
     const uint8_t* mask;
@@ -35,11 +37,6 @@ struct MixedClass {
 
 struct MixedArgs {
     MixedClass c[kMaxClasses];
-    // t1/t2 outputs of the last row pass: null at run time (the textbook map
-    // writes psi only) but passed as arguments, as to the single-size
-    // kernel, so that the compiler builds the same arithmetic for both
-    c2<float>* t1out;
-    c2<float>* t2out;
     int ncls;
     int skip_zero;
     float beta;
@@ -53,16 +50,18 @@ __device__ __forceinline__ int mixed_class(const MixedArgs& a, unsigned b, bool 
     return k;
 }
 
-__global__ __launch_bounds__(256) void k_mixed_cols(const MixedArgs a)
+// 512 threads per block (DmtColCfg), 128 VGPRs: two blocks, four waves per
+// SIMD.
+__global__ __launch_bounds__(512) void k_mixed_cols(const MixedArgs a)
 {
     extern __shared__ unsigned char smem[];
     c2<float>* lds = reinterpret_cast<c2<float>*>(smem);
     const int k = mixed_class(a, blockIdx.x, false);
     const MixedClass& c = a.c[k];
     const unsigned tile = blockIdx.x - c.cb0;
-#define CPRA_MC(NN)                                                                                       \
-    eng_cols_tile<NN, COL_MOD, MAP_BLOCK, false>(tile, lds, nullptr, c.Y, c.Z, c.A, c.count, NN, c.scale, \
-                                                 a.skip_zero, nullptr, nullptr, 0.f, 0, 0)
+#define CPRA_MC(NN)                                                                                    \
+    dmt_cols_tile<NN>(tile, lds, c.Y, c.Z, c.A, c.count, NN, c.scale, a.skip_zero, a.beta, \
+                                 (size_t)c.count * NN * NN)
     switch (c.n) {
     case 64: CPRA_MC(64); break;
     case 128: CPRA_MC(128); break;
@@ -74,12 +73,11 @@ __global__ __launch_bounds__(256) void k_mixed_cols(const MixedArgs a)
 #undef CPRA_MC
 }
 
-// Three waves per SIMD: the 384-point body alone would take 181 VGPRs (two
-// waves for every class); capped at 168 it spills 56 bytes per lane and the
-// batch of 128^2 to 384^2 runs 685 instead of 600 it/s.  The column kernel
-// keeps its 212 VGPRs: capped, it spills 196 bytes per lane and runs slower.
+// Three waves per SIMD (146-157 VGPRs, the 384-point body's): held to four
+// waves (128 VGPRs) it spills about 64 bytes per lane, and the batch of
+// 128^2 to 384^2 runs 848 instead of 925 it/s.
 template <int SPACE, bool LAST>
-__global__ __launch_bounds__(256, 3) void k_mixed_dm_rows(const MixedArgs a)
+__global__ __launch_bounds__(256) void k_mixed_dm_rows(const MixedArgs a)
 {
     extern __shared__ unsigned char smem[];
     c2<float>* lds = reinterpret_cast<c2<float>*>(smem);
@@ -87,9 +85,8 @@ __global__ __launch_bounds__(256, 3) void k_mixed_dm_rows(const MixedArgs a)
     const MixedClass& c = a.c[k];
     const unsigned vb = blockIdx.x - c.rb0;
     const size_t half = (size_t)c.count * c.n * c.n;
-#define CPRA_MR(NN)                                                                                   \
-    dm_rows_block<NN, true, SPACE, LAST>(vb, lds, c.Z, c.Z + half, c.psi, c.mask, c.Y, c.Y + half, \
-                                         a.t1out, a.t2out, (int64_t)c.count * NN, a.beta)
+#define CPRA_MR(NN) \
+    dmt_rows_block<NN, SPACE, LAST>(vb, lds, c.Z, c.psi, c.mask, c.Y, (int64_t)c.count * NN, a.beta, half)
     switch (c.n) {
     case 64: CPRA_MR(64); break;
     case 128: CPRA_MR(128); break;
@@ -110,10 +107,10 @@ struct SizeGeo {
 template <int N> static SizeGeo geo_n()
 {
     SizeGeo g;
-    g.C = SizeCfg<N>::C;
-    g.LPB = DmRowCfg<N>::LPB;
-    g.col_lds = sizeof(c2<float>) * (size_t)N * SizeCfg<N>::C;
-    g.row_lds = sizeof(c2<float>) * (size_t)DmRowCfg<N>::LPB * LdsRow<c2<float>, N>::span();
+    g.C = DmtColCfg<N>::C;
+    g.LPB = DmtRowCfg<N>::LPB;
+    g.col_lds = sizeof(c2<float>) * (size_t)N * DmtColCfg<N>::C;
+    g.row_lds = sizeof(c2<float>) * (size_t)DmtRowCfg<N>::LPB * LdsRow<c2<float>, N>::span();
     return g;
 }
 
@@ -146,19 +143,22 @@ static void rows_n(hipStream_t st, const MixedClass& c, int j0, int cnt, float b
     // patterns [j0, j0 + cnt) of the class
     const size_t pp = (size_t)N * N, half = (size_t)c.count * pp;
     const int64_t lines = (int64_t)cnt * N;
-    constexpr int LPB = DmRowCfg<N>::LPB;
-    hipLaunchKernelGGL((k_dm_rows<N, true, SPACE, LAST>), dim3((unsigned)((lines + LPB - 1) / LPB)),
-                       dim3(DmRowCfg<N>::TPB), 0, st, c.Z + j0 * pp, c.Z + half + j0 * pp, c.psi + j0 * pp,
-                       c.mask + j0 * pp, c.Y + j0 * pp, c.Y + half + j0 * pp, nullptr, nullptr, lines, beta);
+    constexpr int LPB = DmtRowCfg<N>::LPB;
+    hipLaunchKernelGGL((k_dmt_rows<N, SPACE, LAST>), dim3((unsigned)((lines + LPB - 1) / LPB)),
+                       dim3(DmtRowCfg<N>::TPB), 0, st, c.Z + j0 * pp, c.psi + j0 * pp, c.mask + j0 * pp,
+                       c.Y + j0 * pp, lines, beta, (int64_t)half);
 }
 
+// column pass over patterns [j0, j0 + cnt) of the class (both halves)
 template <int N>
-static void cols_n(hipStream_t st, const MixedClass& c, const c2<float>* Y, c2<float>* Z, const float* A,
-                   int64_t B, int64_t amp_batch, int skip)
+static void cols_n(hipStream_t st, const MixedClass& c, int j0, int cnt, int skip, float beta)
 {
-    const unsigned grid = (unsigned)(B * N / SizeCfg<N>::C);
-    hipLaunchKernelGGL((k_eng_cols<N, COL_MOD, MAP_BLOCK, false>), dim3(grid), dim3(SizeCfg<N>::TPB), 0, st, Y, Z,
-                       A, amp_batch, N, c.scale, skip, nullptr, 0, nullptr, 0.f, 0, 0);
+    const size_t pp = (size_t)N * N;
+    const unsigned grid = (unsigned)((int64_t)cnt * N / DmtColCfg<N>::C);
+    const size_t lds = sizeof(c2<float>) * (size_t)N * DmtColCfg<N>::C;
+    hipLaunchKernelGGL((k_dmt_cols<N>), dim3(grid), dim3(DmtColCfg<N>::TPB), lds, st, c.Y + j0 * pp,
+                       c.Z + j0 * pp, c.A + j0 * pp, (int64_t)cnt, N, c.scale, skip, beta,
+                       (int64_t)c.count * pp);
 }
 
 #define CPRA_NSWITCH(NV, CALL)                         \
@@ -211,7 +211,7 @@ struct MixedRun {
     {
         if (mode == CPRA_HIP_MIXED_FUSED) {
             if (kt) kt->begin(h, st, KT_COL);
-            hipLaunchKernelGGL(k_mixed_cols, dim3(col_grid), dim3(256), col_lds, st, a);
+            hipLaunchKernelGGL(k_mixed_cols, dim3(col_grid), dim3(512), col_lds, st, a);
             check_launch("k_mixed_cols");
             if (kt) kt->end(h, st);
             if (kt) kt->begin(h, st, KT_ROW);
@@ -224,16 +224,13 @@ struct MixedRun {
         if (kt) kt->begin(h, st, KT_MIXED);
         for (int k = 0; k < a.ncls; ++k) {
             const MixedClass& c = a.c[k];
-            const size_t pp = (size_t)c.n * c.n, half = (size_t)c.count * pp;
             if (mode == CPRA_HIP_MIXED_PER_CLASS) {
-                CPRA_NSWITCH(c.n, (cols_n<NN>(st, c, c.Y, c.Z, c.A, 2 * (int64_t)c.count, c.count, a.skip_zero)))
+                CPRA_NSWITCH(c.n, (cols_n<NN>(st, c, 0, c.count, a.skip_zero, a.beta)))
                 if (last) rows_split<true>(st, c, 0, c.count);
                 else rows_split<false>(st, c, 0, c.count);
             } else {
                 for (int j = 0; j < c.count; ++j) {
-                    CPRA_NSWITCH(c.n, (cols_n<NN>(st, c, c.Y + j * pp, c.Z + j * pp, c.A + j * pp, 1, 1, a.skip_zero)))
-                    CPRA_NSWITCH(c.n, (cols_n<NN>(st, c, c.Y + half + j * pp, c.Z + half + j * pp, c.A + j * pp, 1, 1,
-                                                  a.skip_zero)))
+                    CPRA_NSWITCH(c.n, (cols_n<NN>(st, c, j, 1, a.skip_zero, a.beta)))
                     if (last) rows_split<true>(st, c, j, 1);
                     else rows_split<false>(st, c, j, 1);
                 }
@@ -282,8 +279,6 @@ int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes,
         r.space = p->space_mode;
         r.mode = launch_mode;
         r.a.ncls = nclasses;
-        r.a.t1out = nullptr;
-        r.a.t2out = nullptr;
         r.a.skip_zero = p->dc_skip_zero;
         r.a.beta = p->beta;
         size_t total = 0;
@@ -314,7 +309,7 @@ int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes,
             c.scale = (float)(1.0 / ((double)in.n * in.n));
             c.cb0 = (unsigned)cb;
             c.rb0 = (unsigned)rb;
-            cb += 2 * (uint64_t)in.count * in.n / sg.C;
+            cb += (uint64_t)in.count * in.n / sg.C;
             rb += ((uint64_t)in.count * in.n + sg.LPB - 1) / sg.LPB;
             r.col_lds = sg.col_lds > r.col_lds ? sg.col_lds : r.col_lds;
             r.row_lds = sg.row_lds > r.row_lds ? sg.row_lds : r.row_lds;

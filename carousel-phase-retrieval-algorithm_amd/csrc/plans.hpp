@@ -120,4 +120,49 @@ template <> struct HasAltRowPlan<512> {
     static constexpr bool value = true;
 };
 
+// Column tiles of the two-pass textbook difference map (dmt_cols_tile): one
+// 512-thread block takes C = 512 / T columns of a pattern for both halves of
+// [Y1; Y2], so it holds the incoming Y1, the transform in flight and the
+// amplitudes at once.  The 192- and 384-point columns take the 12-value
+// plans of the DM row kernels (the 24-value plans leave one or two waves per
+// SIMD with that much state), the 64-point columns 8 values (16 would make
+// the tile wider than the pattern).  LDS: M * C complex64, at most 64 KiB.
+template <int N> struct DmtColCfg {
+    using P = typename SizeCfg<N>::P;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+template <> struct DmtColCfg<64> {
+    using P = Plan<64, 8, 8, 8>;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+template <> struct DmtColCfg<128> {
+    using P = Plan<128, 8, 8, 8, 2>;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+template <> struct DmtColCfg<256> {
+    using P = Plan<256, 8, 8, 8, 4>;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+template <> struct DmtColCfg<512> {
+    using P = Plan<512, 8, 8, 8, 8>;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+template <> struct DmtColCfg<192> {
+    using P = Plan<192, 12, 12, 4, 4>;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+template <> struct DmtColCfg<384> {
+    using P = Plan<384, 12, 4, 4, 4, 6>;
+    static constexpr int TPB = 512, C = TPB / P::T;
+};
+
+// Row plans of the two-pass textbook difference map (dmt_rows_block): the DM
+// row plans, except at 384, where the radix order 4, 4, 4, 6 needs fewer
+// registers than 6, 4, 4, 4.
+template <int N> struct DmtRowCfg : DmRowCfg<N> {};
+template <> struct DmtRowCfg<384> {
+    using P = Plan<384, 12, 4, 4, 4, 6>;
+    static constexpr int TPB = 256, LPB = TPB / P::T;
+};
+
 }  // namespace cpra

@@ -94,12 +94,14 @@ int main()
     both<Plan<64, 16, 16, 4>>("Plan<64, 16, 16, 4>");
     both<Plan<64, 8, 8, 8>>("Plan<64, 8, 8, 8>");
     both<Plan<128, 16, 16, 8>>("Plan<128, 16, 16, 8>");
+    both<Plan<128, 8, 8, 8, 2>>("Plan<128, 8, 8, 8, 2>");
     both<Plan<192, 24, 24, 8>>("Plan<192, 24, 24, 8>");
     both<Plan<192, 12, 12, 4, 4>>("Plan<192, 12, 12, 4, 4>");
     both<Plan<256, 16, 16, 16>>("Plan<256, 16, 16, 16>");
     both<Plan<256, 8, 8, 8, 4>>("Plan<256, 8, 8, 8, 4>");
     both<Plan<384, 24, 6, 8, 8>>("Plan<384, 24, 6, 8, 8>");
     both<Plan<384, 12, 6, 4, 4, 4>>("Plan<384, 12, 6, 4, 4, 4>");
+    both<Plan<384, 12, 4, 4, 4, 6>>("Plan<384, 12, 4, 4, 4, 6>");
     both<Plan<512, 8, 8, 8, 8>>("Plan<512, 8, 8, 8, 8>");
     both<Plan<512, 16, 16, 16, 2>>("Plan<512, 16, 16, 16, 2>");
     both<Plan<512, 32, 32, 16>>("Plan<512, 32, 32, 16>");
