@@ -66,7 +66,7 @@ def measured_traffic(kernel: str, tuning: dict):
     path = os.path.join(ROOT, "profiles", "hio_512x256sq_pmc.md")
     if not os.path.exists(path):
         return None
-    want = ";".join(f"{k}={tuning.get(k)}" for k in ("col_256", "chunk", "rows_sparse_used"))
+    want = ";".join(f"{k}={tuning.get(k)}" for k in ("col_256", "chunk", "split", "rows_sparse_used"))
     for line in open(path):
         parts = line.split()
         if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel

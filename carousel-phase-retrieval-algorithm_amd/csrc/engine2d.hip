@@ -1079,6 +1079,29 @@ void cpra_hip_hio_default_params(cpra_hip_hio_params* p)
     p->use_graph = 1;
 }
 
+// Sub-batches of the HIO run (tuning key 9, split=): with split = s > 1 the
+// batch runs the whole schedule one sub-batch of B/s patterns after another
+// (the patterns are independent, so the results are the same bits).  The
+// state, the intermediate and the amplitudes of a sub-batch of 256 patterns
+// of 256^2 (320 MB) then come largely out of the 256 MB Infinity Cache, which
+// the whole batch (640 MB) does not fit.  Each sub-batch's support goes
+// through one workspace, so every sub-batch has the same pointers and
+// replays the same captured chunk graph.  Applies when s divides the batch
+// and the amplitudes are per pattern or one for all, outside a capture.
+static int hio_split(const cpra_hip_ctx* h)
+{
+    auto it = h->tuning.find((uint64_t)9 << 32);
+    return it == h->tuning.end() || it->second < 1 ? 1 : it->second;
+}
+
+static bool hio_split_applies(const cpra_hip_ctx* h, uint64_t amp_batch, int s)
+{
+    return s > 1 && h->batch % (uint64_t)s == 0 && (amp_batch == h->batch || amp_batch == 1) && !h->capturing;
+}
+
+static int hio_run_batch(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batch,
+                         uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host);
+
 int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_batch,
                      uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host)
 {
@@ -1091,6 +1114,38 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         CPRA_ARG(!(h->capturing && rfactor_host), "cpra_hip_hio_run: the R-factor copy-out needs a "
                                                   "synchronisation, which a stream capture cannot hold");
         engine_check_geometry(h, "cpra_hip_hio_run");
+        const int s = hio_split(h);
+        if (!hio_split_applies(h, amp_batch, s)) return hio_run_batch(h, x, amp, amp_batch, support, p, rfactor_host);
+        const uint64_t B = h->batch, sb = B / (uint64_t)s;
+        const size_t px = (size_t)h->m * h->n;
+        h->ws_sup.ensure(px * sb);
+        uint8_t* sup = static_cast<uint8_t*>(h->ws_sup.ptr);
+        hipStream_t st = h->stream;
+        struct BatchRestore {
+            cpra_hip_ctx* h;
+            uint64_t b;
+            ~BatchRestore() { h->batch = b; }
+        } restore{h, B};
+        h->batch = sb;
+        for (int k = 0; k < s; ++k) {
+            uint8_t* sk = support + (size_t)k * sb * px;
+            CPRA_CHECK_HIP(hipMemcpyAsync(sup, sk, px * sb, hipMemcpyDeviceToDevice, st));
+            const int rc = hio_run_batch(h, static_cast<c2<float>*>(x) + (size_t)k * sb * px,
+                                         amp_batch == B ? amp + (size_t)k * sb * px : amp,
+                                         amp_batch == B ? sb : 1, sup, p,
+                                         rfactor_host ? rfactor_host + (size_t)k * sb : nullptr);
+            if (rc < 0) return rc;
+            CPRA_CHECK_HIP(hipMemcpyAsync(sk, sup, px * sb, hipMemcpyDeviceToDevice, st));
+        }
+        return CPRA_HIP_OK;
+    });
+}
+
+// One HIO run on the handle's batch (h->batch patterns from x).
+static int hio_run_batch(cpra_hip_ctx* h, void* x, const float* amp, uint64_t amp_batch,
+                         uint8_t* support, const cpra_hip_hio_params* p, float* rfactor_host)
+{
+    {
         const Geo g = geo_of(h);
         const int64_t total = (int64_t)g.M * g.N * g.B;
         h->ws_z.ensure(sizeof(c2<float>) * total);
@@ -1218,7 +1273,7 @@ int cpra_hip_hio_run(cpra_hip_handle h, void* x, const float* amp, uint64_t amp_
         }
         kt.collect(h);
         return CPRA_HIP_OK;
-    });
+    }
 }
 
 void cpra_hip_dm_default_params(cpra_hip_dm_params* p)
@@ -1607,6 +1662,7 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
         const bool saved_timing = h->timing;
         h->tuning[(uint64_t)2 << 32] = 0;
         h->tuning[(uint64_t)3 << 32] = 0;
+        h->tuning[(uint64_t)9 << 32] = 1;
         // column variants per state layout (row-major; tile-major for square
         // patterns, where the 32-column tiles do not apply)
         const int nlayout = g.M == g.N ? 2 : 1;
@@ -1730,12 +1786,32 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
             }
         }
         h->tuning[ckey] = best_chunk;
+        // sub-batches run one after another (split=), with the best chunk
+        int best_split = 1;
+        {
+            const uint64_t xkey = (uint64_t)9 << 32;
+            h->tuning[xkey] = 1;
+            double best_s = best_it;
+            for (int sc : {2, 4}) {
+                if (!hio_split_applies(h, amp_batch, sc) || h->batch / (uint64_t)sc < 64) continue;
+                h->tuning[xkey] = sc;
+                const double us = run_us(iters);
+                char buf[64];
+                std::snprintf(buf, sizeof(buf), "iter_split%d_us=%.2f;", sc, us);
+                rep += buf;
+                if (us < best_s) {
+                    best_s = us;
+                    best_split = sc;
+                }
+            }
+            h->tuning[xkey] = best_split;
+        }
         h->timing = saved_timing;
         CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
         CPRA_CHECK_HIP(hipStreamSynchronize(st));
         char head[192];
-        std::snprintf(head, sizeof(head), "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%d;tiled=%d;",
-                      g.M, best_v, g.N, best_rp, best_sp, best_chunk, best_lay);
+        std::snprintf(head, sizeof(head), "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%d;split=%d;tiled=%d;",
+                      g.M, best_v, g.N, best_rp, best_sp, best_chunk, best_split, best_lay);
         const std::string spec = head;
         char desc[96];
         std::snprintf(desc, sizeof(desc), "col_map=%s;col_xcd=%d;",
@@ -1749,8 +1825,9 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
     });
 }
 
-// Applies the col_<M>=, rows_sparse=, chunk=, tiled=, plane= and merge_lines= entries of a
-// spec such as an autotune report; other keys are skipped.
+// Applies the col_<M>=, row_plan_<N>=, rows_sparse=, chunk=, split=, tiled=, plane=,
+// merge_lines= and plane_extract= entries of a spec such as an autotune report;
+// other keys are skipped.
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
 {
     return eguard(h, [&]() -> int {
@@ -1768,6 +1845,10 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
                 p += used;
             } else if (std::sscanf(p, "tiled=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)4 << 32] = v ? 1 : 0;
+                p += used;
+            } else if (std::sscanf(p, "split=%d;%n", &v, &used) == 1 && used > 0) {
+                CPRA_ARG(v >= 1, "cpra_hip_set_tuning: split < 1");
+                h->tuning[(uint64_t)9 << 32] = v;
                 p += used;
             } else if (std::sscanf(p, "plane=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)5 << 32] = v ? 1 : 0;
@@ -1811,10 +1892,10 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
         std::snprintf(out, out_len,
-                      "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%lld;tiled=%d;plane=%d;merge_lines=%d;"
+                      "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%lld;split=%d;tiled=%d;plane=%d;merge_lines=%d;"
                       "plane_extract=%d;rows_sparse_used=%d;graph_captures=%lld;",
                       (int)h->m, col_variant_code(cv), (int)h->n, row_plan(h, (int)h->n), sparse_mode(h),
-                      (long long)iteration_chunk(h), tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0,
+                      (long long)iteration_chunk(h), hio_split(h), tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0,
                       engine_merge_lines(h) ? 1 : 0, engine_plane_extract(h) ? 1 : 0, rows_sparse_used(h),
                       (long long)h->graph_captures);
         return CPRA_HIP_OK;

@@ -475,3 +475,35 @@ def test_autotune_row_plan_and_tuning_cache(hipmod, gpu, synth, tmp_path):
         assert np.array_equal(x.numpy(), data["x0"])
     finally:
         hipmod.set_tuning_cache(None)
+
+
+@pytest.mark.parametrize("split,amp_per_pattern", [(2, True), (4, True), (4, False), (3, True)])
+def test_split_batch_runs_the_same_bits(hipmod, gpu, synth, split, amp_per_pattern):
+    """split=s runs the whole schedule one sub-batch of B/s patterns after
+    another (each sub-batch's support through one workspace): the iterate,
+    the updated support and the R-factors equal the whole-batch run's bits.
+    Amplitudes per pattern or one for all; split=3 does not divide the batch
+    of 8 and runs it whole."""
+    n, P = 128, 8
+    data = synth.make_2d_batch(P, n, 15, 20, 1e6, config=140)
+    amp0 = data["amp"] if amp_per_pattern else data["amp"][:1]
+    kw = dict(iters=40, beta=0.9, real_positive=1, dc_skip_zero=1, sw_every=20)
+    outs = []
+    for s in (1, split):
+        x = hipmod.to_device(data["x0"])
+        sup = hipmod.to_device(data["support"])
+        amp = hipmod.to_device(amp0)
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            h.set_tuning(f"split={s};")
+            assert h.get_tuning()["split"] == str(s)
+            rf = h.hio_run(x, amp, sup, want_rfactor=True, use_graph=1, **kw)
+            # a second call on the same handle replays the kept chunk graph
+            x.upload(data["x0"])
+            sup.upload(data["support"])
+            rf2 = h.hio_run(x, amp, sup, want_rfactor=True, use_graph=1, **kw)
+            h.Sync()
+        assert np.array_equal(rf, rf2)
+        outs.append((x.numpy(), sup.numpy(), rf))
+    assert np.array_equal(outs[1][0], outs[0][0])
+    assert np.array_equal(outs[1][1], outs[0][1])
+    assert np.array_equal(outs[1][2], outs[0][2])
