@@ -37,13 +37,19 @@ namespace cpra {
 // non-template kernels of the engine (one translation unit only)
 // Row bits of a support mask [rows][N] (N % 16 == 0): bit r of the
 // concatenated per-pattern words is set when row r holds a support pixel.
-// 16 threads per row, each testing 16-byte chunks.
-__global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict__ mask,
+// One block of 512 threads per word: 16 threads per row, each testing
+// 16-byte chunks; the 32 rows' bits are gathered in the block and the word
+// stored whole (no atomics, no clearing pass).  Block 0 also clears the
+// natural-order flag.
+__global__ __launch_bounds__(512) void k_support_rows(const uint8_t* __restrict__ mask,
                                                       int64_t nrows, int N,
-                                                      unsigned* __restrict__ rowbits)
+                                                      unsigned* __restrict__ rowbits,
+                                                      unsigned* __restrict__ flag)
 {
+    __shared__ unsigned wbits[8];
     const int lane16 = threadIdx.x & 15;
-    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int rl = threadIdx.x >> 4;  // row of the word, 0..31
+    const int64_t row = (int64_t)blockIdx.x * 32 + rl;
     unsigned any = 0;
     if (row < nrows) {
         const uint4* p = reinterpret_cast<const uint4*>(mask + row * N);
@@ -53,7 +59,20 @@ __global__ __launch_bounds__(256) void k_support_rows(const uint8_t* __restrict_
         }
     }
     for (int off = 8; off > 0; off >>= 1) any |= __shfl_xor(any, off, 16);
-    if (lane16 == 0 && row < nrows && any) atomicOr(&rowbits[row >> 5], 1u << (row & 31));
+    // lanes 0, 16, 32 and 48 of each wave hold its four rows' results
+    const unsigned long long bal = __ballot(any != 0u);
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned nib = (unsigned)(bal & 1ull) | (unsigned)((bal >> 15) & 2ull) |
+                             (unsigned)((bal >> 30) & 4ull) | (unsigned)((bal >> 45) & 8ull);
+        wbits[threadIdx.x >> 6] = nib;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned w = 0;
+        for (int k = 0; k < 8; ++k) w |= wbits[k] << (4 * k);
+        rowbits[blockIdx.x] = w;
+        if (blockIdx.x == 0) flag[0] = 0u;
+    }
 }
 
 // Adaptive rows, one block: when more than 4/5 of the nrows row bits are set,
@@ -596,20 +615,22 @@ static int sparse_mode(const cpra_hip_ctx* h)
 // more than 4/5 of the rows hold support (k_rows_decide); they then do the
 // dense kernels' work at the dense kernels' speed.
 
-// Per pattern M/32 words: bit r set when row r holds a support pixel; one
-// more word after them: the natural-order flag of the adaptive mode.
+// ws_rows: the natural-order flag of the adaptive mode in the first word
+// (64 words reserved, so its place does not depend on the batch, which a
+// split run changes), then per pattern M/32 words with bit r set when row r
+// holds a support pixel.
 static size_t row_words(const cpra_hip_ctx* h)
 {
     return ((size_t)h->batch * h->m + 31) / 32;
 }
 
-static unsigned* row_bits(cpra_hip_ctx* h)
+static unsigned* row_order_flag(cpra_hip_ctx* h)
 {
-    h->ws_rows.ensure(4 * row_words(h) + 256);
+    h->ws_rows.ensure(256 + 4 * row_words(h));
     return static_cast<unsigned*>(h->ws_rows.ptr);
 }
 
-static unsigned* row_order_flag(cpra_hip_ctx* h) { return row_bits(h) + row_words(h); }
+static unsigned* row_bits(cpra_hip_ctx* h) { return row_order_flag(h) + 64; }
 
 // Row bits of the support (the natural-order flag cleared).
 static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* mask)
@@ -617,9 +638,8 @@ static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* 
     const Geo g = geo_of(h);
     unsigned* bits = row_bits(h);
     const int64_t nrows = g.B * g.M;
-    const int64_t words = (int64_t)row_words(h) + 1;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(words, 256, 4096)), dim3(256), 0, st, bits, 0u, words);
-    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)((nrows + 15) / 16)), dim3(256), 0, st, mask, nrows, g.N, bits);
+    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)row_words(h)), dim3(512), 0, st, mask, nrows, g.N, bits,
+                       row_order_flag(h));
     check_launch("k_support_rows");
 }
 
