@@ -634,11 +634,13 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __r
     float mx = -INFINITY;
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        bl[at(m)] = v[m].x;
         mx = fmaxf(mx, v[m].x);
         if constexpr (PAIR) {
-            bl[at(m) + 1] = v[m].y;
+            // the two columns as one 8-byte store
+            *reinterpret_cast<float2*>(bl + at(m)) = make_float2(v[m].x, v[m].y);
             mx = fmaxf(mx, v[m].y);
+        } else {
+            bl[at(m)] = v[m].x;
         }
     });
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
