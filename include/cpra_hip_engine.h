@@ -124,9 +124,10 @@ int cpra_hip_support_from_constraint(cpra_hip_handle h, const float* S, uint8_t*
  * kernel's radix plan where the size has two (256, 512: 16 or 8 values per
  * thread), dense against sparse row passes (rows without a support pixel
  * finished in the column kernel) and the patterns per launch (whole batch or
- * Infinity-Cache-sized chunks); keeps the fastest for this handle, restores x
- * and the support, and writes "col_<m>=<v>;row_plan_<n>=<v>;rows_sparse=<0|1>;
- * chunk=<c>;tiled=<0|1>;...;col_v<k>_us=..;iter_*_us=..;" into report (may
+ * Infinity-Cache-sized chunks) and the run in 2 or 4 sub-batches (split);
+ * keeps the fastest for this handle, restores x and the support, and writes
+ * "col_<m>=<v>;row_plan_<n>=<v>;rows_sparse=<0|1>;chunk=<c>;split=<s>;
+ * tiled=<0|1>;...;col_v<k>_us=..;iter_*_us=..;" into report (may
  * be NULL).  With a tuning cache set (cpra_hip_set_tuning_cache), a pick
  * stored for the same device architecture, pattern size, batch and amplitude
  * batch is applied instead of timing ("...;cached=1;" in the report), and a
@@ -139,7 +140,11 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * hold a support pixel, decided on the device after every shrinkwrap update;
  * chunk: patterns
  * per launch of the iteration kernels, 0 = whole batch; tiled: tile-major
- * state for square patterns), "plane=<0|1>;" (whole-plane LDS transforms
+ * state for square patterns), "split=<s>;" (cpra_hip_hio_run takes the batch
+ * through the whole schedule in s sub-batches of batch/s patterns, one after
+ * another, with the same results; applies when s divides the batch and the
+ * amplitudes are per pattern or one for all, outside a capture; 1 = whole
+ * batch), "plane=<0|1>;" (whole-plane LDS transforms
  * in the carousel's volume constraint for n <= 128; 0 = one pass per axis),
  * "merge_lines=<0|1>;" (the carousel's forward column DFT written
  * straight into the merge's [P][v][u] layout; 0 = the DFT in place and a
@@ -155,7 +160,7 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
  * entry. */
 int cpra_hip_set_tuning_cache(const char* path, int refresh);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
- * tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_extract=<0|1>;row_plan_<n>=<0|1>;
+ * split=<s>;tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_extract=<0|1>;row_plan_<n>=<0|1>;
  * rows_sparse_used=<0|1>;
  * graph_captures=<k>;" (rows_sparse_used: the row mode of the last iterations
  * run; in the adaptive mode it is read from the device, which synchronises
