@@ -535,8 +535,16 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>*
     constexpr int E = PL::E, TT = PL::T, LPB = SizeCfg<N>::LPB;
     constexpr int SPAN = LdsRow<V, N>::span();
     __shared__ V lds[LPB * SPAN];
+    // the row factor g_c(k) / N per position, one expf per position and block
+    // instead of one per element
+    __shared__ float gtab[N];
     const int tid = threadIdx.x;
     const int t = tid % TT, ll = tid / TT;
+    for (int i = tid; i < N; i += blockDim.x) {
+        const float d = (float)(i - N / 2);
+        gtab[i] = expf(gcoef_c * d * d) * invN;
+    }
+    __syncthreads();
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
     // |x| of line `line` of the half starting at line0 (a half's LPB lines lie
@@ -576,8 +584,7 @@ __global__ __launch_bounds__(SizeCfg<N>::TPB) void k_sw_rowblur(const c2<float>*
     fft_line<PL, -1>(v, t, tw, acc);
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const float d = (float)(t + PL::rot(m) * TT - N / 2);
-        v[m] = cscale(v[m], expf(gcoef_c * d * d) * invN);
+        v[m] = cscale(v[m], gtab[t + PL::rot(m) * TT]);
     });
     fft_line<PL, +1>(v, t, tw, acc);
     const int64_t baseA = (int64_t)(lineA0 + ll) * N, baseB = (int64_t)(lineB0 + ll) * N;
@@ -606,8 +613,15 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __r
     constexpr int CW = PAIR ? 2 * C : C;  // columns per block
     __shared__ V lds[M * C];
     __shared__ float wmax[TPB / 64];
+    // the column factor g_r(k) / M per position (one expf per position)
+    __shared__ float gtab[M];
     const int tid = threadIdx.x;
     const int c = tid % C, t = tid / C;
+    for (int i = tid; i < M; i += TPB) {
+        const float d = (float)(i - M / 2);
+        gtab[i] = expf(gcoef_r * d * d) * invM;
+    }
+    __syncthreads();
     const unsigned tiles_per_pattern = (unsigned)N / CW;
     const unsigned b = blockIdx.x / tiles_per_pattern;
     const int col0 = (int)(blockIdx.x - b * tiles_per_pattern) * CW;
@@ -627,8 +641,7 @@ __global__ __launch_bounds__(SizeCfg<M>::TPB) void k_sw_colblur(const float* __r
     fft_line<PL, -1>(v, t, g_tw<M>(), acc);
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const float d = (float)(t + PL::rot(m) * TT - M / 2);
-        v[m] = cscale(v[m], expf(gcoef_r * d * d) * invM);
+        v[m] = cscale(v[m], gtab[t + PL::rot(m) * TT]);
     });
     fft_line<PL, +1>(v, t, g_tw<M>(), acc);
     float mx = -INFINITY;
