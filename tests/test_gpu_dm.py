@@ -300,3 +300,30 @@ def test_dm_reference_200_iterations_within_fp64_spread(hipmod, oracle, gpu, syn
     err = np.abs(rf - rr) / rr
     print(f"DM 200 its: R gpu {rf} fp64 {rr} perturbed {rp}: rel err {err}, FP64 self-spread {spread:.3e}")
     assert err.max() < max(1e-3, 3 * spread)
+
+
+@pytest.mark.parametrize("m,n", [(64, 64), (192, 192), (256, 256), (384, 384), (512, 512), (128, 256),
+                                 (256, 64)])
+@pytest.mark.parametrize("space_mode", [1, 2])
+def test_dm_textbook_multi_iteration_matches_oracle(hipmod, oracle, gpu, m, n, space_mode):
+    """The two-pass textbook map (k_dmt_cols / k_dmt_rows) over several
+    iterations in one call, so that the row pass's next-iteration output
+    [Y1; Y2] (not only the last iteration's psi) is checked against the
+    FP64 oracle's textbook map: 2e-5 per iteration, square and non-square
+    patterns, every column plan."""
+    rng = np.random.default_rng(m * 1000 + n + space_mode)
+    B, iters = 2, 4
+    yy, xx = np.mgrid[0:m, 0:n]
+    sup = (((yy - m / 2) / (0.3 * m)) ** 2 + ((xx - n / 2) / (0.3 * n)) ** 2 <= 1.0).astype(np.uint8)
+    sup = np.broadcast_to(sup, (B, m, n)).copy()
+    obj = (rng.uniform(0.2, 1.0, (B, m, n)) * sup).astype(np.complex64)
+    amp = np.abs(np.fft.fftshift(np.fft.fft2(np.fft.fftshift(obj, axes=(-2, -1))), axes=(-2, -1)))
+    amp = amp.astype(np.float32)
+    psi0 = (rng.standard_normal((B, m, n)) + 1j * rng.standard_normal((B, m, n))).astype(np.complex64)
+    psi = hipmod.to_device(psi0)
+    with hipmod.HipImpl(m, n, 1, B) as h:
+        assert h.dm_run(psi, hipmod.to_device(amp), hipmod.to_device(sup), iters=iters, beta=0.8,
+                        textbook=1, space_mode=space_mode, use_graph=1)
+        h.Sync()
+    ref = oracle.dm_textbook(psi0, amp, 0.8, iters, mask=sup, space_mode=space_mode)
+    assert relerr(psi.numpy(), ref) < 2e-5 * iters, (m, n)
