@@ -406,6 +406,8 @@ int cpra_hip_destroy(cpra_hip_handle h)
     h->ws_z.release();
     h->ws_state.release();
     h->ws_amp_t.release();
+    h->ws_sup.release();
+    h->ws_sbits.release();
     h->ws_blur.release();
     h->ws_keys.release();
     h->ws_part.release();

@@ -114,6 +114,8 @@ struct cpra_hip_ctx {
     cpra::Buffer ws_state, ws_amp_t;
     // one sub-batch's support while the HIO run goes sub-batch by sub-batch
     cpra::Buffer ws_sup;
+    // the HIO support packed one bit per pixel (k_pack_support)
+    cpra::Buffer ws_sbits;
     // gather list of the carousel merge (merge3d.hip)
     int mplan_n = 0, mplan_P = 0, mplan_heavy = 0;
     int64_t mplan_entries = 0;

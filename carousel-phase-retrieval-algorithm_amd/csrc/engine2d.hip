@@ -35,14 +35,28 @@
 namespace cpra {
 
 // non-template kernels of the engine (one translation unit only)
-// Row bits of a support mask [rows][N] (N % 16 == 0): bit r of the
-// concatenated per-pattern words is set when row r holds a support pixel.
-// One block of 512 threads per word: 16 threads per row, each testing
-// 16-byte chunks; the 32 rows' bits are gathered in the block and the word
-// stored whole (no atomics, no clearing pass).  Block 0 also clears the
-// natural-order flag.
-__global__ __launch_bounds__(512) void k_support_rows(const uint8_t* __restrict__ mask,
-                                                      int64_t nrows, int N,
+// 16 bytes of a u8 mask -> 16 bits (bit k: byte k nonzero).
+__device__ __forceinline__ unsigned mask16_bits(uint4 q)
+{
+    unsigned b = 0;
+    const unsigned w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) b |= (((w[i] >> (8 * k)) & 0xffu) != 0u ? 1u : 0u) << (4 * i + k);
+    return b;
+}
+
+// The HIO support of a u8 mask [rows][N] (N % 32 == 0), packed: N/32 words
+// per row with bit j of word w set when pixel 32 w + j is nonzero (the row
+// kernel's support test), and the row bits: bit r of the concatenated
+// per-pattern words set when row r holds a support pixel.  One block of 512
+// threads per 32 rows: 16 threads per row, each taking 16-byte chunks; two
+// neighbouring chunks make one packed word; the 32 rows' bits are gathered in
+// the block and stored as one word (no atomics, no clearing pass).  Block 0
+// also clears the natural-order flag.
+__global__ __launch_bounds__(512) void k_pack_support(const uint8_t* __restrict__ mask, int64_t nrows, int N,
+                                                      unsigned* __restrict__ sbits,
                                                       unsigned* __restrict__ rowbits,
                                                       unsigned* __restrict__ flag)
 {
@@ -50,13 +64,19 @@ __global__ __launch_bounds__(512) void k_support_rows(const uint8_t* __restrict_
     const int lane16 = threadIdx.x & 15;
     const int rl = threadIdx.x >> 4;  // row of the word, 0..31
     const int64_t row = (int64_t)blockIdx.x * 32 + rl;
+    const int chunks = N / 16;
+    const bool okr = row < nrows;
     unsigned any = 0;
-    if (row < nrows) {
-        const uint4* p = reinterpret_cast<const uint4*>(mask + row * N);
-        for (int i = lane16; i < N / 16; i += 16) {
-            const uint4 q = p[i];
-            any |= q.x | q.y | q.z | q.w;
+    for (int j0 = 0; j0 < chunks; j0 += 16) {
+        const int j = j0 + lane16;
+        unsigned half = 0;
+        if (okr && j < chunks) {
+            half = mask16_bits(reinterpret_cast<const uint4*>(mask + row * N)[j]);
+            any |= half;
         }
+        // chunks 2k and 2k + 1 sit in neighbouring lanes
+        const unsigned other = __shfl_xor(half, 1, 16);
+        if (okr && j < chunks && (j & 1) == 0) sbits[row * (N / 32) + j / 2] = half | (other << 16);
     }
     for (int off = 8; off > 0; off >>= 1) any |= __shfl_xor(any, off, 16);
     // lanes 0, 16, 32 and 48 of each wave hold its four rows' results
@@ -173,6 +193,7 @@ struct EngineGraph {
     const void* mask = nullptr;
     const void* ws = nullptr;
     const void* ws_rows = nullptr;
+    const void* sbits = nullptr;
     bool sparse = true;
     int64_t chunk = 0;
     uint64_t amp_batch = 0;
@@ -345,7 +366,7 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
 
 template <int N, int PV>
 static void launch_rows_hio_np(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                               const uint8_t* mask, float beta, int rp, int er,
+                               const unsigned* mask, float beta, int rp, int er,
                                const unsigned* rowbits, const unsigned* order, bool tiled)
 {
     const int64_t lines = g.B * g.M;
@@ -383,7 +404,7 @@ static void launch_rows_hio_np(hipStream_t st, const Geo& g, const void* Z, void
 
 template <int N>
 static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                              const uint8_t* mask, float beta, int rp, int er,
+                              const unsigned* mask, float beta, int rp, int er,
                               const unsigned* rowbits, const unsigned* order, bool tiled, int pv)
 {
     if constexpr (HasAltRowPlan<N>::value) {
@@ -408,8 +429,9 @@ static bool has_alt_row_plan(int n)
     return (n == 256 && HasAltRowPlan<256>::value) || (n == 512 && HasAltRowPlan<512>::value);
 }
 
+// mask: the support packed by k_pack_support (N/32 words per row)
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
-                            const uint8_t* mask, float beta, int rp, int er,
+                            const unsigned* mask, float beta, int rp, int er,
                             const unsigned* rowbits = nullptr, const unsigned* order = nullptr,
                             bool tiled = false, int pv = 0)
 {
@@ -632,25 +654,35 @@ static unsigned* row_order_flag(cpra_hip_ctx* h)
 
 static unsigned* row_bits(cpra_hip_ctx* h) { return row_order_flag(h) + 64; }
 
-// Row bits of the support (the natural-order flag cleared).
-static void support_rows_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* mask)
+// The packed support for the row kernel (N/32 words per row).
+static unsigned* support_bits(cpra_hip_ctx* h)
+{
+    h->ws_sbits.ensure(sizeof(unsigned) * (size_t)h->batch * h->m * (h->n / 32));
+    return static_cast<unsigned*>(h->ws_sbits.ptr);
+}
+
+// The support packed for the row kernel, and its row bits (the
+// natural-order flag cleared).
+static void support_pack_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* mask)
 {
     const Geo g = geo_of(h);
     unsigned* bits = row_bits(h);
     const int64_t nrows = g.B * g.M;
-    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)row_words(h)), dim3(512), 0, st, mask, nrows, g.N, bits,
-                       row_order_flag(h));
-    check_launch("k_support_rows");
+    hipLaunchKernelGGL(k_pack_support, dim3((unsigned)row_words(h)), dim3(512), 0, st, mask, nrows, g.N,
+                       support_bits(h), bits, row_order_flag(h));
+    check_launch("k_pack_support");
 }
 
-// Lists the support rows for the next iterations and returns whether they run
-// the sparse kernels (the adaptive mode decides dense or sparse on the device:
-// no synchronisation, so it also runs inside a stream capture).
+// Packs the support for the row kernels, lists its rows for the next
+// iterations and returns whether they run the sparse kernels (the adaptive
+// mode decides dense or sparse on the device: no synchronisation, so it also
+// runs inside a stream capture).
 static bool decide_sparse(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support)
 {
     const int mode = sparse_mode(h);
+    // the row kernels read the packed support in every row mode
+    support_pack_launch(h, st, support);
     if (mode == SPARSE_OFF) return false;
-    support_rows_launch(h, st, support);
     if (mode == SPARSE_AUTO) {
         const int64_t words = (int64_t)row_words(h);
         hipLaunchKernelGGL(k_rows_decide, dim3(1), dim3(1024), 0, st, row_bits(h), words,
@@ -726,10 +758,13 @@ void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in,
     launch_amp_retile(st, n, patterns, in, out);
 }
 
+// mask: the support, whose packed copy (support_bits, written by
+// decide_sparse) the row kernels read.
 static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
                           int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p,
                           bool sparse, KernelTimer* kt = nullptr, bool tiled = false)
 {
+    (void)mask;
     const Geo g = geo_of(h);
     const float scale = (float)(1.0 / ((double)g.M * g.N));
     const unsigned* bits = sparse ? row_bits(h) : nullptr;
@@ -738,7 +773,7 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
     for (int64_t p0 = 0; p0 < g.B; p0 += chunk) {
         const Geo gc{g.M, g.N, (g.B - p0) < chunk ? (g.B - p0) : chunk};
         void* xc = static_cast<c2<float>*>(x) + p0 * px;
-        const uint8_t* mc = mask + p0 * px;
+        const unsigned* mc = support_bits(h) + p0 * g.M * (g.N / 32);
         const unsigned* bc = bits ? bits + p0 * g.M / 32 : nullptr;
         const unsigned* order = bits ? row_order_flag(h) : nullptr;
         if (kt) kt->begin(h, st, KT_COL);
@@ -789,7 +824,7 @@ void engine_hio_rows(cpra_hip_ctx* h, hipStream_t st, void* Y, const uint8_t* su
 {
     const Geo g = geo_of(h);
     if (kt) kt->begin(h, st, KT_ROW);
-    launch_rows_hio(st, g, h->ws_z.ptr, Y, support, p->beta, p->real_positive, p->error_reduction,
+    launch_rows_hio(st, g, h->ws_z.ptr, Y, support_bits(h), p->beta, p->real_positive, p->error_reduction,
                     nullptr, nullptr, false, row_plan(h, g.N));
     if (kt) kt->end(h, st);
 }
@@ -947,7 +982,7 @@ hipGraphExec_t kept_graph_exec(cpra_hip_ctx* h, KeptGraph& kg, hipGraph_t g)
 void engine_state_key(const cpra_hip_ctx* h, std::vector<uint64_t>& key)
 {
     for (const Buffer* b : {&h->ws_z, &h->ws_blur, &h->ws_keys, &h->ws_part, &h->ws_part2, &h->ws_misc,
-                            &h->ws_backup, &h->ws_rows, &h->ws_dm, &h->ws_trig, &h->ws_mcount,
+                            &h->ws_backup, &h->ws_rows, &h->ws_sbits, &h->ws_sup, &h->ws_dm, &h->ws_trig, &h->ws_mcount,
                             &h->ws_mlist, &h->ws_mheavy, &h->ws_state, &h->ws_amp_t, &h->ws_zsup})
         key.push_back((uint64_t)(uintptr_t)b->ptr);
     for (const auto& kv : h->tuning) {
@@ -1215,6 +1250,7 @@ static int hio_run_batch(cpra_hip_ctx* h, void* x, const float* amp, uint64_t am
             EngineGraph*& eg = h->graph[sp ? 1 : 0];
             const bool same = eg && eg->exec && eg->x == Y && eg->amp == A && eg->mask == support &&
                               eg->ws == h->ws_z.ptr && eg->ws_rows == (sp ? h->ws_rows.ptr : nullptr) &&
+                              eg->sbits == h->ws_sbits.ptr &&
                               eg->chunk == iteration_chunk(h) &&
                               eg->amp_batch == amp_batch && eg->iters == chunk &&
                               eg->beta == p->beta && eg->rp == p->real_positive &&
@@ -1238,6 +1274,7 @@ static int hio_run_batch(cpra_hip_ctx* h, void* x, const float* amp, uint64_t am
                 eg->mask = support;
                 eg->ws = h->ws_z.ptr;
                 eg->ws_rows = sp ? h->ws_rows.ptr : nullptr;
+                eg->sbits = h->ws_sbits.ptr;
                 eg->sparse = sp;
                 eg->chunk = iteration_chunk(h);
                 eg->amp_batch = amp_batch;

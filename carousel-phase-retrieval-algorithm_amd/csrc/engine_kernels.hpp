@@ -318,9 +318,59 @@ enum RowMode { ROW_HIO = 0, ROW_ER = 1 };
 // TILED: Z and Y in the tile-major layout of N x N patterns (Tiled<N>), the
 // mask row-major; a block's LPB lines then lie in one pattern (LPB | N).
 // PV: the radix-plan variant (HioRowPlan).
+// The support bit of element m of this thread's line (position t + rot(m) T,
+// T the plan's threads per line) from the line's N/32 packed words sw
+// (k_pack_support: bit j of word w is pixel 32 w + j).  T divides 32 or 32
+// divides T, so the word index is a compile-time constant (plus t / 32 when
+// T > 32) and the bit a lane offset.
+template <class PL, int m>
+__device__ __forceinline__ bool support_bit(const unsigned* sw, int t)
+{
+    constexpr int TT = PL::T;
+    constexpr int base = PL::rot(m) * TT;
+    static_assert(32 % TT == 0 || TT % 32 == 0, "threads per line and 32 must nest");
+    if constexpr (TT <= 32) {
+        return (sw[base >> 5] >> (t + (base & 31))) & 1u;
+    } else if constexpr (TT == 64) {
+        return (((t >> 5) ? sw[(base >> 5) + 1] : sw[base >> 5]) >> (t & 31)) & 1u;
+    } else {
+        static_assert(TT <= 64, "at most 64 threads per line");
+        return false;
+    }
+}
+
+// The N/32 packed support words of a line into registers (16-byte loads when
+// N is a multiple of 128, else 8-byte; every lane of the line loads the same
+// words).
+template <int N>
+__device__ __forceinline__ void load_support_words(unsigned (&sw)[N / 32], const unsigned* __restrict__ p,
+                                                   bool ok)
+{
+    if constexpr (N % 128 == 0) {
+        sfor<N / 128>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const uint4 w = ok ? reinterpret_cast<const uint4*>(p)[q] : make_uint4(0, 0, 0, 0);
+            sw[4 * q] = w.x;
+            sw[4 * q + 1] = w.y;
+            sw[4 * q + 2] = w.z;
+            sw[4 * q + 3] = w.w;
+        });
+    } else {
+        static_assert(N % 64 == 0, "whole 8-byte pairs");
+        sfor<N / 64>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const uint2 w = ok ? reinterpret_cast<const uint2*>(p)[q] : make_uint2(0, 0);
+            sw[2 * q] = w.x;
+            sw[2 * q + 1] = w.y;
+        });
+    }
+}
+
+// sbits: the support packed by k_pack_support, N/32 words per line (line-
+// major, whatever the state's layout).
 template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false, int PV = 0>
 __global__ __launch_bounds__((HioRowPlan<N, PV>::TPB)) void k_eng_rows_hio(
-    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const uint8_t* __restrict__ mask,
+    const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const unsigned* __restrict__ sbits,
     int64_t nlines, float beta, float invN, int real_positive,
     const unsigned* __restrict__ rowbits, const unsigned* __restrict__ natural_order = nullptr)
 {
@@ -383,15 +433,14 @@ __global__ __launch_bounds__((HioRowPlan<N, PV>::TPB)) void k_eng_rows_hio(
     };
     const c2<float>* Zb = Z + pbase;
     c2<float>* Yb = Y + pbase;
-    const uint8_t* Sb = mask + bbase;
+    unsigned sw[N / 32];
+    load_support_words<N>(sw, sbits + (size_t)(inb ? line : 0) * (N / 32), inb);
     LdsRow<V, N, true> acc{lds + ll * SPAN};
     const V* tw = g_tw<N>();
     V xp[E], xo[E];
-    uint8_t sm[E];
     const BufC64 zr(Zb, (unsigned)(N * N * sizeof(V))), yr(Yb, (unsigned)(N * N * sizeof(V)));
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const int o = lo + PL::rot(m) * TT;
         if constexpr (TILED) {
             constexpr unsigned kbytes = (unsigned)(TL::template kpart<PL::rot(m)>() * sizeof(V));
             xp[m] = inb ? zr.ld(sl * (unsigned)sizeof(V), kbytes) : mk<V>(0, 0);
@@ -400,13 +449,12 @@ __global__ __launch_bounds__((HioRowPlan<N, PV>::TPB)) void k_eng_rows_hio(
             xp[m] = inb ? (Zb + kb(mc))[sl] : mk<V>(0, 0);
             xo[m] = inb ? (Yb + kb(mc))[sl] : mk<V>(0, 0);
         }
-        sm[m] = inb ? Sb[o] : (uint8_t)0;
     });
     fft_line<PL, +1>(xp, t, tw, acc);
     fft_line<PL, +1>(xo, t, tw, acc);
     sfor<E>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const bool s = sm[m] != 0;
+        const bool s = support_bit<PL, m>(sw, t);
         const V p = xp[m];
         const bool ok = s && (!real_positive || p.x >= 0.f);
         V r;

@@ -126,7 +126,7 @@ template <> struct HasAltRowPlan<512> {
 // amplitudes at once.  The 192- and 384-point columns take the 12-value
 // plans of the DM row kernels (the 24-value plans leave one or two waves per
 // SIMD with that much state), the 64-point columns 8 values (16 would make
-// the tile wider than the pattern).  LDS: M * C complex64, at most 64 KiB.
+// the tile wider than the pattern).  LDS: M * C complex64, 32 or 48 KiB.
 template <int N> struct DmtColCfg {
     using P = typename SizeCfg<N>::P;
     static constexpr int TPB = 512, C = TPB / P::T;
