@@ -507,3 +507,29 @@ def test_split_batch_runs_the_same_bits(hipmod, gpu, synth, split, amp_per_patte
     assert np.array_equal(outs[1][0], outs[0][0])
     assert np.array_equal(outs[1][1], outs[0][1])
     assert np.array_equal(outs[1][2], outs[0][2])
+
+
+@pytest.mark.parametrize("n", [64, 192, 256, 512])
+def test_support_values_other_than_one_count_as_support(hipmod, oracle, gpu, synth, n):
+    """The row kernel reads the support packed one bit per pixel
+    (k_pack_support): any nonzero byte is support, as in the oracle.  One
+    step with the support's ones replaced by 2..255 gives the bits of the
+    0/1 support, and matches the oracle."""
+    P = 2
+    data = synth.make_2d_batch(P, n, n * 0.12, n * 0.18, 1e6, config=150 + n)
+    sup01 = (data["obj"] > 0).astype(np.uint8)
+    rng = np.random.default_rng(n)
+    supk = (sup01 * rng.integers(2, 256, sup01.shape)).astype(np.uint8)
+    outs = []
+    for s in (sup01, supk):
+        x = hipmod.to_device(data["x0"])
+        with hipmod.HipImpl(n, n, 1, P) as h:
+            h.set_tuning("rows_sparse=1;")
+            h.hio_run(x, hipmod.to_device(data["amp"]), hipmod.to_device(s), iters=1, beta=0.9,
+                      real_positive=1, dc_skip_zero=1, sw_every=0, use_graph=0)
+            h.Sync()
+        outs.append(x.numpy())
+    assert np.array_equal(outs[0], outs[1])
+    ref = oracle.hio_step(data["x0"], data["amp"], supk, 0.9, True, True)
+    keep = ~boundary_pixels(oracle, data["x0"], dict(data, support=supk), 1)
+    assert relerr(outs[1][keep], ref[keep]) < 1e-5
