@@ -61,12 +61,18 @@ def log(msg):
 def measured_traffic(kernel: str, tuning: dict):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known traffic), taken
-    with the same column variant, patterns per launch and row mode as `tuning`
+    with the same column variant, patterns per launch (the chunk within a sub-batch)
+    and row mode as `tuning`
     (on the initial support of the workload, as the event-timed iterations)."""
     path = os.path.join(ROOT, "profiles", "hio_512x256sq_pmc.md")
     if not os.path.exists(path):
         return None
-    want = ";".join(f"{k}={tuning.get(k)}" for k in ("col_256", "chunk", "split", "rows_sparse_used"))
+    # patterns per launch: the chunk within a sub-batch of the split
+    try:
+        ppl = min(int(tuning.get("chunk")), int(tuning.get("batch", 512)) // max(int(tuning.get("split", 1)), 1))
+    except (TypeError, ValueError):
+        return None
+    want = f"col_256={tuning.get('col_256')};ppl={ppl};rows_sparse_used={tuning.get('rows_sparse_used')}"
     for line in open(path):
         parts = line.split()
         if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel

@@ -1862,6 +1862,11 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
                 }
             }
             h->tuning[xkey] = best_split;
+            // a chunk as large as a sub-batch is the whole sub-batch
+            if (best_chunk > 0 && (uint64_t)best_chunk >= h->batch / (uint64_t)best_split) {
+                best_chunk = 0;
+                h->tuning[ckey] = 0;
+            }
         }
         h->timing = saved_timing;
         CPRA_CHECK_HIP(hipMemcpyAsync(support, bs, (size_t)total, hipMemcpyDeviceToDevice, st));
