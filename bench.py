@@ -50,7 +50,7 @@ METRIC = "phase-retrieval iters/s (512×256² batch, 128³ carousel); % HBM peak
 # algorithmic bytes per pixel of each fused kernel (complex64 = 8 B,
 # amplitude float = 4 B, support u8 = 1 B)
 COL_BYTES_PER_PX = 8 + 4 + 8        # read Y, read A, write Z
-ROW_BYTES_PER_PX = 8 + 8 + 1 + 8    # read Z, read Y, read support, write Y
+ROW_BYTES_PER_PX = 8 + 8 + 0.125 + 8  # read Z, read Y, read the support (one bit per pixel), write Y
 HIO_ALGO_BYTES_PER_PX = 21          # x read + write, A, support (one iteration)
 
 
