@@ -54,12 +54,17 @@ __device__ __forceinline__ unsigned mask16_bits(uint4 q)
 // threads per 32 rows: 16 threads per row, each taking 16-byte chunks; two
 // neighbouring chunks make one packed word; the 32 rows' bits are gathered in
 // the block and stored as one word (no atomics, no clearing pass).  Block 0
-// also clears the natural-order flag.
+// also clears the natural-order flag and, if given, the nkeys per-pattern
+// maximum keys of the next shrinkwrap update (which its column blur raises
+// with atomicMax).
 __global__ __launch_bounds__(512) void k_pack_support(const uint8_t* __restrict__ mask, int64_t nrows, int N,
                                                       unsigned* __restrict__ sbits,
                                                       unsigned* __restrict__ rowbits,
-                                                      unsigned* __restrict__ flag)
+                                                      unsigned* __restrict__ flag,
+                                                      unsigned* __restrict__ keys, int64_t nkeys)
 {
+    if (blockIdx.x == 0 && keys)
+        for (int64_t i = threadIdx.x; i < nkeys; i += blockDim.x) keys[i] = 0u;
     __shared__ unsigned wbits[8];
     const int lane16 = threadIdx.x & 15;
     const int rl = threadIdx.x >> 4;  // row of the word, 0..31
@@ -573,9 +578,11 @@ static void launch_sw_colblur(hipStream_t st, const Geo& g, const float* r1, flo
 
 // Shrinkwrap on the state (from_state) or on a real-space x; writes the mask
 // and, if blur_out != NULL, copies the blurred image there.
+// keys_clear: the per-pattern maximum keys are already 0 (the HIO run's
+// k_pack_support clears them after every update's threshold).
 static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, bool from_state,
                               uint8_t* mask, float sigma, float frac, float* blur_out,
-                              bool tiled = false)
+                              bool tiled = false, bool keys_clear = false)
 {
     const Geo g = geo_of(h);
     const int64_t px = (int64_t)g.M * g.N;
@@ -592,7 +599,7 @@ static void shrinkwrap_launch(cpra_hip_ctx* h, hipStream_t st, const void* src, 
     const float gr = (float)(k / ((double)g.M * g.M));
     const float gc = (float)(k / ((double)g.N * g.N));
     launch_sw_rowblur(st, g, from_state, tiled, src, r1, gc);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(g.B, 256)), dim3(256), 0, st, keys, 0u, g.B);
+    if (!keys_clear) hipLaunchKernelGGL(k_fill_u32, dim3(grid_1d(g.B, 256)), dim3(256), 0, st, keys, 0u, g.B);
     launch_sw_colblur(st, g, r1, bl, keys, gr);
     if (px % 16 == 0 && reinterpret_cast<uintptr_t>(bl) % 16 == 0 && reinterpret_cast<uintptr_t>(mask) % 16 == 0)
         hipLaunchKernelGGL(k_sw_thresh16, dim3(grid_1d(total / 16, 256, 65536)), dim3(256), 0, st,
@@ -668,8 +675,12 @@ static void support_pack_launch(cpra_hip_ctx* h, hipStream_t st, const uint8_t* 
     const Geo g = geo_of(h);
     unsigned* bits = row_bits(h);
     const int64_t nrows = g.B * g.M;
+    // the shrinkwrap keys are cleared here for the next update when the
+    // handle holds them (an HIO run), so that the update needs no fill pass
+    unsigned* keys = h->ws_keys.bytes >= sizeof(unsigned) * (size_t)g.B ? static_cast<unsigned*>(h->ws_keys.ptr)
+                                                                           : nullptr;
     hipLaunchKernelGGL(k_pack_support, dim3((unsigned)row_words(h)), dim3(512), 0, st, mask, nrows, g.N,
-                       support_bits(h), bits, row_order_flag(h));
+                       support_bits(h), bits, row_order_flag(h), keys, (int64_t)g.B);
     check_launch("k_pack_support");
 }
 
@@ -1303,7 +1314,8 @@ static int hio_run_batch(cpra_hip_ctx* h, void* x, const float* amp, uint64_t am
                 hipStream_t save = h->stream;
                 h->stream = gst;
                 kt.begin(h, gst, KT_SW);
-                shrinkwrap_launch(h, gst, Y, true, support, sigma, p->sw_frac, nullptr, tl);
+                // the keys were cleared by the last k_pack_support (decide_sparse)
+                shrinkwrap_launch(h, gst, Y, true, support, sigma, p->sw_frac, nullptr, tl, true);
                 kt.end(h, gst);
                 // the new support's rows, and the row mode for the next chunk
                 // (after the last update only the exit follows)
