@@ -273,11 +273,13 @@ def kernel_entry(ms, cnt, iters, bytes_per_iter, launches_per_iter=1):
             "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
-def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_iters=10, fp64=True):
+def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_iters=10, fp64=True,
+                   tuning=""):
     """3D carousel: 180 projections of 128^2 (12 random ellipsoids in 128^3,
     Poisson noise at 1e7 photons), 50 outer iterations of [1 HIO iteration on
     every projection, merge, 3D constraint, re-projection]; graph-captured
-    outer body.  Returns a result record (value = outer iterations / s)."""
+    outer body.  `tuning`: a fixed tuning string for the run (profiler A/B
+    runs).  Returns a result record (value = outer iterations / s)."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -286,6 +288,8 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_ite
     dev = {k: hip.to_device(d[k]) for k in ("x0", "amp", "support2d", "angles", "support3d")}
     vol = hip.zeros((n, n, n), np.complex64)
     h = hip.HipImpl(n, n, 1, P)
+    if tuning:
+        h.set_tuning(tuning)
     run = lambda k, g=1: h.carousel3d_run(dev["x0"], dev["amp"], dev["support2d"], dev["angles"],
                                           dev["support3d"], vol, outer_iters=k, inner_iters=1,
                                           beta=0.9, use_graph=g)
@@ -876,7 +880,8 @@ def main():
               "gd3d": bench_gd3d,
               "mixed": bench_mixed, "conv3d": bench_conv3d,
               "ref_benchmark": bench_ref_benchmark}[args.workload]
-        rec = fn(hip, synth, ge, fp64=fp64)
+        extra = {"tuning": args.tuning} if args.workload == "carousel3d" and args.tuning else {}
+        rec = fn(hip, synth, ge, fp64=fp64, **extra)
         rec = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                "dtype": "c64", "data": "synthetic", **rec}
         print(json.dumps(rec), flush=True)
