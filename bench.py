@@ -326,29 +326,28 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_ite
     h.set_kernel_timing(False)
     px2, vx = P * n * n, n ** 3
     K = timing_iters
+    planes = n <= 128 and not (tuning and "plane=0" in tuning)
     spec = {  # kind -> algorithmic bytes per outer iteration, launches per iteration
-        # the first iteration's column pass reads the row-spectral state
-        # (20 B/px); the others start from the extraction's spectra and write
-        # both the state and Z (28 B/px: read F, A; write Y, Z)
-        "column": (px2 * (COL_BYTES_PER_PX + 28 * (K - 1)) / K, 1),
+        "column": (px2 * COL_BYTES_PER_PX, 1),
         "row": (px2 * ROW_BYTES_PER_PX, 1),
-        # K forward column DFTs and one column IDFT at the end of the run
-        "proj_fft": (px2 * 16 * (K + 1) / K, (K + 1) / K),
-        "merge": (px2 * 8 + vx * 8, 1),                # gather r (slices already [P][v][u]), volume w
-        # n <= 128: the inverse whole-plane kernel and the z-line kernel (the
-        # forward plane kernel runs fused with the re-projection, except in
-        # the run's last iteration, which writes the volume); above, five
-        # passes
-        # (the 3D support is read as bits, packed once per run)
-        "volume": ((2 * vx * 16 + vx / 8 + vx * 16 / K, 2 + 1 / K) if n <= 128 else (5 * vx * 16 + vx / 8, 5)),
-        # read the volume, write the slices: n <= 128 the forward plane
-        # transform in LDS and the re-projection from there (the last
-        # iteration: the extraction from the volume); above, the extraction
-        "extract": (px2 * 8 + vx * 8, 1),
+        # the row-spectral projections into the merge's layout [P][v][u]
+        "proj_transpose": (px2 * 16, 1),
+        "merge": (px2 * 8 + vx * 8, 1),                # gather r (slices [P][v][u]), volume w
     }
+    if planes:
+        # per z-plane: IDFT_xy, the constraint (support as bits), DFT_xy and
+        # the plane's rows of every slice: read the volume, write the slices
+        spec["volume"] = (vx * 8 + vx / 8 + px2 * 8, 1)
+    else:
+        # x and y passes around the constraint kernel (support as bytes),
+        # then the re-projection
+        spec["volume"] = (4 * vx * 16 + vx * 17, 5)
+        spec["extract"] = (px2 * 8 + vx * 8, 1)
     kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}
     dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
-    algo = px2 * HIO_ALGO_BYTES_PER_PX + (px2 * 8 + vx * 8) * 2 + vx * 16 + vx // 8
+    # one HIO iteration, the merge (slices r, volume w) and the re-projection
+    # (volume r, slices w), the constraint's support as bits
+    algo = px2 * HIO_ALGO_BYTES_PER_PX + (px2 * 8 + vx * 8) * 2 + vx // 8
     rec = {
         "metric": "carousel outer iters/s (180 x 128^2 projections, 128^3 volume)",
         "value": round(its, 2), "unit": "iters/s", "steps": outer, "warmup": warmup,

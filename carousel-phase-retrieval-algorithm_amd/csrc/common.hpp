@@ -109,7 +109,7 @@ struct cpra_hip_ctx {
     std::map<uint64_t, void*> twiddles;
     // engine workspaces
     cpra::Buffer ws_z, ws_blur, ws_keys, ws_part, ws_part2, ws_misc, ws_backup, ws_rows, ws_dm, ws_trig, ws_mcount, ws_mlist, ws_mheavy;
-    cpra::Buffer ws_zsup;  // the carousel's 3D support packed for the z-line kernel
+    cpra::Buffer ws_zsup;  // the carousel's 3D support packed one bit per voxel
     // tile-major HIO state and amplitudes (engine2d.hip)
     cpra::Buffer ws_state, ws_amp_t;
     // one sub-batch's support while the HIO run goes sub-batch by sub-batch
@@ -202,8 +202,6 @@ void fft_pass_strided(cpra_hip_ctx* h, void* data, int n, int64_t lines, int64_t
 // with dir -1, shift on, no scale) written line-major to out[line * n + u].
 // Returns false, launching nothing, for sizes it has no kernel for (n other
 // than 64, 128, 192, 256, 384, 512, or lines / G not whole tiles).
-bool fft_pass_strided_lines_fwd(cpra_hip_ctx* h, const void* data, void* out, int n, int64_t lines,
-                                int64_t es, int64_t G, int64_t GS);
 bool fft_fast_size(int n);
 // Event pair around one engine kernel when the handle's timing is on; the
 // elapsed time is read back after the run (kernel_timer_collect).
@@ -244,7 +242,7 @@ struct KernelTimer {
 
 // kinds of cpra_hip_get_kernel_times
 enum KernelKind {
-    KT_COL = 0, KT_ROW = 1, KT_SW = 2, KT_GD = 3, KT_MERGE = 4, KT_VOL = 5, KT_EXTRACT = 6, KT_PFFT = 7,
+    KT_COL = 0, KT_ROW = 1, KT_SW = 2, KT_GD = 3, KT_MERGE = 4, KT_VOL = 5, KT_EXTRACT = 6, KT_PTRANS = 7,
     KT_MIXED = 8, KT_PENCIL = 9
 };
 
@@ -256,24 +254,21 @@ bool engine_tiled_state(const cpra_hip_ctx* h);
 // whole-plane x/y transforms in the carousel's volume constraint (tuning plane=)
 bool engine_plane_fft(const cpra_hip_ctx* h);
 bool engine_merge_lines(const cpra_hip_ctx* h);
-bool engine_plane_extract(const cpra_hip_ctx* h);
+int engine_plane_groups(const cpra_hip_ctx* h);
 void engine_retile(hipStream_t st, int n, int64_t patterns, int dir, const void* in, void* out);
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out);
 // engine2d.hip: one HIO iteration on the row-spectral state (see there)
 void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support);
-void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
+bool engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
                           int64_t amp_batch, const uint8_t* support, const cpra_hip_hio_params* p,
-                          KernelTimer* kt);
-bool engine_hio_dense_whole(const cpra_hip_ctx* h);
-void engine_hio_rows(cpra_hip_ctx* h, hipStream_t st, void* Y, const uint8_t* support,
-                     const cpra_hip_hio_params* p, KernelTimer* kt);
+                          KernelTimer* kt, void* lines_out = nullptr);
 // merge3d.hip: complex64 gather merge R^T(s)/R^T(1) and re-projection R
 void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P);
 void extract_c64(cpra_hip_ctx* h, const void* vol, void* slices, const float* angles, int n, int P);
 // merge through a gather list built once per geometry (synchronises)
 void merge_plan_c64(cpra_hip_ctx* h, const float* angles, int n, int P);
-void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P);
 void* merge_lines_ws(cpra_hip_ctx* h, int n, int P);
+void merge_transpose_c64(cpra_hip_ctx* h, const void* slices, void* sT, int n, int P);
 void merge_apply_lines_c64(cpra_hip_ctx* h, const void* sT, void* vol, int n, int P);
 
 // frees the captured engine graphs of a handle (engine2d.hip)

@@ -372,7 +372,7 @@ static void launch_cols(cpra_hip_ctx* h, hipStream_t st, const Geo& g, int mode,
 template <int N, int PV>
 static void launch_rows_hio_np(hipStream_t st, const Geo& g, const void* Z, void* Y,
                                const unsigned* mask, float beta, int rp, int er,
-                               const unsigned* rowbits, const unsigned* order, bool tiled)
+                               const unsigned* rowbits, const unsigned* order, bool tiled, void* lt)
 {
     const int64_t lines = g.B * g.M;
     constexpr int LPB = HioRowPlan<N, PV>::LPB;
@@ -382,6 +382,19 @@ static void launch_rows_hio_np(hipStream_t st, const Geo& g, const void* Z, void
     const float invN = 1.0f / (float)N;
     const dim3 blk(HioRowPlan<N, PV>::TPB);
     const bool exact = lines % LPB == 0;
+    if (lt) {
+        if constexpr (kRowsLinesFit<N, PV> && N % LPB == 0) {
+            auto* o = static_cast<c2<float>*>(lt);
+            if (er)
+                hipLaunchKernelGGL((k_eng_rows_hio<N, ROW_ER, true, false, false, PV, true>), dim3(grid), blk, 0, st,
+                                   z, y, mask, lines, beta, invN, rp, nullptr, nullptr, o);
+            else
+                hipLaunchKernelGGL((k_eng_rows_hio<N, ROW_HIO, true, false, false, PV, true>), dim3(grid), blk, 0,
+                                   st, z, y, mask, lines, beta, invN, rp, nullptr, nullptr, o);
+            return;
+        }
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "line-major row store: not available for n=%d", N);
+    }
     if (tiled) {
         if (g.M != N || N % LPB != 0) fail(CPRA_HIP_ERR_UNSUPPORTED, "tile-major rows: square patterns only");
 #define CPRA_ROWST(MODE, SP) \
@@ -410,15 +423,30 @@ static void launch_rows_hio_np(hipStream_t st, const Geo& g, const void* Z, void
 template <int N>
 static void launch_rows_hio_n(hipStream_t st, const Geo& g, const void* Z, void* Y,
                               const unsigned* mask, float beta, int rp, int er,
-                              const unsigned* rowbits, const unsigned* order, bool tiled, int pv)
+                              const unsigned* rowbits, const unsigned* order, bool tiled, int pv, void* lt)
 {
     if constexpr (HasAltRowPlan<N>::value) {
         if (pv == 1) {
-            launch_rows_hio_np<N, 1>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled);
+            launch_rows_hio_np<N, 1>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, lt);
             return;
         }
     }
-    launch_rows_hio_np<N, 0>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled);
+    launch_rows_hio_np<N, 0>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, lt);
+}
+
+// Whether the row kernel can store line-major (lt) for width n and plan pv:
+// dense row-major rows of whole patterns, whose block stages its rows in LDS.
+static bool rows_lines_supported(int n, int pv)
+{
+    switch (n) {
+    case 64: return kRowsLinesFit<64, 0>;
+    case 128: return kRowsLinesFit<128, 0>;
+    case 192: return kRowsLinesFit<192, 0>;
+    case 256: return pv == 1 ? kRowsLinesFit<256, 1> : kRowsLinesFit<256, 0>;
+    case 384: return kRowsLinesFit<384, 0>;
+    case 512: return pv == 1 ? kRowsLinesFit<512, 1> : kRowsLinesFit<512, 0>;
+    }
+    return false;
 }
 
 // Tuning key 8: the HIO row kernel's radix-plan variant (HioRowPlan) for
@@ -435,18 +463,22 @@ static bool has_alt_row_plan(int n)
 }
 
 // mask: the support packed by k_pack_support (N/32 words per row)
+// lt: store the rows line-major there instead of into Y (rows_lines_supported,
+// dense rows of row-major patterns)
 static void launch_rows_hio(hipStream_t st, const Geo& g, const void* Z, void* Y,
                             const unsigned* mask, float beta, int rp, int er,
                             const unsigned* rowbits = nullptr, const unsigned* order = nullptr,
-                            bool tiled = false, int pv = 0)
+                            bool tiled = false, int pv = 0, void* lt = nullptr)
 {
+    if (lt && (tiled || rowbits || g.M != g.N))
+        fail(CPRA_HIP_ERR_UNSUPPORTED, "line-major row store: dense rows of square row-major patterns only");
     switch (g.N) {
-    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
-    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
-    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
-    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
-    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
-    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv); break;
+    case 64: launch_rows_hio_n<64>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv, lt); break;
+    case 128: launch_rows_hio_n<128>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv, lt); break;
+    case 192: launch_rows_hio_n<192>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv, lt); break;
+    case 256: launch_rows_hio_n<256>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv, lt); break;
+    case 384: launch_rows_hio_n<384>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv, lt); break;
+    case 512: launch_rows_hio_n<512>(st, g, Z, Y, mask, beta, rp, er, rowbits, order, tiled, pv, lt); break;
     }
     check_launch("k_eng_rows_hio");
 }
@@ -747,21 +779,21 @@ bool engine_plane_fft(const cpra_hip_ctx* h)
     return it == h->tuning.end() || it->second != 0;
 }
 
-// The carousel's forward column DFT written straight into the merge's layout
-// (every carousel size), tuning key 6 (1 = on, the default; 0 = the DFT in
-// place and a transpose before the merge).
+// The carousel's last HIO row pass of an outer iteration stores the state
+// straight into the merge's line-major layout where it can (tuning key 6:
+// 1, the default); 0 stores it row-major and transposes it before the merge.
 bool engine_merge_lines(const cpra_hip_ctx* h)
 {
     auto it = h->tuning.find((uint64_t)6 << 32);
     return it == h->tuning.end() || it->second != 0;
 }
 
-// Tuning key 7: the carousel's forward plane transform and re-projection in
-// one kernel (1, the default) or two.
-bool engine_plane_extract(const cpra_hip_ctx* h)
+// Tuning key 7: angle groups per plane of the carousel's plane kernel (0,
+// the default: enough to fill the GPU where the angles allow).
+int engine_plane_groups(const cpra_hip_ctx* h)
 {
     auto it = h->tuning.find((uint64_t)7 << 32);
-    return it == h->tuning.end() || it->second != 0;
+    return it == h->tuning.end() ? 0 : it->second;
 }
 
 void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in, float* out)
@@ -771,9 +803,12 @@ void engine_amp_retile(hipStream_t st, int n, int64_t patterns, const float* in,
 
 // mask: the support, whose packed copy (support_bits, written by
 // decide_sparse) the row kernels read.
-static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
+// lt: where the row pass may store the updated state line-major instead of
+// into x (one launch per pass, dense rows, row-major state); returns whether
+// it did.
+static bool hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float* amp,
                           int64_t amp_batch, const uint8_t* mask, const cpra_hip_hio_params* p,
-                          bool sparse, KernelTimer* kt = nullptr, bool tiled = false)
+                          bool sparse, KernelTimer* kt = nullptr, bool tiled = false, void* lt = nullptr)
 {
     (void)mask;
     const Geo g = geo_of(h);
@@ -781,6 +816,8 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
     const unsigned* bits = sparse ? row_bits(h) : nullptr;
     const int64_t chunk = iteration_chunk(h);
     const size_t px = (size_t)g.M * g.N;
+    if (lt && (sparse || tiled || chunk != g.B || g.M != g.N || !rows_lines_supported(g.N, row_plan(h, g.N))))
+        lt = nullptr;
     for (int64_t p0 = 0; p0 < g.B; p0 += chunk) {
         const Geo gc{g.M, g.N, (g.B - p0) < chunk ? (g.B - p0) : chunk};
         void* xc = static_cast<c2<float>*>(x) + p0 * px;
@@ -793,9 +830,10 @@ static void hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* x, const float*
         if (kt) kt->end(h, st);
         if (kt) kt->begin(h, st, KT_ROW);
         launch_rows_hio(st, gc, h->ws_z.ptr, xc, mc, p->beta, p->real_positive, p->error_reduction,
-                        bc, order, tiled, row_plan(h, g.N));
+                        bc, order, tiled, row_plan(h, g.N), lt);
         if (kt) kt->end(h, st);
     }
+    return lt != nullptr;
 }
 
 // One HIO iteration on the row-spectral state Y of the handle's batch, for
@@ -815,29 +853,15 @@ void engine_hio_prepare(cpra_hip_ctx* h, hipStream_t st, const uint8_t* support)
         h->hio_sparse = rows_sparse_used(h) == 1;
 }
 
-void engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
+// lines_out: where the iteration's row pass may store the updated state in
+// the line-major layout lines_out[p][v][u] = Y[p][u][v] instead of into Y
+// (dense rows in one launch per pass); returns whether it did (else Y holds
+// the state as usual).
+bool engine_hio_iteration(cpra_hip_ctx* h, hipStream_t st, void* Y, const float* amp,
                           int64_t amp_batch, const uint8_t* support, const cpra_hip_hio_params* p,
-                          KernelTimer* kt)
+                          KernelTimer* kt, void* lines_out)
 {
-    hio_iteration(h, st, Y, amp, amp_batch, support, p, h->hio_sparse, kt);
-}
-
-// Whether the carousel's iterations run dense rows in one launch per pass
-// (then a caller may run its own column pass into ws_z and engine_hio_rows).
-bool engine_hio_dense_whole(const cpra_hip_ctx* h)
-{
-    return !h->hio_sparse && iteration_chunk(h) == (int64_t)h->batch;
-}
-
-// The row pass of one dense HIO iteration: Y updated from Y and Z = ws_z.
-void engine_hio_rows(cpra_hip_ctx* h, hipStream_t st, void* Y, const uint8_t* support,
-                     const cpra_hip_hio_params* p, KernelTimer* kt)
-{
-    const Geo g = geo_of(h);
-    if (kt) kt->begin(h, st, KT_ROW);
-    launch_rows_hio(st, g, h->ws_z.ptr, Y, support_bits(h), p->beta, p->real_positive, p->error_reduction,
-                    nullptr, nullptr, false, row_plan(h, g.N));
-    if (kt) kt->end(h, st);
+    return hio_iteration(h, st, Y, amp, amp_batch, support, p, h->hio_sparse, kt, false, lines_out);
 }
 
 template <int N>
@@ -1899,9 +1923,9 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
     });
 }
 
-// Applies the col_<M>=, row_plan_<N>=, rows_sparse=, chunk=, split=, tiled=, plane=,
-// merge_lines= and plane_extract= entries of a spec such as an autotune report;
-// other keys are skipped.
+// Applies the col_<M>=, row_plan_<N>=, rows_sparse=, chunk=, split=, tiled=,
+// plane=, merge_lines= and plane_groups= entries of a spec such as an autotune
+// report; other keys are skipped.
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
 {
     return eguard(h, [&]() -> int {
@@ -1930,8 +1954,10 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec)
             } else if (std::sscanf(p, "merge_lines=%d;%n", &v, &used) == 1 && used > 0) {
                 h->tuning[(uint64_t)6 << 32] = v ? 1 : 0;
                 p += used;
-            } else if (std::sscanf(p, "plane_extract=%d;%n", &v, &used) == 1 && used > 0) {
-                h->tuning[(uint64_t)7 << 32] = v ? 1 : 0;
+            } else if (std::sscanf(p, "plane_groups=%d;%n", &v, &used) == 1 && used > 0) {
+                CPRA_ARG(v == 0 || v == 1 || v == 2 || v == 4 || v == 8,
+                         "cpra_hip_set_tuning: plane_groups must be 0, 1, 2, 4 or 8");
+                h->tuning[(uint64_t)7 << 32] = v;
                 p += used;
             } else if (std::sscanf(p, "row_plan_%d=%d;%n", &m, &v, &used) == 2 && used > 0) {
                 CPRA_ARG(v == 0 || (v == 1 && has_alt_row_plan(m)),
@@ -1966,11 +1992,11 @@ int cpra_hip_get_tuning(cpra_hip_handle h, char* out, uint64_t out_len)
         CPRA_ARG(h && out && out_len > 0, "cpra_hip_get_tuning: bad argument");
         const ColVariant cv = col_variant(h, (int)h->m);
         std::snprintf(out, out_len,
-                      "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%lld;split=%d;tiled=%d;plane=%d;merge_lines=%d;"
-                      "plane_extract=%d;rows_sparse_used=%d;graph_captures=%lld;",
+                      "col_%d=%d;row_plan_%d=%d;rows_sparse=%d;chunk=%lld;split=%d;tiled=%d;plane=%d;"
+                      "merge_lines=%d;plane_groups=%d;rows_sparse_used=%d;graph_captures=%lld;",
                       (int)h->m, col_variant_code(cv), (int)h->n, row_plan(h, (int)h->n), sparse_mode(h),
                       (long long)iteration_chunk(h), hio_split(h), tiled_state(h) ? 1 : 0, engine_plane_fft(h) ? 1 : 0,
-                      engine_merge_lines(h) ? 1 : 0, engine_plane_extract(h) ? 1 : 0, rows_sparse_used(h),
+                      engine_merge_lines(h) ? 1 : 0, engine_plane_groups(h), rows_sparse_used(h),
                       (long long)h->graph_captures);
         return CPRA_HIP_OK;
     });

@@ -366,13 +366,21 @@ __device__ __forceinline__ void load_support_words(unsigned (&sw)[N / 32], const
     }
 }
 
+// Whether the HIO row kernel's block can stage its rows as [N][LPB + 1] in
+// its LDS for the line-major store (LT).
+template <int N, int PV> constexpr bool kRowsLinesFit =
+    HioRowPlan<N, PV>::LPB * LdsRow<c2<float>, N>::span() >= N * (HioRowPlan<N, PV>::LPB + 1);
+
 // sbits: the support packed by k_pack_support, N/32 words per line (line-
-// major, whatever the state's layout).
-template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false, int PV = 0>
+// major, whatever the state's layout).  LT: dense row-major patterns whose
+// updated rows are stored line-major into lt (lt[p][v][u] = Y[p][u][v], the
+// carousel merge's layout) instead of Y.
+template <int N, int MODE, bool EXACT, bool SPARSE, bool TILED = false, int PV = 0, bool LT = false>
 __global__ __launch_bounds__((HioRowPlan<N, PV>::TPB)) void k_eng_rows_hio(
     const c2<float>* __restrict__ Z, c2<float>* __restrict__ Y, const unsigned* __restrict__ sbits,
     int64_t nlines, float beta, float invN, int real_positive,
-    const unsigned* __restrict__ rowbits, const unsigned* __restrict__ natural_order = nullptr)
+    const unsigned* __restrict__ rowbits, const unsigned* __restrict__ natural_order = nullptr,
+    c2<float>* __restrict__ lt = nullptr)
 {
     using V = c2<float>;
     using PL = typename HioRowPlan<N, PV>::P;
@@ -467,7 +475,28 @@ __global__ __launch_bounds__((HioRowPlan<N, PV>::TPB)) void k_eng_rows_hio(
         xo[m] = r;
     });
     fft_line<PL, -1>(xo, t, tw, acc);
-    if (valid) {
+    if constexpr (LT) {
+        // the block's rows r0 .. r0 + LPB - 1 of pattern pb staged through LDS
+        // as [v][LPB + 1], then each v's LPB values stored as one contiguous
+        // run of lt[pb][v][r0 ..]
+        constexpr int TPB = HioRowPlan<N, PV>::TPB, LS = LPB + 1;
+        static_assert(!SPARSE && !TILED && EXACT && N % LPB == 0 && kRowsLinesFit<N, PV> && (N * LPB) % TPB == 0,
+                      "line-major store");
+        __syncthreads();  // every line's exchanges done before the LDS is reused
+        sfor<E>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            lds[(t + PL::rot(m) * TT) * LS + ll] = xo[m];
+        });
+        __syncthreads();
+        const size_t row0 = (size_t)vb * LPB;
+        c2<float>* ob = lt + (row0 / N) * N * N + row0 % N;
+        sfor<N * LPB / TPB>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const int i = tid + k * TPB;
+            const int v = i / LPB, ul = i % LPB;
+            ob[(size_t)v * N + ul] = lds[v * LS + ul];
+        });
+    } else if (valid) {
         sfor<E>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             if constexpr (TILED) {

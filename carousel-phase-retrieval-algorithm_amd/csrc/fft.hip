@@ -80,46 +80,6 @@ __global__ __launch_bounds__(W * SizeCfg<N>::TPB) void k_fft_strided(
     });
 }
 
-// The same column transform, complex64, with its output written line-major:
-// out[line * N + u] is point u of line `line`.  Over slices [P][u][v] that is
-// [P][v][u], the layout of the carousel's gather merge, so the merge needs no
-// transpose of its own.  The block's lines are consecutive, so its output is
-// one contiguous run of C*N values: the transformed tile goes through LDS
-// (rows padded by one element) and is stored with consecutive lanes on
-// consecutive addresses.  The values are those of k_fft_strided.
-template <int N, int DIR, bool SHIFT>
-__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_fft_strided_lines(
-    const c2<float>* __restrict__ data, c2<float>* __restrict__ out, int64_t es, int64_t G, int64_t GS)
-{
-    using V = c2<float>;
-    using PL = typename SizeCfg<N>::P;
-    constexpr int E = PL::E, C = SizeCfg<N>::C, SP = N + 1, TPB = SizeCfg<N>::TPB;
-    __shared__ V lds[SP * C];
-    const int tid = threadIdx.x;
-    const int c = tid % C, t = tid / C;
-    const int64_t line0 = (int64_t)blockIdx.x * C;
-    const V* blk = data + (line0 / G) * GS + (line0 % G);
-    const size_t loff = (size_t)c + (size_t)t * (size_t)es;
-    V v[E];
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        constexpr int pm = SHIFT ? PL::rot(m) : m;
-        v[m] = (blk + (size_t)(pm * PL::T) * (size_t)es)[loff];
-    });
-    LdsCol<V, C> acc{lds + c};
-    fft_line<PL, DIR>(v, t, g_tw<N, float>(), acc);
-    __syncthreads();
-    sfor<E>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        constexpr int pm = SHIFT ? PL::rot(m) : m;
-        lds[c * SP + pm * PL::T + t] = v[m];
-    });
-    __syncthreads();
-    V* o = out + line0 * N;
-#pragma unroll 4
-    for (int i = tid; i < C * N; i += TPB) o[i] = lds[(i / N) * SP + i % N];
-}
-
 // ---------------------------------------------------------------------------
 // generic kernel: any n whose prime factors are <= 64, one line per block,
 // whole line in LDS, runtime radix sequence (fallback for non-fast sizes)
@@ -351,41 +311,6 @@ static void strided_t(cpra_hip_ctx* h, void* data, int n, int64_t lines, int64_t
     case 384: launch_strided<384, T>(h, data, lines, es, G, GS, dir, shift, scale); return;
     case 512: launch_strided<512, T>(h, data, lines, es, G, GS, dir, shift, scale); return;
     }
-}
-
-template <int N>
-static void launch_strided_lines(cpra_hip_ctx* h, const void* data, void* out, int64_t lines, int64_t es,
-                                 int64_t G, int64_t GS)
-{
-    hipLaunchKernelGGL((k_fft_strided_lines<N, -1, true>), dim3((unsigned)(lines / SizeCfg<N>::C)),
-                       dim3(SizeCfg<N>::TPB), 0, h->stream, static_cast<const c2<float>*>(data),
-                       static_cast<c2<float>*>(out), es, G, GS);
-    check_launch("k_fft_strided_lines");
-}
-
-bool fft_pass_strided_lines_fwd(cpra_hip_ctx* h, const void* data, void* out, int n, int64_t lines,
-                                int64_t es, int64_t G, int64_t GS)
-{
-    int C = 0;
-    switch (n) {
-    case 64: C = SizeCfg<64>::C; break;
-    case 128: C = SizeCfg<128>::C; break;
-    case 192: C = SizeCfg<192>::C; break;
-    case 256: C = SizeCfg<256>::C; break;
-    case 384: C = SizeCfg<384>::C; break;
-    case 512: C = SizeCfg<512>::C; break;
-    default: return false;
-    }
-    if (lines <= 0 || G % C != 0 || lines % C != 0 || lines / C > 0x7fffffff) return false;
-    switch (n) {
-    case 64: launch_strided_lines<64>(h, data, out, lines, es, G, GS); break;
-    case 128: launch_strided_lines<128>(h, data, out, lines, es, G, GS); break;
-    case 192: launch_strided_lines<192>(h, data, out, lines, es, G, GS); break;
-    case 256: launch_strided_lines<256>(h, data, out, lines, es, G, GS); break;
-    case 384: launch_strided_lines<384>(h, data, out, lines, es, G, GS); break;
-    case 512: launch_strided_lines<512>(h, data, out, lines, es, G, GS); break;
-    }
-    return true;
 }
 
 void fft_pass_contig(cpra_hip_ctx* h, void* data, int n, int64_t lines, int dir, bool shift,

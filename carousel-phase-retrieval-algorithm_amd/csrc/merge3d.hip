@@ -738,18 +738,15 @@ void merge_apply_lines_c64(cpra_hip_ctx* h, const void* sT_, void* vol, int n, i
     check_launch("k_merge_apply");
 }
 
-// R^T(s)/R^T(1) of complex64 slices [P][u][v] through the list of
-// merge_plan_c64 (transposed into the merge's layout first).
-void merge_apply_c64(cpra_hip_ctx* h, const void* slices, void* vol, int n, int P)
+// complex64 slices [P][u][v] into the merge's layout sT [P][v][u]
+void merge_transpose_c64(cpra_hip_ctx* h, const void* slices, void* sT, int n, int P)
 {
-    CPRA_ARG(h->mplan_n == n && h->mplan_P == P, "carousel merge: no gather list for this geometry");
     using E = c2<float>;
-    E* sT = static_cast<E*>(merge_lines_ws(h, n, P));
     hipLaunchKernelGGL((k_transpose_slices<E>), dim3((n + 31) / 32, (n + 31) / 32, P), dim3(256), 0,
-                       h->stream, static_cast<const E*>(slices), sT, n);
+                       h->stream, static_cast<const E*>(slices), static_cast<E*>(sT), n);
     check_launch("k_transpose_slices");
-    merge_apply_lines_c64(h, sT, vol, n, P);
 }
+
 
 // complex64 merge / extraction on the handle's stream (the carousel driver)
 void merge_c64(cpra_hip_ctx* h, const void* slices, void* vol, const float* angles, int n, int P)

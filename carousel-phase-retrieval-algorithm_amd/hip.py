@@ -530,7 +530,7 @@ class HipImpl:
 
     def kernel_times(self):
         """{kind: (total_ms, launches)} per fused-kernel kind."""
-        names = ("column", "row", "shrinkwrap", "gd", "merge", "volume", "extract", "proj_fft",
+        names = ("column", "row", "shrinkwrap", "gd", "merge", "volume", "extract", "proj_transpose",
                  "mixed_iteration", "pencil")
         ms = (C.c_double * len(names))()
         cnt = (C.c_int64 * len(names))()

@@ -104,15 +104,17 @@ def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth, outer):
     assert np.array_equal(vg, ve)
 
 
-@pytest.mark.parametrize("n,P", [(64, 36), (128, 60)])
-def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P):
-    """For n <= 128 the volume constraint's x and y transforms run as one
-    whole-plane LDS kernel per direction (tuning plane=1, the default); plane=0
-    runs the one-axis passes.  Each line goes through the same plan and
+@pytest.mark.parametrize("n,P,use_graph", [(64, 36, 0), (128, 60, 0), (128, 180, 1), (128, 7, 1)])
+def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P, use_graph):
+    """For n <= 128 each z-plane's inverse transform, 3D constraint, forward
+    transform and share of the re-projection run as one LDS kernel, with
+    one to four angle groups per plane (tuning plane=1, the default);
+    plane=0 runs one-axis passes around a constraint kernel and the
+    re-projection on its own.  Each line goes through the same plan and
     twiddles, so the results agree to rounding after several iterations."""
     d = synth.make_carousel(n, P, config=53)
-    x1, v1, _ = run_gpu(hipmod, d, 3, use_graph=0, tuning="plane=1;")
-    x0, v0, _ = run_gpu(hipmod, d, 3, use_graph=0, tuning="plane=0;")
+    x1, v1, _ = run_gpu(hipmod, d, 3, use_graph=use_graph, tuning="plane=1;")
+    x0, v0, _ = run_gpu(hipmod, d, 3, use_graph=use_graph, tuning="plane=0;")
     ex, ev = relerr(x1, x0), relerr(v1, v0)
     print(f"plane vs passes n={n}: x {ex:.2e} vol {ev:.2e} bitwise={np.array_equal(x1, x0)}")
     assert ex < 1e-5 and ev < 1e-5
@@ -122,12 +124,13 @@ def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P):
         assert h.get_tuning()["plane"] == "0"
 
 
-@pytest.mark.parametrize("n,P", [(64, 36), (128, 60), (192, 16), (256, 16), (384, 8), (512, 8)])
+@pytest.mark.parametrize("n,P", [(64, 36), (128, 60), (192, 16), (256, 16), (384, 8), (512, 4)])
 def test_carousel_merge_lines_match_transpose(hipmod, gpu, synth, n, P):
-    """The forward column DFT writes the merge's [P][v][u] layout directly
-    (tuning merge_lines=1, the default); merge_lines=0 transforms in place and
-    transposes before the merge.  The same plan, twiddles and summation order
-    run in both, so the results are bit-identical."""
+    """The last HIO row pass of each outer iteration stores the projections
+    straight into the merge's [P][v][u] layout where its block can stage them
+    (tuning merge_lines=1, the default; n <= 384); merge_lines=0 stores them
+    row-major and transposes them before the merge.  The same values are
+    stored either way, so the results are bit-identical."""
     d = synth.make_carousel(n, P, config=54)
     x1, v1, _ = run_gpu(hipmod, d, 2, use_graph=0, tuning="merge_lines=1;")
     x0, v0, _ = run_gpu(hipmod, d, 2, use_graph=0, tuning="merge_lines=0;")
@@ -139,6 +142,25 @@ def test_carousel_merge_lines_match_transpose(hipmod, gpu, synth, n, P):
         assert h.get_tuning()["merge_lines"] == "1"
         assert h.set_tuning("merge_lines=0;")
         assert h.get_tuning()["merge_lines"] == "0"
+
+
+@pytest.mark.parametrize("n,P", [(64, 36), (128, 180), (128, 7)])
+def test_carousel_plane_groups_agree_bitwise(hipmod, gpu, synth, n, P):
+    """The plane kernel's angle groups (tuning plane_groups; 0 picks enough
+    to fill the GPU) each transform the plane and sample their angles' rows
+    with the same arithmetic, so every group count gives the same bits."""
+    d = synth.make_carousel(n, P, config=57)
+    ref = run_gpu(hipmod, d, 3, use_graph=1, tuning="plane_groups=1;")
+    for g in (0, 2, 4):
+        got = run_gpu(hipmod, d, 3, use_graph=1, tuning=f"plane_groups={g};")
+        assert np.array_equal(got[0], ref[0]), g
+        assert np.array_equal(got[1], ref[1]), g
+    with hipmod.HipImpl(n, n, 1, P) as h:
+        assert h.get_tuning()["plane_groups"] == "0"
+        assert h.set_tuning("plane_groups=2;")
+        assert h.get_tuning()["plane_groups"] == "2"
+        with pytest.raises(hipmod.CpraHipError):
+            h.set_tuning("plane_groups=3;")
 
 
 def test_carousel_rfactor_falls(hipmod, gpu, synth):
@@ -176,21 +198,3 @@ def test_carousel_unsupported(hipmod, gpu):
     with hipmod.HipImpl(n, n, 1, 4) as h:
         with pytest.raises(hipmod.CpraHipError):
             h.carousel3d_run(z, a, s2, ang, s3, z, outer_iters=1)   # vol aliases x
-
-
-@pytest.mark.parametrize("n,P", [(64, 36), (128, 180), (128, 7)])
-def test_carousel_fused_plane_extract_matches_two_kernels(hipmod, gpu, synth, n, P):
-    """For n <= 128 the volume's forward plane transform and the
-    re-projection run as one kernel (tuning plane_extract=1, the default),
-    each angle group transforming its plane itself; plane_extract=0 runs the
-    plane kernel and the plane-staged extraction.  The same transforms and
-    the same sampling arithmetic: identical bits, the volume included."""
-    d = synth.make_carousel(n, P, config=57)
-    x1, v1, _ = run_gpu(hipmod, d, 3, use_graph=1, tuning="plane_extract=1;")
-    x0, v0, _ = run_gpu(hipmod, d, 3, use_graph=1, tuning="plane_extract=0;")
-    assert np.array_equal(x1, x0)
-    assert np.array_equal(v1, v0)
-    with hipmod.HipImpl(n, n, 1, P) as h:
-        assert h.get_tuning()["plane_extract"] == "1"
-        assert h.set_tuning("plane_extract=0;")
-        assert h.get_tuning()["plane_extract"] == "0"
