@@ -343,7 +343,10 @@ def bench_carousel(hip, synth, ge, n=128, P=180, outer=50, warmup=15, timing_ite
         # then the re-projection
         spec["volume"] = (4 * vx * 16 + vx * 17, 5)
         spec["extract"] = (px2 * 8 + vx * 8, 1)
-    kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()}
+    # (the transpose runs only where the row pass cannot store the merge's
+    # layout itself)
+    kern = {k: kernel_entry(kt[k][0], kt[k][1], timing_iters, b, l) for k, (b, l) in spec.items()
+            if kt[k][1] > 0}
     dom = max(kern, key=lambda k: kern[k]["ms_per_iteration"])
     # one HIO iteration, the merge (slices r, volume w) and the re-projection
     # (volume r, slices w), the constraint's support as bits

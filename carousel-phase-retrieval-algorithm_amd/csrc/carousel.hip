@@ -154,11 +154,26 @@ __device__ __forceinline__ void plane2d_axes(c2<float>* lds)
     });
 }
 
-// The constraint on the NAT plane z in lds: (Re(w) * scale)^+ inside the
-// support, 0 elsewhere; sw: the plane's support words (N/32 per row,
-// k_pack_plane_support).  Each thread keeps one x (TPB is a multiple of N).
+// This thread's support words of plane z for plane_project: element (y, x)
+// with x = tid % N and y = tid / N + k * (TPB / N), k < K (TPB is a multiple
+// of N); sw: the plane's words (N/32 per row, k_pack_plane_support).  Loaded
+// at the kernel's start, so their latency overlaps the plane's load.
 template <int N>
-__device__ __forceinline__ void plane_project(c2<float>* lds, const unsigned* __restrict__ sw, float scale)
+__device__ __forceinline__ void plane_support_words(const unsigned* __restrict__ sw,
+                                                    unsigned (&w)[PlaneGeom<N>::K])
+{
+    using G = PlaneGeom<N>;
+    const int x = (int)threadIdx.x % N, y0 = (int)threadIdx.x / N;
+    sfor<G::K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        w[k] = sw[(y0 + k * (G::TPB / N)) * (N / 32) + (x >> 5)];
+    });
+}
+
+// The constraint on the NAT plane in lds: (Re(w) * scale)^+ inside the
+// support, 0 elsewhere; w: plane_support_words.
+template <int N>
+__device__ __forceinline__ void plane_project(c2<float>* lds, const unsigned (&w)[PlaneGeom<N>::K], float scale)
 {
     using G = PlaneGeom<N>;
     const int x = (int)threadIdx.x % N, y0 = (int)threadIdx.x / N;
@@ -168,7 +183,7 @@ __device__ __forceinline__ void plane_project(c2<float>* lds, const unsigned* __
         const int y = y0 + k * (G::TPB / N);
         c2<float>& a = lds[y * G::SP + G::pad(x)];
         const float re = a.x * scale;
-        a = mk<c2<float>>((sw[y * (N / 32) + (x >> 5)] & bit) && re > 0.f ? re : 0.f, 0.f);
+        a = mk<c2<float>>((w[k] & bit) && re > 0.f ? re : 0.f, 0.f);
     });
     __syncthreads();
 }
@@ -190,7 +205,11 @@ __device__ __forceinline__ void plane2d_store(c2<float>* Pp, const c2<float>* ld
 // DFT_xy in LDS, and samples row u = z of its angles' slices from there with
 // the bilinear weights and FP64 geometry of k_extract_plane.  The groups put
 // the CUs that one block per plane leaves idle to work on the sampling; each
-// group transforms the plane itself.  V is only read.
+// group transforms the plane itself.  Block b runs on XCD b mod 8, and the
+// groups of a plane are blocks 8 apart, so they share that XCD's L2 and the
+// plane is fetched from beyond it about once (its load phase measured 4.6
+// against 7.7 us at 128^2 with the groups on neighbouring XCDs).  V is only
+// read.  The grid is N * groups blocks (N a multiple of 8).
 template <int N>
 __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane_constraint_extract(
     const c2<float>* __restrict__ V, const unsigned* __restrict__ sbits, float scale,
@@ -200,12 +219,16 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane_constraint_extra
     using G = PlaneGeom<N>;
     __shared__ Vt lds[N * G::SP];
     __shared__ double tl[2 * kPlaneExtractMaxAngles];
-    const int u = (int)blockIdx.x / groups, g = (int)blockIdx.x % groups;
+    static_assert(N % 8 == 0, "planes dealt to the 8 XCDs");
+    const int xcd = (int)blockIdx.x & 7, i8 = (int)blockIdx.x >> 3;
+    const int u = (i8 / groups) * 8 + xcd, g = i8 % groups;
     const int p0 = (int)((int64_t)g * P / groups), p1 = (int)((int64_t)(g + 1) * P / groups);
     for (int i = threadIdx.x; i < 2 * (p1 - p0); i += blockDim.x) tl[i] = trig[2 * p0 + i];
+    unsigned sw[G::K];
+    plane_support_words<N>(sbits + (size_t)u * N * (N / 32), sw);
     plane2d_load<N>(V + (size_t)u * (N * N), lds);
     plane2d_axes<N, +1>(lds);
-    plane_project<N>(lds, sbits + (size_t)u * N * (N / 32), scale);
+    plane_project<N>(lds, sw, scale);
     plane2d_axes<N, -1>(lds);  // ends with a barrier
     const double c = (double)(N / 2);
     for (int i = threadIdx.x; i < (p1 - p0) * N; i += blockDim.x) {
@@ -242,9 +265,11 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane_constraint(c2<fl
 {
     __shared__ c2<float> lds[N * PlaneGeom<N>::SP];
     c2<float>* Pp = V + (size_t)blockIdx.x * (N * N);
+    unsigned sw[PlaneGeom<N>::K];
+    plane_support_words<N>(sbits + (size_t)blockIdx.x * N * (N / 32), sw);
     plane2d_load<N>(Pp, lds);  // every load of the plane done before the stores below
     plane2d_axes<N, +1>(lds);
-    plane_project<N>(lds, sbits + (size_t)blockIdx.x * N * (N / 32), scale);
+    plane_project<N>(lds, sw, scale);
     plane2d_store<N>(Pp, lds);
 }
 
