@@ -144,14 +144,15 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * through the whole schedule in s sub-batches of batch/s patterns, one after
  * another, with the same results; applies when s divides the batch and the
  * amplitudes are per pattern or one for all, outside a capture; 1 = whole
- * batch), "plane=<0|1>;" (whole-plane LDS transforms
- * in the carousel's volume constraint for n <= 128; 0 = one pass per axis),
- * "merge_lines=<0|1>;" (the carousel's forward column DFT written
- * straight into the merge's [P][v][u] layout; 0 = the DFT in place and a
- * transpose before the merge) and "plane_extract=<0|1>;" (n <= 128: the
- * volume's forward plane transform and the re-projection in one kernel; 0 =
- * two) and "row_plan_<n>=<0|1>;" (the HIO row kernel's radix plan for
- * width n; 1 exists for n = 256 and 512). */
+ * batch), "plane=<0|1>;" (n <= 128: the carousel's 3D constraint and
+ * re-projection as one kernel per z-plane in LDS; 0 = one pass per axis,
+ * a constraint kernel and the re-projection on its own), "merge_lines=<0|1>;"
+ * (the carousel's last HIO row pass of an outer iteration stores the
+ * projections straight into the merge's [P][v][u] layout where it can;
+ * 0 = row-major and a transpose before the merge), "plane_groups=<g>;"
+ * (angle groups per plane of the carousel's plane kernel, 1, 2, 4 or 8;
+ * 0 = enough to fill the GPU, the default) and "row_plan_<n>=<0|1>;" (the
+ * HIO row kernel's radix plan for width n; 1 exists for n = 256 and 512). */
 int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
 /* The autotuner's persistent cache for the whole process: a JSON object
  * mapping "<gcn arch>/hio2d/<m>x<n>/b<batch>/a<amp_batch>" to the tuning
@@ -160,7 +161,7 @@ int cpra_hip_set_tuning(cpra_hip_handle h, const char* spec);
  * entry. */
 int cpra_hip_set_tuning_cache(const char* path, int refresh);
 /* Current choices as "col_<m>=<v>;rows_sparse=<0|1|2>;chunk=<patterns>;
- * split=<s>;tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_extract=<0|1>;row_plan_<n>=<0|1>;
+ * split=<s>;tiled=<0|1>;plane=<0|1>;merge_lines=<0|1>;plane_groups=<g>;row_plan_<n>=<0|1>;
  * rows_sparse_used=<0|1>;
  * graph_captures=<k>;" (rows_sparse_used: the row mode of the last iterations
  * run; in the adaptive mode it is read from the device, which synchronises
