@@ -235,19 +235,6 @@ static int col_tile_rt(int M)
     return 0;
 }
 
-static int row_lpb_rt(int N)
-{
-    switch (N) {
-    case 64: return SizeCfg<64>::LPB;
-    case 128: return SizeCfg<128>::LPB;
-    case 192: return SizeCfg<192>::LPB;
-    case 256: return SizeCfg<256>::LPB;
-    case 384: return SizeCfg<384>::LPB;
-    case 512: return SizeCfg<512>::LPB;
-    }
-    return 0;
-}
-
 void engine_check_geometry(cpra_hip_ctx* h, const char* who)
 {
     if (h->dtype != CPRA_HIP_F32)
