@@ -101,7 +101,7 @@ constexpr int kPlaneExtractMaxAngles = 256;
 template <int N> struct PlaneGeom {
     using PL = typename SizeCfg<N>::P;
     static constexpr int E = PL::E, TT = PL::T, TPB = N * TT, SP = N + N / 16, K = N * N / TPB;
-    static_assert(64 % TT == 0 && TPB % 64 == 0 && N % 32 == 0 && TPB % N == 0, "plane layout");
+    static_assert(64 % TT == 0 && TPB % 64 == 0 && N % 32 == 0 && TPB % N == 0 && K == E, "plane layout");
     static __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 };
 
@@ -123,67 +123,88 @@ __device__ __forceinline__ void plane2d_load(const c2<float>* Pp, c2<float>* lds
     __syncthreads();
 }
 
-// The NAT plane in lds transformed along x, then y (sign DIR, unnormalised);
-// the result is left in NAT layout.
-template <int N, int DIR>
-__device__ __forceinline__ void plane2d_axes(c2<float>* lds)
+// One axis step on the plane in lds: every line of the current layout (the
+// rows of lds: x-lines in NAT, y-lines in TRN) transformed with sign DIR,
+// unnormalised, in registers with its exchanges in the line's own span; then
+// written transposed (TRANSPOSE: NAT <-> TRN) or back in place.
+template <int N, int DIR, bool TRANSPOSE>
+__device__ __forceinline__ void plane2d_step(c2<float>* lds)
 {
     using Vt = c2<float>;
     using G = PlaneGeom<N>;
     using PL = typename G::PL;
     constexpr int E = G::E, TT = G::TT, SP = G::SP;
-    const int tid = threadIdx.x;
     auto pad = [](int i) { return G::pad(i); };
-    const int t = tid % TT, line = tid / TT;
+    const int t = (int)threadIdx.x % TT, line = (int)threadIdx.x / TT;
     Vt* lb = lds + line * SP;
-    // NAT -> (x) -> TRN, then TRN -> (y) -> NAT
-    sfor<2>([&](auto) {
-        Vt v[E];
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            v[m] = lb[pad(t + PL::rot(m) * TT)];
-        });
-        LdsRow<Vt, N, true> acc{lb};
-        fft_line<PL, DIR>(v, t, g_tw<N>(), acc);
-        __syncthreads();  // every exchange done before the transposed writes
-        sfor<E>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            lds[(t + PL::rot(m) * TT) * SP + pad(line)] = v[m];
-        });
-        __syncthreads();
+    Vt v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = lb[pad(t + PL::rot(m) * TT)];
     });
+    LdsRow<Vt, N, true> acc{lb};
+    fft_line<PL, DIR>(v, t, g_tw<N>(), acc);
+    if constexpr (TRANSPOSE) __syncthreads();  // every exchange done before the transposed writes
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int pos = t + PL::rot(m) * TT;
+        if constexpr (TRANSPOSE) lds[pos * SP + pad(line)] = v[m];
+        else lb[pad(pos)] = v[m];  // the line's own span: its exchanges are this wave's
+    });
+    __syncthreads();
 }
 
-// This thread's support words of plane z for plane_project: element (y, x)
-// with x = tid % N and y = tid / N + k * (TPB / N), k < K (TPB is a multiple
-// of N); sw: the plane's words (N/32 per row, k_pack_plane_support).  Loaded
-// at the kernel's start, so their latency overlaps the plane's load.
+// This thread's support words of plane z for plane2d_ystep: element m of
+// y-line x = tid / T sits at y = tid % T + rot(m) * T, and its bit is bit
+// x % 32 of word (y, x / 32) of the plane's words (N/32 per row,
+// k_pack_plane_support).  Loaded at the kernel's start, so their latency
+// overlaps the plane's load.
 template <int N>
 __device__ __forceinline__ void plane_support_words(const unsigned* __restrict__ sw,
-                                                    unsigned (&w)[PlaneGeom<N>::K])
+                                                    unsigned (&w)[PlaneGeom<N>::E])
 {
     using G = PlaneGeom<N>;
-    const int x = (int)threadIdx.x % N, y0 = (int)threadIdx.x / N;
-    sfor<G::K>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        w[k] = sw[(y0 + k * (G::TPB / N)) * (N / 32) + (x >> 5)];
+    using PL = typename G::PL;
+    const int t = (int)threadIdx.x % G::TT, x = (int)threadIdx.x / G::TT;
+    sfor<G::E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        w[m] = sw[(t + PL::rot(m) * G::TT) * (N / 32) + (x >> 5)];
     });
 }
 
-// The constraint on the NAT plane in lds: (Re(w) * scale)^+ inside the
-// support, 0 elsewhere; w: plane_support_words.
-template <int N>
-__device__ __forceinline__ void plane_project(c2<float>* lds, const unsigned (&w)[PlaneGeom<N>::K], float scale)
+// The y-lines (TRN rows) of the x-transformed plane: IDFT_y, the constraint
+// (Re(w) * scale)^+ inside the support and 0 elsewhere (Im 0), and with FWD
+// DFT_y, all in registers, one line per T threads; written transposed (NAT).
+// Taking the constraint between the two y transforms saves the plane a round
+// trip through LDS and a pass of its own.  w: plane_support_words.
+template <int N, bool FWD>
+__device__ __forceinline__ void plane2d_ystep(c2<float>* lds, const unsigned (&w)[PlaneGeom<N>::E], float scale)
 {
+    using Vt = c2<float>;
     using G = PlaneGeom<N>;
-    const int x = (int)threadIdx.x % N, y0 = (int)threadIdx.x / N;
-    const unsigned bit = 1u << (x & 31);
-    sfor<G::K>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        const int y = y0 + k * (G::TPB / N);
-        c2<float>& a = lds[y * G::SP + G::pad(x)];
-        const float re = a.x * scale;
-        a = mk<c2<float>>((w[k] & bit) && re > 0.f ? re : 0.f, 0.f);
+    using PL = typename G::PL;
+    constexpr int E = G::E, TT = G::TT, SP = G::SP;
+    auto pad = [](int i) { return G::pad(i); };
+    const int t = (int)threadIdx.x % TT, line = (int)threadIdx.x / TT;  // line = x
+    Vt* lb = lds + line * SP;
+    Vt v[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = lb[pad(t + PL::rot(m) * TT)];
+    });
+    LdsRow<Vt, N, true> acc{lb};
+    fft_line<PL, +1>(v, t, g_tw<N>(), acc);
+    const unsigned bit = 1u << (line & 31);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float re = v[m].x * scale;
+        v[m] = mk<Vt>((w[m] & bit) && re > 0.f ? re : 0.f, 0.f);
+    });
+    if constexpr (FWD) fft_line<PL, -1>(v, t, g_tw<N>(), acc);
+    __syncthreads();  // every exchange done before the transposed writes
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        lds[(t + PL::rot(m) * TT) * SP + pad(line)] = v[m];
     });
     __syncthreads();
 }
@@ -201,8 +222,9 @@ __device__ __forceinline__ void plane2d_store(c2<float>* Pp, const c2<float>* ld
 
 // The 3D constraint of plane z and its share of the re-projection: block
 // (z, angle group g of `groups`) loads plane z of the merged volume (spectral
-// in x and y, real in z), runs IDFT_xy, the constraint (scale 1/n^2) and
-// DFT_xy in LDS, and samples row u = z of its angles' slices from there with
+// in x and y, real in z), runs IDFT_x, then IDFT_y, the constraint (scale
+// 1/n^2) and DFT_y on each y-line in registers, then DFT_x, all in LDS, and
+// samples row u = z of its angles' slices from there with
 // the bilinear weights and FP64 geometry of k_extract_plane.  The groups put
 // the CUs that one block per plane leaves idle to work on the sampling; each
 // group transforms the plane itself.  Block b runs on XCD b mod 8, and the
@@ -224,12 +246,12 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane_constraint_extra
     const int u = (i8 / groups) * 8 + xcd, g = i8 % groups;
     const int p0 = (int)((int64_t)g * P / groups), p1 = (int)((int64_t)(g + 1) * P / groups);
     for (int i = threadIdx.x; i < 2 * (p1 - p0); i += blockDim.x) tl[i] = trig[2 * p0 + i];
-    unsigned sw[G::K];
+    unsigned sw[G::E];
     plane_support_words<N>(sbits + (size_t)u * N * (N / 32), sw);
     plane2d_load<N>(V + (size_t)u * (N * N), lds);
-    plane2d_axes<N, +1>(lds);
-    plane_project<N>(lds, sw, scale);
-    plane2d_axes<N, -1>(lds);  // ends with a barrier
+    plane2d_step<N, +1, true>(lds);  // IDFT_x: NAT -> TRN
+    plane2d_ystep<N, true>(lds, sw, scale);  // IDFT_y, constraint, DFT_y: TRN -> NAT
+    plane2d_step<N, -1, false>(lds);  // DFT_x in place; ends with a barrier
     const double c = (double)(N / 2);
     for (int i = threadIdx.x; i < (p1 - p0) * N; i += blockDim.x) {
         const int pl = i / N, v = i % N;
@@ -265,11 +287,11 @@ __global__ __launch_bounds__(N * SizeCfg<N>::P::T) void k_plane_constraint(c2<fl
 {
     __shared__ c2<float> lds[N * PlaneGeom<N>::SP];
     c2<float>* Pp = V + (size_t)blockIdx.x * (N * N);
-    unsigned sw[PlaneGeom<N>::K];
+    unsigned sw[PlaneGeom<N>::E];
     plane_support_words<N>(sbits + (size_t)blockIdx.x * N * (N / 32), sw);
     plane2d_load<N>(Pp, lds);  // every load of the plane done before the stores below
-    plane2d_axes<N, +1>(lds);
-    plane_project<N>(lds, sw, scale);
+    plane2d_step<N, +1, true>(lds);
+    plane2d_ystep<N, false>(lds, sw, scale);
     plane2d_store<N>(Pp, lds);
 }
 
