@@ -32,9 +32,10 @@
 // (IDFT_xy, DFT_xy unnormalised; the row-spectral state carries no scale).
 // Steps 2-6 are a transpose of Y into the merge's layout, the merge, and for
 // n <= 128 one kernel per z-plane: IDFT_xy, the constraint and DFT_xy in LDS,
-// then the plane's rows of every slice sampled from there.  Above 128 the
-// plane transforms are one-axis passes around a constraint kernel, and the
-// re-projection is a kernel of its own.  The outer body is captured in a
+// then the plane's rows of every slice sampled from there.  Above 128 they
+// are an x pass, one y-line kernel (IDFT_y, the constraint and DFT_y in
+// registers) and an x pass back, and the re-projection is a kernel of its
+// own.  The outer body is captured in a
 // hipGraph, ten outer iterations per graph, which the handle keeps across
 // calls and replays; its working set (the projections, the transposed slices
 // and the volume: 56 MB at 180 x 128^2 and 128^3) stays in the Infinity Cache.
@@ -64,17 +65,74 @@ __global__ __launch_bounds__(256) void k_pack_plane_support(const uint8_t* __res
     }
 }
 
-// The constraint in place over the volume [total]: v = (Re(w) * scale)^+
-// where S is nonzero, 0 elsewhere (Im v = 0).  The path with one-axis passes.
-__global__ __launch_bounds__(256) void k_project_real_pos(c2<float>* __restrict__ V,
-                                                         const uint8_t* __restrict__ S, int64_t total,
-                                                         float scale)
+// The y-lines of the x-transformed volume [N][N][N], for the sizes whose
+// planes do not fit in LDS (and tuning plane=0): IDFT_y, the constraint
+// (Re(w) * scale)^+ inside the support and 0 elsewhere (Im 0), and with FWD
+// DFT_y, per line in registers, in place.  Block (z, tile): C consecutive x
+// of plane z in the strided-line mapping of the column kernels (T threads per
+// line, element m at row t + rot(m) * T).  sbits: k_pack_plane_support's
+// words.  One pass for what the inverse y pass, a constraint pass and the
+// forward y pass did.
+template <int N, bool FWD>
+__global__ __launch_bounds__(SizeCfg<N>::TPB) void k_ystep_constraint(c2<float>* V,
+                                                                     const unsigned* __restrict__ sbits,
+                                                                     float scale)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const float re = V[i].x * scale;
-        V[i] = mk<c2<float>>(S[i] != 0 && re > 0.f ? re : 0.f, 0.f);
+    using Vt = c2<float>;
+    using PL = typename SizeCfg<N>::P;
+    constexpr int E = PL::E, TT = PL::T, C = SizeCfg<N>::C;
+    static_assert(N % C == 0, "whole column tiles");
+    __shared__ Vt lds[N * C];
+    const int c = (int)threadIdx.x % C, t = (int)threadIdx.x / C;
+    const int z = (int)blockIdx.x / (N / C), x = ((int)blockIdx.x % (N / C)) * C + c;
+    Vt* Vp = V + (size_t)z * N * N + x;
+    const unsigned* sw = sbits + (size_t)z * N * (N / 32) + (x >> 5);
+    const unsigned bit = 1u << (x & 31);
+    auto row = [&](int m) { return t + PL::rot(m) * TT; };
+    Vt v[E];
+    unsigned w[E];
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = Vp[(size_t)row(m) * N];
+        w[m] = sw[row(m) * (N / 32)];
+    });
+    LdsCol<Vt, C, false> acc{lds + c};
+    fft_line<PL, +1>(v, t, g_tw<N>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float re = v[m].x * scale;
+        v[m] = mk<Vt>((w[m] & bit) && re > 0.f ? re : 0.f, 0.f);
+    });
+    if constexpr (FWD) fft_line<PL, -1>(v, t, g_tw<N>(), acc);
+    sfor<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        Vp[(size_t)row(m) * N] = v[m];
+    });
+}
+
+template <int N>
+static void ystep_n(hipStream_t st, c2<float>* V, const unsigned* sbits, bool fwd)
+{
+    const dim3 grid((unsigned)(N * (N / SizeCfg<N>::C)));
+    const float scale = (float)(1.0 / ((double)N * N));
+    if (fwd)
+        hipLaunchKernelGGL((k_ystep_constraint<N, true>), grid, dim3(SizeCfg<N>::TPB), 0, st, V, sbits, scale);
+    else
+        hipLaunchKernelGGL((k_ystep_constraint<N, false>), grid, dim3(SizeCfg<N>::TPB), 0, st, V, sbits, scale);
+}
+
+static void ystep_launch(hipStream_t st, int n, c2<float>* V, const unsigned* sbits, bool fwd)
+{
+    switch (n) {
+    case 64: ystep_n<64>(st, V, sbits, fwd); break;
+    case 128: ystep_n<128>(st, V, sbits, fwd); break;
+    case 192: ystep_n<192>(st, V, sbits, fwd); break;
+    case 256: ystep_n<256>(st, V, sbits, fwd); break;
+    case 384: ystep_n<384>(st, V, sbits, fwd); break;
+    case 512: ystep_n<512>(st, V, sbits, fwd); break;
+    default: fail(CPRA_HIP_ERR_UNSUPPORTED, "carousel: no y-line kernel for n=%d", n);
     }
+    check_launch("k_ystep_constraint");
 }
 
 // Centred 2D transforms along x and y of the z-planes of the volume
@@ -368,8 +426,7 @@ struct Carousel {
     int64_t amp_batch;
     const uint8_t* sup2d;
     const float* angles;
-    const uint8_t* sup3d;    // [n][n][n] bytes
-    const unsigned* sbits;   // sup3d packed by pack_plane_support
+    const unsigned* sbits;   // the 3D support packed by pack_plane_support
     c2<float>* vol;          // [n][n][n]: spectral in (x, y), real in z
     cpra_hip_hio_params hp;
     int inner;
@@ -381,16 +438,12 @@ struct Carousel {
         return engine_plane_fft(h) && plane_fits(n) && h->mplan_n == n && h->mplan_P == P && h->ws_trig.ptr;
     }
 
-    // vol <- v = (Re IDFT_xy(vol) / n^2)^+ * S3 with one-axis passes
-    void constraint_passes()
+    // vol <- v = (Re IDFT_xy(vol) / n^2)^+ * S3, then with fwd DFT_y(v): the
+    // x pass and one y-line kernel
+    void constraint_passes(bool fwd)
     {
-        const int64_t W = (int64_t)n * n;
-        fft_pass_contig(h, vol, n, W, +1, true, 1.0, false);            // x
-        fft_pass_strided(h, vol, n, W, n, n, W, +1, true, 1.0, false);  // y
-        const int64_t total = W * n;
-        hipLaunchKernelGGL(k_project_real_pos, dim3((unsigned)grid_1d(total, 256, 8192)), dim3(256), 0, h->stream,
-                           vol, sup3d, total, (float)(1.0 / ((double)n * n)));
-        check_launch("k_project_real_pos");
+        fft_pass_contig(h, vol, n, (int64_t)n * n, +1, true, 1.0, false);  // x
+        ystep_launch(h->stream, n, vol, sbits, fwd);
     }
 
     // One outer iteration on h->stream: Y (the row-spectral state) in and
@@ -425,14 +478,13 @@ struct Carousel {
             return;
         }
         if (kt) kt->begin(h, h->stream, KT_VOL);
-        constraint_passes();
-        fft_pass_strided(h, vol, n, W, n, n, W, -1, true, 1.0, false);  // y
-        fft_pass_contig(h, vol, n, W, -1, true, 1.0, false);            // x
+        constraint_passes(true);
+        fft_pass_contig(h, vol, n, W, -1, true, 1.0, false);  // x
         if (kt) kt->end(h, h->stream);
         if (kt) kt->begin(h, h->stream, KT_EXTRACT);
         extract_c64(h, vol, Y, angles, n, P);
         if (kt) kt->end(h, h->stream);
-        if (last) constraint_passes();  // DFT_xy(v) back to v
+        if (last) constraint_passes(false);  // DFT_xy(v) back to v
     }
 };
 
@@ -491,7 +543,6 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         cr.amp_batch = (int64_t)amp_batch;
         cr.sup2d = support2d;
         cr.angles = angles;
-        cr.sup3d = support3d;
         cr.sbits = nullptr;  // packed below
         cr.vol = static_cast<c2<float>*>(vol);
         cpra_hip_hio_default_params(&cr.hp);
@@ -504,7 +555,7 @@ int cpra_hip_carousel3d_run(cpra_hip_handle h, void* x, const float* amp, uint64
         hipStream_t st = h->stream;
         engine_hio_prepare(h, st, support2d);
         if (p->outer_iters > 0) merge_plan_c64(h, angles, n, cr.P);  // the footprint, once per run
-        if (plane_fits(n)) cr.sbits = pack_plane_support(h, st, n, support3d);
+        cr.sbits = pack_plane_support(h, st, n, support3d);
         // enter the row-spectral state of the projections
         fft_pass_contig(h, x, n, (int64_t)cr.P * n, -1, true, 1.0, false);
         KernelTimer kt;

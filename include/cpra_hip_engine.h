@@ -145,8 +145,8 @@ int cpra_hip_autotune_2d(cpra_hip_handle h, void* x, const float* amp, uint64_t 
  * another, with the same results; applies when s divides the batch and the
  * amplitudes are per pattern or one for all, outside a capture; 1 = whole
  * batch), "plane=<0|1>;" (n <= 128: the carousel's 3D constraint and
- * re-projection as one kernel per z-plane in LDS; 0 = one pass per axis,
- * a constraint kernel and the re-projection on its own), "merge_lines=<0|1>;"
+ * re-projection as one kernel per z-plane in LDS; 0 = x passes around a
+ * y-line constraint kernel, and the re-projection on its own), "merge_lines=<0|1>;"
  * (the carousel's last HIO row pass of an outer iteration stores the
  * projections straight into the merge's [P][v][u] layout where it can;
  * 0 = row-major and a transpose before the merge), "plane_groups=<g>;"
