@@ -510,7 +510,7 @@ def bench_gd3d(hip, synth, ge, n=256, steps=500, warmup=10, timing_iters=20, fp6
 
 
 def bench_conv3d(hip, synth, ge, n=512, iters=20, warmup=3, timing_iters=3, prim_iters=3,
-                 fp64=True, check_n=128):
+                 fp64=True, check_n=128, tuning=""):
     """The reference's conventional 3D reconstruction at its benchmark scale
     (src/CONV_MKL.cpp:47-63 at M = N = L = 512, src/benchmark_mkl.cpp:7-13):
     difference-map iterations (t1, t2 carried) on one 512^3 complex64 volume,
@@ -528,6 +528,8 @@ def bench_conv3d(hip, synth, ge, n=512, iters=20, warmup=3, timing_iters=3, prim
     S = hip.DeviceArray((1, n, n, n), f32)
     mask = hip.DeviceArray((1, n, n, n), u8)
     h = hip.HipImpl(n, n, n, 1)
+    if tuning:
+        h.set_tuning(tuning)
     for i, (arr, num) in enumerate(((psi, 2 * vox), (A, vox), (S, vox))):
         h.Initialize(arr, num, seed=5000 + i)
     h.support_from_constraint(S, mask, vox)
@@ -882,7 +884,7 @@ def main():
               "gd3d": bench_gd3d,
               "mixed": bench_mixed, "conv3d": bench_conv3d,
               "ref_benchmark": bench_ref_benchmark}[args.workload]
-        extra = {"tuning": args.tuning} if args.workload == "carousel3d" and args.tuning else {}
+        extra = {"tuning": args.tuning} if args.workload in ("carousel3d", "conv3d") and args.tuning else {}
         rec = fn(hip, synth, ge, fp64=fp64, **extra)
         rec = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                "dtype": "c64", "data": "synthetic", **rec}
