@@ -124,8 +124,8 @@ n_patterns, threads, kind, min_s = int(sys.argv[2]), int(sys.argv[3]), sys.argv[
 if not mp.available():
     print(json.dumps({"error": mp.load_error()}))
     sys.exit(0)
-if kind == "config0":
-    data = synth.config0()
+if kind == "8starts":
+    data = synth.multistart_256()
 else:
     data = synth.make_2d_batch(n_patterns, 256, 30.0, 50.0, 1e6, starts_per_pattern=1, config=1)
 sched = dict(synth.HIO_SCHEDULE)
@@ -177,7 +177,7 @@ def cpu_baseline_mkl_path(n_patterns=32):
     run (OMP_NUM_THREADS when set) and on 8 threads, the reference
     benchmark's setting.  Runs in a child process that never touches the
     GPU.  Returns None when libmkl_rt cannot be loaded."""
-    out = _mkl_path_runs(n_patterns, "config1", 10.0)
+    out = _mkl_path_runs(n_patterns, "batch", 10.0)
     if out is None or "error" in out:
         return out
     # the faster of the two thread counts is the baseline (on a shared host
@@ -630,7 +630,7 @@ def bench_conv3d(hip, synth, ge, n=512, iters=20, warmup=3, timing_iters=3, prim
     return rec
 
 
-def bench_hio_config0(hip, synth, ge, n=256, reps=5, fp64=True):
+def bench_hio_8starts(hip, synth, ge, n=256, reps=5, fp64=True):
     """The CPU-runnable configuration: 8 random starts of one 256^2 pattern
     (disc radius 40 +- 5 px, 1e6 photons), the 200-iteration HIO + shrinkwrap
     schedule (beta 0.9, every 20, sigma 3.0 -> 1.5, threshold 0.1).  GPU: the
@@ -646,7 +646,7 @@ def bench_hio_config0(hip, synth, ge, n=256, reps=5, fp64=True):
 
     import numpy as np
 
-    d = synth.config0(n)
+    d = synth.multistart_256(n)
     B = d["x0"].shape[0]
     sched = dict(synth.HIO_SCHEDULE)
     iters = sched.pop("iters")
@@ -667,7 +667,7 @@ def bench_hio_config0(hip, synth, ge, n=256, reps=5, fp64=True):
         rf = h.rfactor(x, amp)
     el = statistics.median(times)
     its = iters / el
-    log(f"bench[config0]: {iters} iterations of {B} starts in {el * 1e3:.2f} ms -> {its:.1f} it/s")
+    log(f"bench[8starts]: {iters} iterations of {B} starts in {el * 1e3:.2f} ms -> {its:.1f} it/s")
     rec = {
         "metric": "batched HIO iters/s (8 starts of one 256^2 pattern, 200-iteration shrinkwrap schedule)",
         "value": round(its, 2), "unit": "iters/s", "steps": iters, "warmup": iters,
@@ -677,7 +677,7 @@ def bench_hio_config0(hip, synth, ge, n=256, reps=5, fp64=True):
                    "graph": True, "runs": [round(iters / t, 1) for t in times]},
         "rfactor_final": [round(float(v), 6) for v in rf] if rf is not None else None,
     }
-    out = _mkl_path_runs(B, "config0", 2.0)
+    out = _mkl_path_runs(B, "8starts", 2.0)
     if out is not None and "error" not in out:
         best = max(out, key=lambda t: out[t]["pattern_iters_per_s"])
         top = out[best]
@@ -826,7 +826,7 @@ def bench_mixed(hip, synth, ge, sizes=(128, 192, 256, 384), per_class=256, iters
     return rec
 
 
-WORKLOADS = ("hio2d", "hio2d_config0", "carousel3d", "gd2d", "gd3d", "mixed", "conv3d", "ref_benchmark")
+WORKLOADS = ("hio2d", "hio2d_8starts", "carousel3d", "gd2d", "gd3d", "mixed", "conv3d", "ref_benchmark")
 
 
 def main():
@@ -880,7 +880,7 @@ def main():
         b_.free()
         log("bench: copy calibration kernel done (2^26 complex64)")
     if args.workload != "hio2d":
-        fn = {"hio2d_config0": bench_hio_config0, "carousel3d": bench_carousel, "gd2d": bench_gd2d,
+        fn = {"hio2d_8starts": bench_hio_8starts, "carousel3d": bench_carousel, "gd2d": bench_gd2d,
               "gd3d": bench_gd3d,
               "mixed": bench_mixed, "conv3d": bench_conv3d,
               "ref_benchmark": bench_ref_benchmark}[args.workload]
@@ -1082,7 +1082,7 @@ def main():
 
     if not args.no_extra:
         result["workloads"] = {}
-        for name, fn in (("hio2d_config0", bench_hio_config0), ("carousel3d", bench_carousel),
+        for name, fn in (("hio2d_8starts", bench_hio_8starts), ("carousel3d", bench_carousel),
                          ("gd2d", bench_gd2d), ("gd3d", bench_gd3d),
                          ("mixed", bench_mixed), ("conv3d", bench_conv3d),
                          ("ref_benchmark", bench_ref_benchmark)):

@@ -87,12 +87,12 @@ def make_2d_batch(n_patterns: int, n: int = 256, radius_lo: float = 35.0,
     }
 
 
-def config0(n: int = 256):
+def multistart_256(n: int = 256):
     """8 random starts of one 256x256 pattern, disc radius 40 +- 5 px."""
     return make_2d_batch(1, n, 35.0, 45.0, 1e6, starts_per_pattern=8, config=0)
 
 
-def config1(n_patterns: int = 512, n: int = 256):
+def batch_512x256(n_patterns: int = 512, n: int = 256):
     """512 patterns of 256x256, each its own object, radii U[30, 50] px."""
     return make_2d_batch(n_patterns, n, 30.0, 50.0, 1e6, starts_per_pattern=1, config=1)
 

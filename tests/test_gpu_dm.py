@@ -274,8 +274,8 @@ def _dm_reference_oracle_rfactor(oracle, psi0, A, S, iters, perturb=0.0):
 
 
 def test_dm_reference_rfactor_within_1e3_after_20_iterations(hipmod, oracle, gpu, synth):
-    """The north star's end-to-end bar, R-factor within 1e-3 relative of the
-    FP64 oracle after a fixed iteration count, for the reference's own DM
+    """The end-to-end bar, R-factor within 1e-3 relative of the FP64 oracle
+    after a fixed iteration count, for the reference's own DM
     variant at 256^2: 20 iterations plus the final projection.  (The FP64
     map itself moves R by 3e-5 at 20 iterations from a 6e-8 perturbation of
     the start, 5e-4 at 30 and 1e-2 at 200: the map is expanding, so longer

@@ -7,9 +7,11 @@ orc_converge_error and orc_final_projection on the same inputs.
 
 Also, without a GPU: HIP_Impl.hpp compiled inside the reference tree
 (CPRA_HIP_IN_REFERENCE_TREE against the reference's own include/), and the
-registration that INTEGRATION.md describes applied to a scratch copy of the
-reference's headers, compiled with Cpra<float, IMPL_TYPE::HIP> used as the
-reference's programs use it.  Both skip where the reference is absent."""
+backend's registration (IMPL_TYPE::HIP in CPRA_Impl.hpp, its case in
+Impl_Factory.hpp and in Allocator.hpp's NewAllocator) applied to a scratch
+copy of the reference's headers, compiled with Cpra<float, IMPL_TYPE::HIP>
+used as the reference's programs use it.  Both skip where the reference is
+absent."""
 import os
 import re
 import shutil
@@ -79,8 +81,9 @@ def test_hip_impl_compiles_inside_the_reference_tree(tmp_path):
 
 
 def _apply_integration_recipe(inc):
-    """The three header edits of INTEGRATION.md section 2, applied to a
-    scratch copy of the reference's include/."""
+    """The backend's registration in three reference headers (the HIP
+    enumerator, the factory case, the allocator case honouring
+    gpu_managed), applied to a scratch copy of the reference's include/."""
     p = inc / "CPRA_Impl.hpp"
     s = p.read_text()
     s2 = s.replace("enum IMPL_TYPE {CUDA, MKL};", "enum IMPL_TYPE {CUDA, MKL, HIP};")
