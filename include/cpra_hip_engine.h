@@ -256,14 +256,15 @@ int cpra_hip_dm_mixed_run(cpra_hip_handle h, const cpra_hip_size_class* classes,
  * x [P][n][n] complex64 (in/out), amp [amp_batch][n][n], support2d u8
  * [P][n][n], angles [P] float radians, support3d u8 [n][n][n] (device
  * pointers).  vol [n][n][n] complex64 receives the constrained real-space
- * object of the last iteration.  rfactor_host (NULL or P floats): R-factor of
- * the final projections. */
+ * object of the last iteration (the run also uses it as its volume workspace,
+ * so its prior contents are lost).  rfactor_host (NULL or P floats): R-factor
+ * of the final projections. */
 typedef struct {
     int32_t outer_iters;   /* outer iterations                                  */
     int32_t inner_iters;   /* HIO iterations on the projections per outer one   */
     float beta;            /* HIO feedback                                      */
     int32_t dc_skip_zero;  /* 1: A == 0 leaves the Fourier pixel unconstrained  */
-    int32_t use_graph;     /* capture one outer iteration in a hipGraph, replay */
+    int32_t use_graph;     /* capture ten outer iterations per hipGraph, replay */
 } cpra_hip_carousel_params;
 
 void cpra_hip_carousel_default_params(cpra_hip_carousel_params* p);
