@@ -667,3 +667,44 @@ def test_carousel3d_composition(oracle):
             assert np.abs(xo - xe).max() < 1e-12 * np.abs(xe).max()
             assert (w.real < 0).any() and (np.abs(w.imag) > 1e-3 * np.abs(w).max()).any()
             assert ((s3 == 0) & (w.real > 0)).any()
+
+
+def _cfft(x, axis, inverse=False):
+    """Centred DFT along one axis (index n/2 is frequency 0), unnormalised
+    forward, 1/n inverse."""
+    f = np.fft.ifft if inverse else np.fft.fft
+    return np.fft.fftshift(f(np.fft.ifftshift(x, axes=axis), axis=axis), axes=axis)
+
+
+def test_merge_and_extraction_commute_with_the_rotation_axis_transform(oracle):
+    """The identity the GPU carousel's outer iteration rests on: slice row u
+    lies in the plane z = u, and the merge and the re-projection combine
+    whole lines along the rotation axis with weights that do not depend on
+    the position along it, so they commute with the centred transform along
+    that axis:
+        IDFT_z(merge(F)) = merge(IDFT_u(F)),  extract(DFT_z(V)) = DFT_u(extract(V)).
+    First the 1D centred transforms here are pinned to the oracle's 2D and 3D
+    ones (their compositions give orc_forward2d/backward2d/backward3d).  A
+    transform along the other slice axis (v), or along the volume's y, breaks
+    both identities: the check cannot pass by construction."""
+    rng = np.random.default_rng(11)
+    n, P = 16, 7
+    ang = rng.uniform(0, np.pi, P)
+    F = rng.standard_normal((P, n, n)) + 1j * rng.standard_normal((P, n, n))
+    V = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    # the 1D transforms against the oracle's
+    assert np.allclose(oracle.forward2d(F), _cfft(_cfft(F, -1), -2), rtol=1e-12, atol=1e-9)
+    assert np.allclose(oracle.backward2d(F), _cfft(_cfft(F, -1, True), -2, True), rtol=1e-12, atol=1e-12)
+    assert np.allclose(oracle.backward3d(V), _cfft(_cfft(_cfft(V, -1, True), -2, True), 0, True),
+                       rtol=1e-12, atol=1e-12)
+    # merge: slices [P][u][v] (u along z), volume [z][y][x]
+    lhs = _cfft(oracle.slice_merge(F, ang, True), 0, True)
+    rhs = oracle.slice_merge(_cfft(F, 1, True), ang, True)
+    assert np.abs(lhs).max() > 1e-3
+    assert np.allclose(lhs, rhs, rtol=1e-12, atol=1e-12)
+    assert not np.allclose(lhs, oracle.slice_merge(_cfft(F, 2, True), ang, True), atol=1e-6)
+    # re-projection
+    lhs = oracle.slice_extract(_cfft(V, 0), ang, True)
+    rhs = _cfft(oracle.slice_extract(V, ang, True), 1)
+    assert np.allclose(lhs, rhs, rtol=1e-12, atol=1e-9)
+    assert not np.allclose(oracle.slice_extract(_cfft(V, 1), ang, True), rhs, atol=1e-6)
