@@ -40,16 +40,20 @@ def run_gpu(hipmod, d, outer, inner=1, use_graph=1, want_rfactor=False, tuning=N
     return x.numpy(), vol.numpy(), rf
 
 
-# Every outer iteration after the first starts from the extraction's spectra:
-# with dense rows its first column pass reads them directly
-# (k_car_cols_from_spectrum); rows_sparse=1 takes the column IDFT pass and the
-# engine's sparse kernels instead; inner = 0 runs no HIO step at all.
+# Every outer iteration starts from the row-spectral state that the
+# re-projection leaves; with dense rows the last HIO row pass stores the
+# merge's layout itself, rows_sparse=1 takes the engine's sparse kernels and a
+# transpose before the merge, and inner = 0 runs no HIO step at all.  Up to
+# 128 the volume constraint and re-projection run as one kernel per z-plane,
+# above as x passes around a y-line constraint kernel and a re-projection
+# kernel; 512 takes the widest merge (8 z per lane) and y-line plans.
 @pytest.mark.parametrize("n,P,outer,inner,tuning", [(64, 36, 1, 1, None), (64, 36, 1, 2, None),
                                                     (64, 24, 2, 1, None), (64, 24, 3, 2, None),
                                                     (64, 24, 3, 1, "rows_sparse=1;"),
                                                     (64, 24, 2, 0, None), (128, 180, 1, 1, None),
                                                     (128, 60, 2, 1, None), (192, 8, 1, 1, None),
-                                                    (256, 8, 2, 1, None), (384, 4, 1, 1, None)])
+                                                    (256, 8, 2, 1, None), (384, 4, 1, 1, None),
+                                                    (512, 4, 1, 1, None)])
 def test_carousel_matches_oracle(hipmod, oracle, gpu, synth, n, P, outer, inner, tuning):
     d = synth.make_carousel(n, P, config=40 + n)
     x, vol, _ = run_gpu(hipmod, d, outer, inner, use_graph=0, tuning=tuning)
@@ -68,9 +72,10 @@ def test_carousel_matches_oracle(hipmod, oracle, gpu, synth, n, P, outer, inner,
 @pytest.mark.parametrize("radius_frac,expect_sparse", [(0.5, "0"), (0.25, "1")])
 def test_carousel_row_mode_follows_support(hipmod, oracle, gpu, synth, radius_frac, expect_sparse):
     """The adaptive row mode is decided once per run from the 2D supports: the
-    full disc (every row holds support) runs the dense kernels and the fused
-    first column pass; a disc of radius n/4 (half the rows) runs sparse rows
-    and the unfused sequence.  Both match the oracle."""
+    full disc (every row holds support) runs the dense kernels, the last of
+    which stores the merge's layout; a disc of radius n/4 (half the rows)
+    runs sparse rows and a transpose before the merge.  Both match the
+    oracle."""
     n, P, outer = 64, 24, 2
     d = synth.make_carousel(n, P, config=55)
     k = np.arange(n) - n / 2
