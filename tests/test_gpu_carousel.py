@@ -98,11 +98,12 @@ def test_carousel_row_mode_follows_support(hipmod, oracle, gpu, synth, radius_fr
     assert ex < 2e-5 * outer and ev < 2e-5 * outer
 
 
-@pytest.mark.parametrize("outer", [6, 25])
-def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth, outer):
+@pytest.mark.parametrize("n,P,outer", [(64, 36, 25), (64, 36, 6), (192, 8, 14)])
+def test_carousel_graph_matches_eager_bitwise(hipmod, gpu, synth, n, P, outer):
     # 25 outer iterations: three eager, two replays of the ten-iteration
-    # body, two eager; 6: too short for a body, all eager
-    d = synth.make_carousel(64, 36, config=50)
+    # body, two eager; 6: too short for a body, all eager; 14 at 192: one
+    # replay of the x passes and the y-line constraint kernel
+    d = synth.make_carousel(n, P, config=50)
     xg, vg, _ = run_gpu(hipmod, d, outer, use_graph=1)
     xe, ve, _ = run_gpu(hipmod, d, outer, use_graph=0)
     assert np.array_equal(xg, xe)
@@ -114,7 +115,7 @@ def test_carousel_plane_kernels_match_axis_passes(hipmod, gpu, synth, n, P, use_
     """For n <= 128 each z-plane's inverse transform, 3D constraint, forward
     transform and share of the re-projection run as one LDS kernel, with
     one to four angle groups per plane (tuning plane=1, the default);
-    plane=0 runs one-axis passes around a constraint kernel and the
+    plane=0 runs x passes around the y-line constraint kernel and the
     re-projection on its own.  Each line goes through the same plan and
     twiddles, so the results agree to rounding after several iterations."""
     d = synth.make_carousel(n, P, config=53)
